@@ -1,0 +1,144 @@
+// Internal definitions shared by the apspie gfx950 kernels and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <set>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/apspie.h"
+
+namespace apspie {
+
+constexpr int kWave = 64;  // CDNA wavefront width
+
+// ------------------------------------------------------------------ errors
+struct Error {
+    int status;
+    std::string msg;
+};
+
+#define APSPIE_HIP(call)                                                        \
+    do {                                                                        \
+        hipError_t e_ = (call);                                                 \
+        if (e_ != hipSuccess) {                                                 \
+            throw ::apspie::Error{APSPIE_ERR_DEVICE,                             \
+                                  std::string(#call) + ": " + hipGetErrorString(e_)}; \
+        }                                                                       \
+    } while (0)
+
+#define APSPIE_CHECK(cond, status, msg)                   \
+    do {                                                  \
+        if (!(cond)) throw ::apspie::Error{(status), (msg)}; \
+    } while (0)
+
+#define APSPIE_LAUNCH_CHECK() APSPIE_HIP(hipGetLastError())
+
+// ------------------------------------------------------------------ context
+struct Buffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct Ctx;
+
+struct Graph {
+    Ctx *owner = nullptr;
+    int device = 0;
+    int32_t m = 0, n = 0;
+    int64_t nnz = 0;
+    int32_t *rowptr = nullptr;
+    int32_t *col = nullptr;
+    float *val = nullptr;
+    bool owns = false;
+    int64_t bytes = 0;
+    // derived, built lazily and cached with the graph
+    int32_t *lowcnt = nullptr;  // per row: number of cols < row (TC)
+    int32_t max_degree = -1;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    int64_t budget = 0;  // 0 = unlimited
+    int64_t used = 0;    // bytes held: uploaded graphs + scratch
+    std::map<std::string, Buffer> scratch;
+    std::map<std::string, int64_t> options;
+    std::string last_error;
+    apspie_stats stats{};
+    bool profiling = false;
+    int cus = 256;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // pinned host word for counters read back by the host
+    void *pinned = nullptr;
+    // graphs whose memory is accounted to this context
+    std::set<Graph *> graphs;
+};
+
+int64_t opt(Ctx *c, const char *name, int64_t dflt);
+
+// Scratch buffer by name, grown on demand (contents not preserved on growth).
+void *scratch(Ctx *c, const std::string &name, size_t bytes);
+void *dev_alloc(Ctx *c, size_t bytes);  // counted against the budget
+void dev_free(Ctx *c, void *p, size_t bytes);
+
+// Run `f` translating exceptions into status codes and recording the message.
+template <class F>
+int guard(Ctx *c, F &&f) {
+    try {
+        f();
+        return APSPIE_OK;
+    } catch (const Error &e) {
+        if (c) c->last_error = e.msg;
+        return e.status;
+    } catch (const std::bad_alloc &) {
+        if (c) c->last_error = "host allocation failed";
+        return APSPIE_ERR_OOM;
+    }
+}
+
+void set_device(Ctx *c);
+void sync(Ctx *c);
+// copy-in / copy-out helpers honouring apspie_mem
+void copy_to_device(Ctx *c, void *dst, const void *src, size_t bytes, int mem);
+void copy_from_device(Ctx *c, void *dst, const void *src, size_t bytes, int mem);
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// number of blocks to fill the chip for a kernel with `threads` per block
+int fill_grid(Ctx *c, int threads, int per_cu);
+
+// ----------------------------------------------------------- kernel entry
+// (host-side launchers defined in the .hip files)
+void bfs_run(Ctx *c, const Graph *g, int32_t source, int32_t *depth_dev);
+void sssp_run(Ctx *c, const Graph *g, int32_t source, float *dist_dev);
+void mis_run(Ctx *c, const Graph *g, const uint32_t *prio_dev, int32_t *out_dev);
+void tc_run(Ctx *c, const Graph *g, uint64_t *total, int32_t *per_row_dev);
+int64_t spmspv_run(Ctx *c, const Graph *g, int semiring, int64_t nx, const int32_t *x_ind,
+                   const float *x_val, const int32_t *mask, int32_t *y_ind, float *y_val,
+                   int path);
+Graph *spgemm_run(Ctx *c, const Graph *a, const Graph *b);  // returns new apspie_graph
+int64_t scan_exclusive_dev(Ctx *c, const int32_t *in, int32_t *out, int64_t n);
+void sort_pairs_dev(Ctx *c, uint32_t *keys, uint32_t *vals, int64_t n, int bits);
+void bfs_batch_run(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc,
+                   int32_t *depth_dev);
+void sssp_batch_run(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc,
+                    float *dist_dev);
+void graph_from_coo_dev(Ctx *c, Graph *g, const int32_t *row, const int32_t *col,
+                        const float *val, int64_t nnz, int symmetrize);
+void graph_transpose_dev(Ctx *c, const Graph *g, Graph *t);
+void mt19937_priorities(Ctx *c, uint32_t seed, int32_t m, uint32_t *out_dev);
+void fill_i32(Ctx *c, int32_t *p, int32_t v, int64_t n);
+
+}  // namespace apspie
+
+// The opaque ABI handles are the internal structs.
+struct apspie_ctx : apspie::Ctx {};
+struct apspie_graph : apspie::Graph {};

@@ -1,0 +1,213 @@
+// Sparse frontier machinery shared by the SpMSpV-based traversals.
+//
+// A frontier is the sparse vector x of one SpMSpV step, stored compacted in
+// HBM.  Only entries with out-degree > 0 are kept (others expand to nothing).
+// Per entry j:
+//   v[j]      vertex id (the nonzero's index)
+//   start[j]  rowptr[v]             -- the frontier column gather
+//   off[j]    sum of degrees of entries before j   -- exclusive scan of
+//             column lengths, produced on the fly: a block appends its
+//             entries with ONE 64-bit atomic that reserves both the entry
+//             slots (high word) and the edge range (low word), so offsets are
+//             monotone in slot order without a separate scan pass.
+//   tfirst[t] the entry that owns edge t*TILE: the merge-path partition of
+//             the expanded edge list into TILE-sized tiles, written by the
+//             producer for each tile boundary that falls inside its range.
+// The consumer (k_expand) walks tiles of TILE consecutive expanded edges;
+// every tile finds its entries from tfirst, stages their offsets in LDS and
+// maps each edge slot to its entry by binary search -- a load-balanced
+// expansion whose cost does not depend on the degree skew.
+#pragma once
+
+#include "common.hpp"
+#include "device.hpp"
+
+namespace apspie {
+
+struct FrontierBufs {
+    int32_t *v;
+    int32_t *start;
+    int32_t *off;
+    int32_t *tfirst;
+};
+
+// counter word: (entries << 32) | expanded edges
+__host__ __device__ inline uint32_t cnt_entries(uint64_t c) { return uint32_t(c >> 32); }
+__host__ __device__ inline uint32_t cnt_edges(uint64_t c) { return uint32_t(c); }
+
+constexpr uint64_t kLow40 = (uint64_t(1) << 40) - 1;
+
+// Block-wide append of up to K entries per thread.  cd[k] == 0 means "no
+// entry".  All threads of the block must call it.
+template <int BLOCK, int K, int TILE, bool STORE_V>
+__device__ __forceinline__ void produce(const int32_t (&cv)[K], const int32_t (&crs)[K],
+                                        const int32_t (&cd)[K], FrontierBufs nxt,
+                                        uint64_t *cnt_next, uint64_t *s_scan,
+                                        uint64_t *s_base) {
+    uint32_t c = 0, dsum = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        c += cd[k] > 0 ? 1u : 0u;
+        dsum += (uint32_t)cd[k];
+    }
+    uint64_t packed = (uint64_t(c) << 40) | uint64_t(dsum);
+    uint64_t total;
+    uint64_t excl = dev::block_exclusive_add64<BLOCK>(packed, s_scan, total);
+    if (total == 0) return;  // uniform across the block
+    if (threadIdx.x == 0) {
+        uint64_t add = ((total >> 40) << 32) | (total & kLow40);
+        *s_base = dev::atomic_add_dev(cnt_next, add);
+    }
+    __syncthreads();
+    uint64_t base = *s_base;
+    __syncthreads();  // s_base is reused by the next call
+    if (c == 0) return;
+    uint32_t slot = cnt_entries(base) + uint32_t(excl >> 40);
+    int64_t o = (int64_t)cnt_edges(base) + (int64_t)(excl & kLow40);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        int32_t d = cd[k];
+        if (d > 0) {
+            if (STORE_V) nxt.v[slot] = cv[k];
+            nxt.start[slot] = crs[k];
+            nxt.off[slot] = (int32_t)o;
+            // tiles whose first edge lies in [o, o + d)
+            for (int64_t t = (o + TILE - 1) / TILE; t * TILE < o + d; ++t)
+                nxt.tfirst[t] = (int32_t)slot;
+            ++slot;
+            o += d;
+        }
+    }
+}
+
+// Generic load-balanced expansion over one frontier.  Op supplies:
+//   __device__ bool edge(int32_t u, int32_t e, int32_t v)
+//       per expanded edge (u -> v, e = CSR position); returns true if v is
+//       claimed for the next frontier (each v at most once per level).
+// When PRODUCE is false nothing is appended.
+template <int BLOCK, int TILE, bool STORE_V_IN, bool STORE_V_OUT, bool PRODUCE, class Op>
+__global__ __launch_bounds__(BLOCK) void k_expand(Op op, const int32_t *__restrict__ rowptr,
+                                                  const int32_t *__restrict__ col, int32_t nnz,
+                                                  FrontierBufs cur, const uint64_t *cnt_cur,
+                                                  FrontierBufs nxt, uint64_t *cnt_next,
+                                                  uint64_t *cnt_clear, uint64_t *stats) {
+    constexpr int K = TILE / BLOCK;
+    __shared__ int32_t s_off[TILE + 2];
+    __shared__ int32_t s_start[TILE + 2];
+    __shared__ int32_t s_v[STORE_V_IN ? TILE + 2 : 1];
+    __shared__ uint64_t s_scan[BLOCK / 64 + 1];
+    __shared__ uint64_t s_base;
+
+    const uint64_t cc = *cnt_cur;
+    const int32_t F = (int32_t)cnt_entries(cc);
+    const int32_t E = (int32_t)cnt_edges(cc);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (cnt_clear) *cnt_clear = 0;
+        if (stats && F > 0) {
+            stats[0] += (uint64_t)E;  // edges traversed
+            stats[1] += (uint64_t)F;  // entries expanded
+            stats[2] += 1;            // levels with work
+        }
+    }
+    if (F == 0) return;
+    const int32_t ntiles = (int32_t)((E + TILE - 1) / TILE);
+    for (int32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        // entries owning this tile; every degree is >= 1, so at most TILE + 1.
+        // The clamps only guard memory if the frontier were inconsistent.
+        const int32_t i0 = min(max(cur.tfirst[t], 0), F - 1);
+        const int32_t i1 = (t + 1 < ntiles) ? min(max(cur.tfirst[t + 1], i0), F - 1) : F - 1;
+        const int32_t nv = min(i1 - i0 + 1, TILE + 1);
+        for (int32_t j = threadIdx.x; j < nv; j += BLOCK) {
+            s_off[j] = cur.off[i0 + j];
+            s_start[j] = cur.start[i0 + j];
+            if (STORE_V_IN) s_v[j] = cur.v[i0 + j];
+        }
+        __syncthreads();
+        const int32_t e0 = t * TILE;
+        int32_t cv[K], crs[K], cd[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            cv[k] = 0;
+            crs[k] = 0;
+            cd[k] = 0;
+            const int32_t e = e0 + threadIdx.x + k * BLOCK;
+            if (e < E) {
+                const int j = dev::upper_bound_m1(s_off, 0, nv, e);
+                const int32_t idx = s_start[j] + (e - s_off[j]);
+                if ((uint32_t)idx >= (uint32_t)nnz) continue;
+                const int32_t v = col[idx];
+                const int32_t u = STORE_V_IN ? s_v[j] : -1;
+                if (op.edge(u, idx, v) && PRODUCE) {
+                    const int32_t rs = rowptr[v];
+                    cv[k] = v;
+                    crs[k] = rs;
+                    cd[k] = rowptr[v + 1] - rs;
+                }
+            }
+        }
+        if (PRODUCE)
+            produce<BLOCK, K, TILE, STORE_V_OUT>(cv, crs, cd, nxt, cnt_next, s_scan, &s_base);
+        __syncthreads();  // s_off / s_start are overwritten by the next tile
+    }
+}
+
+// Build a frontier from a vertex predicate over [0, n) or from a vertex
+// list.  Op supplies  __device__ bool keep(int32_t v).
+template <int BLOCK, int K, int TILE, bool STORE_V, class Op>
+__global__ __launch_bounds__(BLOCK) void k_build(Op op, const int32_t *__restrict__ rowptr,
+                                                 const int32_t *list, int32_t n,
+                                                 const uint64_t *n_from_counter,
+                                                 FrontierBufs nxt, uint64_t *cnt_next) {
+    __shared__ uint64_t s_scan[BLOCK / 64 + 1];
+    __shared__ uint64_t s_base;
+    if (n_from_counter) n = (int32_t)cnt_entries(*n_from_counter);
+    const int64_t per_block = (int64_t)BLOCK * K;
+    for (int64_t b0 = (int64_t)blockIdx.x * per_block; b0 < n; b0 += (int64_t)gridDim.x * per_block) {
+        int32_t cv[K], crs[K], cd[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            cv[k] = 0;
+            crs[k] = 0;
+            cd[k] = 0;
+            const int64_t i = b0 + threadIdx.x + (int64_t)k * BLOCK;
+            if (i < n) {
+                const int32_t v = list ? list[i] : (int32_t)i;
+                if (op.keep(v)) {
+                    const int32_t rs = rowptr[v];
+                    cv[k] = v;
+                    crs[k] = rs;
+                    cd[k] = rowptr[v + 1] - rs;
+                }
+            }
+        }
+        produce<BLOCK, K, TILE, STORE_V>(cv, crs, cd, nxt, cnt_next, s_scan, &s_base);
+    }
+}
+
+// Frontier storage sized for a graph: two ping-pong frontiers, a ring of
+// three counters and a stats block.
+struct FrontierSet {
+    FrontierBufs f[2];
+    uint64_t *ctl;  // [0..2] counters, [4..11] stats
+    int64_t cap_entries = 0, cap_tiles = 0;
+};
+
+inline FrontierSet frontier_alloc(Ctx *c, const std::string &tag, int32_t m, int64_t nnz,
+                                  int tile, bool with_v) {
+    FrontierSet s;
+    const int64_t ent = std::max<int64_t>(m, 1) + 1;
+    const int64_t tiles = nnz / tile + 2;
+    for (int i = 0; i < 2; ++i) {
+        std::string p = tag + std::to_string(i);
+        s.f[i].v = with_v ? (int32_t *)scratch(c, p + "_v", sizeof(int32_t) * ent) : nullptr;
+        s.f[i].start = (int32_t *)scratch(c, p + "_s", sizeof(int32_t) * ent);
+        s.f[i].off = (int32_t *)scratch(c, p + "_o", sizeof(int32_t) * ent);
+        s.f[i].tfirst = (int32_t *)scratch(c, p + "_t", sizeof(int32_t) * tiles);
+    }
+    s.ctl = (uint64_t *)scratch(c, tag + "_ctl", sizeof(uint64_t) * 16);
+    s.cap_entries = ent;
+    s.cap_tiles = tiles;
+    return s;
+}
+
+}  // namespace apspie

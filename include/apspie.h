@@ -1,0 +1,240 @@
+/*
+ * apspie.h -- C ABI of the MI355X (gfx950) SpMSpV graph engine and hash
+ * SpGEMM.  Plain C types only; every function returns an int status
+ * (APSPIE_OK == 0, negative apspie_status on failure) and never exits.
+ *
+ * The reference (owensgroup/apspie) exposes this path as C++ templates in
+ * headers, parameterised by raw device pointers and an mgpu::CudaContext&.
+ * Each entry point below names the reference interface it replaces
+ * (file:line in the apspie tree).  A header-only C++ layer with the
+ * reference's own names and argument order sits on top (apspie.hpp).
+ *
+ * Threading: one apspie_ctx per host thread.  A context owns one HIP stream,
+ * its scratch memory and its tuning table; calls on a context are
+ * synchronous unless the name ends in _async.  Graphs are resident in HBM
+ * and may be shared by contexts on the same device.
+ */
+#ifndef APSPIE_H
+#define APSPIE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define APSPIE_ABI_VERSION 1
+
+typedef struct apspie_ctx apspie_ctx;
+typedef struct apspie_graph apspie_graph;
+typedef struct apspie_server apspie_server;
+
+typedef enum {
+    APSPIE_OK = 0,
+    APSPIE_ERR_INVALID = -1,     /* bad argument (null pointer, id out of range) */
+    APSPIE_ERR_OOM = -2,         /* device allocation failed */
+    APSPIE_ERR_DEVICE = -3,      /* HIP runtime / kernel error */
+    APSPIE_ERR_BUDGET = -4,      /* request exceeds the context's HBM budget */
+    APSPIE_ERR_OVERFLOW = -5,    /* internal capacity exceeded */
+    APSPIE_ERR_UNSUPPORTED = -6, /* combination not implemented */
+    APSPIE_ERR_BUSY = -7         /* server queue full / not ready */
+} apspie_status;
+
+/* where a caller-owned buffer lives */
+typedef enum { APSPIE_MEM_HOST = 0, APSPIE_MEM_DEVICE = 1 } apspie_mem;
+
+/* semirings of the SpMSpV pipeline */
+typedef enum {
+    APSPIE_SR_PLUS_TIMES = 0, /* (+, x)      mXv, mXv.cuh:110-247 */
+    APSPIE_SR_BOOL = 1,       /* (or, and)   spmspvBfs, spmspvBfs.cuh:96-243 */
+    APSPIE_SR_MIN_PLUS = 2,   /* (min, +)    spmspvSssp, spmspvSssp.cuh:91-291 */
+    APSPIE_SR_MAX_FIRST = 3   /* (max, x_u)  spmspvMis, spmspvMis.cuh:277-289 */
+} apspie_semiring;
+
+/* SpMSpV execution path (both give identical results on integer data) */
+typedef enum {
+    APSPIE_PATH_AUTO = 0,   /* tuner picks per call */
+    APSPIE_PATH_SORT = 1,   /* expand -> key-value radix sort -> segmented reduce */
+    APSPIE_PATH_ATOMIC = 2  /* expand -> atomic accumulate into a dense row -> compact */
+} apspie_path;
+
+/* Unreached SSSP distance, the reference's sentinel (sssp.cuh:48-49). */
+#define APSPIE_SSSP_INF 1.70141e+38f
+
+/* CSR graph / matrix.  32-bit row offsets and column indices, rows sorted.
+ * val may be NULL (pattern only).  Mirrors d_matrix (matrix.hpp:6-37) in its
+ * CSR use (h_/d_cscColPtr, RowInd, Val fed with (row, col), matrix.cpp:55-111). */
+typedef struct {
+    int32_t m, n;
+    int64_t nnz;
+    const int32_t *rowptr; /* m+1 */
+    const int32_t *col;    /* nnz */
+    const float *val;      /* nnz or NULL */
+} apspie_csr;
+
+/* COO triples, the three raw arrays the drivers pass around
+ * (test/bfs/main.cu:70-86, coo.cuh:24-100). */
+typedef struct {
+    int32_t m, n;
+    int64_t nnz;
+    const int32_t *row;
+    const int32_t *col;
+    const float *val; /* nnz or NULL */
+} apspie_coo;
+
+/* ------------------------------------------------------------------ context */
+/* Replaces mgpu::CreateCudaDevice / mgpu::CudaContext& (test/bfs/main.cu:27). */
+int apspie_ctx_create(int device, apspie_ctx **out);
+int apspie_ctx_destroy(apspie_ctx *ctx);
+/* Run on a caller stream (hipStream_t) instead of the context's own; NULL
+ * restores the context stream. */
+int apspie_ctx_set_stream(apspie_ctx *ctx, void *hip_stream);
+void *apspie_ctx_stream(apspie_ctx *ctx);
+/* Cap on HBM the context may hold (graphs it uploads + scratch); 0 = none. */
+int apspie_ctx_set_hbm_budget(apspie_ctx *ctx, int64_t bytes);
+int64_t apspie_ctx_hbm_used(apspie_ctx *ctx);
+/* Tuning knobs by name (tile sizes, bin thresholds, path choice); see
+ * apspie_ctx_get_option for the list of names. */
+int apspie_ctx_set_option(apspie_ctx *ctx, const char *name, int64_t value);
+int apspie_ctx_get_option(apspie_ctx *ctx, const char *name, int64_t *value);
+/* Release cached scratch memory. */
+int apspie_ctx_trim(apspie_ctx *ctx);
+const char *apspie_last_error(apspie_ctx *ctx);
+const char *apspie_status_str(int status);
+int apspie_abi_version(void);
+
+/* Counters from the last algorithm call on this context. */
+typedef struct {
+    int64_t edges_traversed; /* sum of out-degrees of expanded frontier vertices */
+    int64_t vertices_reached;
+    int32_t iterations;      /* BFS/SSSP levels, MIS rounds */
+    int32_t host_syncs;      /* host round trips taken */
+    double elapsed_ms;       /* device time of the call (hipEvent) */
+    double kernel_ms;        /* summed time of the dominant kernel (profiling on) */
+    int64_t kernel_launches; /* launches of the dominant kernel (profiling on) */
+} apspie_stats;
+int apspie_last_stats(apspie_ctx *ctx, apspie_stats *out);
+
+/* ------------------------------------------------------------------- graphs */
+/* Copy a host CSR into HBM (replaces the H2D copies of the drivers,
+ * test/bfs/main.cu:118-120, and buildMatrix's device half, matrix.cpp:107-110). */
+int apspie_graph_upload(apspie_ctx *ctx, const apspie_csr *host, apspie_graph **out);
+/* Wrap caller-owned device CSR arrays without copying: the form the
+ * reference's entry points take (bfs.cuh:32-33, sssp.cuh:27-28). */
+int apspie_graph_wrap(apspie_ctx *ctx, const apspie_csr *device, apspie_graph **out);
+/* Build a CSR on the device from COO triples (host or device memory):
+ * optional symmetrize + dedup + self-loop removal (makeSymmetric,
+ * util.cuh:571-643) then COO->CSR (buildMatrix matrix.cpp:55-111;
+ * cusparseXcoo2csr util.cuh:456-471). */
+int apspie_graph_from_coo(apspie_ctx *ctx, const apspie_coo *coo, int mem, int symmetrize,
+                          apspie_graph **out);
+int apspie_graph_destroy(apspie_graph *g);
+int apspie_graph_info(const apspie_graph *g, int32_t *m, int32_t *n, int64_t *nnz,
+                      int64_t *hbm_bytes);
+/* Copy the CSR back to host arrays (rowptr m+1, col nnz, val nnz or NULL). */
+int apspie_graph_download(apspie_ctx *ctx, const apspie_graph *g, int32_t *rowptr,
+                          int32_t *col, float *val);
+/* Transpose (csr2csc, util.cuh:473-511; cusparseScsr2csc). */
+int apspie_graph_transpose(apspie_ctx *ctx, const apspie_graph *g, apspie_graph **out);
+
+/* --------------------------------------------------------------- algorithms */
+/* Level-synchronous BFS, depth[v] = hops from source, -1 unreached.
+ * Replaces bfs<T>() (bfs.cuh:32-125) and spmspvBfs() (spmspvBfs.cuh:96-243);
+ * runs to convergence (the reference's depth argument is not needed). */
+int apspie_bfs(apspie_ctx *ctx, const apspie_graph *g, int32_t source, int32_t *depth,
+               int mem);
+/* Single-source shortest paths under (min,+) over the graph's FP32 weights;
+ * unreached = APSPIE_SSSP_INF.  Replaces sssp<V>() (sssp.cuh:27-115) +
+ * spmspvSssp() (spmspvSssp.cuh:91-291), with real weights in every round. */
+int apspie_sssp(apspie_ctx *ctx, const apspie_graph *g, int32_t source, float *dist,
+                int mem);
+/* Maximal independent set by Luby rounds under the max-priority semiring;
+ * in_set[v] = 1 in the set, 0 not.  prio (m uint32, host or device per
+ * prio_mem) may be NULL: then prio[v] = v-th draw of mt19937 seeded 5489.
+ * Ties break to the lower id.  Replaces spmspvMis() (spmspvMis.cuh:149-344). */
+int apspie_mis(apspie_ctx *ctx, const apspie_graph *g, const uint32_t *prio, int prio_mem,
+               int32_t *in_set, int mem);
+/* Triangle count by masked SpGEMM L.L o L on the strictly-lower part:
+ * per_row[r] = triangles whose largest vertex is r (may be NULL), *total on
+ * the host.  Replaces spgemm() + ewiseMultTc() + Reduce (tc.cuh:62-199,
+ * test/tc/main.cu:243-249), exact in uint64. */
+int apspie_tc(apspie_ctx *ctx, const apspie_graph *g, uint64_t *total, int32_t *per_row,
+              int mem);
+
+/* --------------------------------------------------------------- primitives */
+/* y = A^T x under a semiring (sparse mXv, mXv.cuh:249-389): x as sorted
+ * indices + values (x_val NULL => all ones); mask (dense int32 m, NULL => none)
+ * drops outputs with mask[v] != 0.  y_ind/y_val need capacity m; *ny is
+ * written on the host.  All arrays live in `mem`. */
+int apspie_spmspv(apspie_ctx *ctx, const apspie_graph *g, int semiring, int64_t nx,
+                  const int32_t *x_ind, const float *x_val, const int32_t *mask,
+                  int64_t *ny, int32_t *y_ind, float *y_val, int mem, int path);
+/* C = A * B over (+,x) by row-binned hash SpGEMM (spgemm(), spgemm.cuh:131-342
+ * + LaunchKernel spgemmKernel.cuh:438-738).  C is a new device graph with
+ * sorted rows. */
+int apspie_spgemm(apspie_ctx *ctx, const apspie_graph *a, const apspie_graph *b,
+                  apspie_graph **c);
+/* Exclusive prefix sum of n int32 (mgpu::Scan<MgpuScanTypeExc>); *total on host. */
+int apspie_scan_exclusive(apspie_ctx *ctx, const int32_t *in, int32_t *out, int64_t n,
+                          int64_t *total, int mem);
+/* Stable key-value radix sort on the low `bits` bits of uint32 keys
+ * (cub::DeviceRadixSort::SortPairs); vals may be NULL (SortKeys). */
+int apspie_sort_pairs(apspie_ctx *ctx, uint32_t *keys, uint32_t *vals, int64_t n, int bits,
+                      int mem);
+
+/* ------------------------------------------------------------------ batched */
+/* Many sources over one resident graph, advanced together as a sparse
+ * frontier matrix (one bit / one distance column per source).  Output is
+ * row-major [nsrc][m].  No reference counterpart: equal, per source, to
+ * apspie_bfs / apspie_sssp. */
+int apspie_bfs_batch(apspie_ctx *ctx, const apspie_graph *g, const int32_t *sources,
+                     int32_t nsrc, int32_t *depth, int mem);
+int apspie_sssp_batch(apspie_ctx *ctx, const apspie_graph *g, const int32_t *sources,
+                      int32_t nsrc, float *dist, int mem);
+/* Autotune batched/plain tile and bin choices for this graph; stores the
+ * winners in the context's tuning table. */
+int apspie_autotune(apspie_ctx *ctx, const apspie_graph *g, int32_t batch);
+
+/* ---------------------------------------------------------------- synthetic */
+/* Host-side graph construction (synthetic inputs; the reference has none).
+ * Two-call protocol: pass NULL arrays to get sizes. */
+int apspie_gen_rmat(int scale, int edge_factor, double a, double b, double c,
+                    uint32_t seed, int32_t *rowptr, int32_t *col, int64_t *nnz);
+int apspie_gen_lattice(int32_t width, uint32_t seed, int32_t *rowptr, int32_t *col,
+                       int64_t *nnz);
+int apspie_gen_weights(int32_t m, const int32_t *rowptr, const int32_t *col, uint32_t seed,
+                       float *val);
+/* Read a MatrixMarket file into COO host arrays (readEdge/readMtx,
+ * util.cuh:354-430; first index -> col, second -> row). */
+int apspie_read_mtx(const char *path, int32_t *m, int32_t *n, int64_t *nnz, int32_t *row,
+                    int32_t *col, float *val);
+
+/* ------------------------------------------------------------------ serving */
+typedef enum { APSPIE_Q_BFS = 0, APSPIE_Q_SSSP = 1, APSPIE_Q_TC = 2 } apspie_query_kind;
+
+typedef struct {
+    int32_t max_batch;     /* sources per batched launch */
+    int32_t slots;         /* concurrent batches in flight (streams) */
+    int64_t hbm_budget;    /* bytes for resident graphs + per-slot scratch */
+    int32_t queue_capacity;
+} apspie_server_config;
+
+/* Continuous-batching query server over HBM-resident graphs. */
+int apspie_server_create(int device, const apspie_server_config *cfg, apspie_server **out);
+int apspie_server_destroy(apspie_server *s);
+/* Admit a host CSR into the resident pool; fails with APSPIE_ERR_BUDGET. */
+int apspie_server_add_graph(apspie_server *s, const apspie_csr *host, int32_t *graph_id);
+/* Enqueue a query; returns a ticket.  source ignored for TC. */
+int apspie_server_submit(apspie_server *s, int32_t graph_id, int kind, int32_t source,
+                         int64_t *ticket);
+/* Wait for a ticket; copies the result (m int32 depths / m float distances /
+ * 1 uint64 count) into `out` (host) and reports arrival->completion latency. */
+int apspie_server_wait(apspie_server *s, int64_t ticket, void *out, int64_t out_bytes,
+                       double *latency_ms);
+int apspie_server_drain(apspie_server *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
