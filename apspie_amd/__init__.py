@@ -92,6 +92,10 @@ def _declare(L):
         "apspie_ctx_set_option": ([vp, C.c_char_p, i64], C.c_int),
         "apspie_ctx_get_option": ([vp, C.c_char_p, C.POINTER(i64)], C.c_int),
         "apspie_ctx_trim": ([vp], C.c_int),
+        "apspie_ctx_sync": ([vp], C.c_int),
+        "apspie_dev_alloc": ([vp, i64, pp], C.c_int),
+        "apspie_dev_free": ([vp, vp, i64], C.c_int),
+        "apspie_memcpy": ([vp, vp, vp, i64, C.c_int], C.c_int),
         "apspie_last_stats": ([vp, C.POINTER(Stats)], C.c_int),
         "apspie_graph_upload": ([vp, C.POINTER(_Csr), pp], C.c_int),
         "apspie_graph_wrap": ([vp, C.POINTER(_Csr), pp], C.c_int),
@@ -259,6 +263,13 @@ class Context:
     def trim(self):
         _check(self._L.apspie_ctx_trim(self.h), self.h)
 
+    # -- device memory (plain HIP allocations owned by the caller)
+    def dev_alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def sync(self):
+        _check(self._L.apspie_ctx_sync(self.h), self.h)
+
     # -- graphs
     def upload(self, rowptr, col, val=None, n=None):
         return Graph.upload(self, rowptr, col, val, n)
@@ -343,6 +354,31 @@ class Context:
         _check(self._L.apspie_autotune(self.h, g.h, int(batch)), self.h)
 
 
+class DeviceBuffer:
+    """A device allocation made through the library (counts against the
+    context's HBM budget); .ptr is the raw device address."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx, self.nbytes = ctx, int(nbytes)
+        p = C.c_void_p()
+        _check(lib().apspie_dev_alloc(ctx.h, self.nbytes, C.byref(p)), ctx.h)
+        self.ptr = p.value
+
+    def to_host(self, dtype, count):
+        out = np.empty(count, dtype)
+        _check(lib().apspie_memcpy(self.ctx.h, _p(out), self.ptr, out.nbytes, 1), self.ctx.h)
+        return out
+
+    def from_host(self, arr):
+        arr = np.ascontiguousarray(arr)
+        _check(lib().apspie_memcpy(self.ctx.h, self.ptr, _p(arr), arr.nbytes, 0), self.ctx.h)
+
+    def free(self):
+        if self.ptr and self.ctx.h:
+            lib().apspie_dev_free(self.ctx.h, self.ptr, self.nbytes)
+        self.ptr = None
+
+
 class Graph:
     """An HBM-resident CSR matrix."""
 
@@ -360,6 +396,8 @@ class Graph:
         rowptr = np.ascontiguousarray(rowptr, np.int32)
         col = np.ascontiguousarray(col, np.int32)
         val = None if val is None else np.ascontiguousarray(val, np.float32)
+        if val is not None and len(val) == 0:
+            val = np.zeros(1, np.float32)  # weighted but edgeless: keep a non-null pointer
         m = len(rowptr) - 1
         csr = _Csr(m, m if n is None else n, len(col), _p(rowptr).value,
                    _p(col).value if len(col) else None,

@@ -1,0 +1,287 @@
+// Batched multi-source BFS / SSSP: the frontier is a sparse m x B matrix
+// (SpMSpM instead of SpMSpV).  No reference counterpart; per source the
+// results equal apspie_bfs / apspie_sssp.
+//
+// Layout (vertex-major, HBM): W = ceil(B/32) words of bits per vertex for
+// visited / current-frontier / next-frontier, and for SSSP B contiguous FP32
+// distances per vertex, so one expanded edge touches contiguous words.
+// A level is one load-balanced expansion over the vertices whose frontier row
+// is non-empty (the same tile machinery as the single-source path):
+//   BFS : new = front[u] & ~visited[v]; fetch-or into visited[v] decides which
+//         source bits this thread discovered first -> depth writes; the
+//         discovered bits go to next[v] and v joins the next frontier once.
+//   SSSP: for each source bit s of front[u]: relax dist[v][s] with an atomic
+//         min; improved bits go to next[v].
+// then the current frontier rows are cleared for reuse.  Level loops run in
+// chunks captured into a hipGraph (replayed until the frontier is empty), so
+// the host pays one launch per chunk instead of two per level.
+#include <algorithm>
+#include <map>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "frontier.hpp"
+#include "timing.hpp"
+
+namespace apspie {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kTile = 2048;
+constexpr int kMaxBatch = 512;  // sources per pass (larger batches are split)
+
+template <int W>
+struct BfsBatchOp {
+    const uint32_t *front;
+    uint32_t *vis;
+    uint32_t *next;
+    int32_t *stamp;
+    int32_t *depth;  // [nsrc][m]
+    int32_t m;
+    const int32_t *lvl;  // device counter: depth of this level's discoveries
+    int32_t nsrc;
+    __device__ __forceinline__ bool edge(int32_t u, int32_t, int32_t v) const {
+        const int32_t level = *lvl;
+        bool any = false;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const uint32_t f = front[(int64_t)u * W + w];
+            if (!f) continue;
+            uint32_t *pv = vis + (int64_t)v * W + w;
+            uint32_t cand = f & ~*pv;
+            if (!cand) continue;
+            const uint32_t old = dev::atomic_or_dev(pv, cand);
+            uint32_t mine = cand & ~old;
+            if (!mine) continue;
+            __hip_atomic_fetch_or(next + (int64_t)v * W + w, mine, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+            any = true;
+            while (mine) {
+                const int b = __ffs(mine) - 1;
+                mine &= mine - 1;
+                const int s = w * 32 + b;
+                if (s < nsrc) depth[(int64_t)s * m + v] = level;
+            }
+        }
+        return any && dev::atomic_exch_dev(stamp + v, level) != level;
+    }
+};
+
+template <int W>
+struct SsspBatchOp {
+    const uint32_t *front;
+    uint32_t *next;
+    int32_t *stamp;
+    int32_t *dist;  // [m][32W] float bits
+    const float *wt;
+    const int32_t *lvl;  // device round counter
+    __device__ __forceinline__ bool edge(int32_t u, int32_t e, int32_t v) const {
+        const int32_t round = *lvl;
+        constexpr int B = 32 * W;
+        const float we = wt[e];
+        bool any = false;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            uint32_t f = front[(int64_t)u * W + w];
+            uint32_t improved = 0;
+            while (f) {
+                const int b = __ffs(f) - 1;
+                f &= f - 1;
+                const int s = w * 32 + b;
+                const int32_t nd = dev::fbits(dev::bitsf(dist[(int64_t)u * B + s]) + we);
+                int32_t *pv = dist + (int64_t)v * B + s;
+                if (nd >= *pv) continue;
+                if (nd < dev::atomic_min_dev(pv, nd)) improved |= 1u << b;
+            }
+            if (improved) {
+                __hip_atomic_fetch_or(next + (int64_t)v * W + w, improved, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+                any = true;
+            }
+        }
+        return any && dev::atomic_exch_dev(stamp + v, round) != round;
+    }
+};
+
+// zero the frontier rows of the entries just expanded; advance the level
+template <int W>
+__global__ void k_clear_rows(FrontierBufs cur, const uint64_t *cnt, uint32_t *front,
+                             int32_t *lvl) {
+    const int32_t F = (int32_t)cnt_entries(*cnt);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && F > 0) *lvl += 1;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)F * W;
+         i += (int64_t)gridDim.x * blockDim.x)
+        front[(int64_t)cur.v[i / W] * W + (i % W)] = 0;
+}
+
+// seed: source i -> bit i of its vertex; depth/dist 0 at the source
+template <int W>
+__global__ void k_batch_seed(const int32_t *srcs, int32_t nsrc, uint32_t *vis, uint32_t *front,
+                             int32_t *depth, int32_t m, int32_t *dist) {
+    for (int i = threadIdx.x; i < nsrc; i += blockDim.x) {
+        const int32_t s = srcs[i];
+        if (vis) atomicOr(vis + (int64_t)s * W + i / 32, 1u << (i % 32));
+        atomicOr(front + (int64_t)s * W + i / 32, 1u << (i % 32));
+        if (depth) depth[(int64_t)i * m + s] = 0;
+        if (dist) dist[(int64_t)s * (32 * W) + i] = 0;
+    }
+}
+
+struct SeedOp {
+    __device__ __forceinline__ bool keep(int32_t) const { return true; }
+};
+
+// dist [m][B] -> out [nsrc][m]
+__global__ void k_transpose_dist(const int32_t *dist, int32_t m, int32_t B, int32_t nsrc,
+                                 float *out) {
+    __shared__ float tile[32][33];
+    const int64_t v0 = (int64_t)blockIdx.x * 32, s0 = (int64_t)blockIdx.y * 32;
+    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 32 x 8
+    for (int r = ty; r < 32; r += 8) {
+        const int64_t v = v0 + r, s = s0 + tx;
+        if (v < m && s < nsrc) tile[r][tx] = dev::bitsf(dist[v * B + s]);
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int64_t s = s0 + r, v = v0 + tx;
+        if (v < m && s < nsrc) out[s * m + v] = tile[tx][r];
+    }
+}
+
+// One batched traversal pass for up to 32*W sources.
+template <int W, bool SSSP>
+void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, int32_t *depth,
+                float *dist_out) {
+    constexpr int B = 32 * W;
+    const int32_t m = g->m;
+    const int64_t words = (int64_t)m * W;
+    FrontierSet fs = frontier_alloc(c, "batch", m, g->nnz, kTile, true);
+    uint32_t *vis = SSSP ? nullptr : (uint32_t *)scratch(c, "batch_vis", sizeof(uint32_t) * words);
+    uint32_t *fa = (uint32_t *)scratch(c, "batch_fa", sizeof(uint32_t) * words);
+    uint32_t *fb = (uint32_t *)scratch(c, "batch_fb", sizeof(uint32_t) * words);
+    int32_t *stamp = (int32_t *)scratch(c, "batch_stamp", sizeof(int32_t) * m);
+    int32_t *dist = SSSP ? (int32_t *)scratch(c, "batch_dist", sizeof(int32_t) * m * (int64_t)B)
+                         : nullptr;
+    int32_t *srcs = (int32_t *)scratch(c, "batch_srcs", sizeof(int32_t) * (B + 1));
+    APSPIE_HIP(hipMemcpyAsync(srcs, src_host, sizeof(int32_t) * nsrc, hipMemcpyHostToDevice,
+                              c->stream));
+    // distinct source vertices form the first frontier
+    std::vector<int32_t> uniq(src_host, src_host + nsrc);
+    std::sort(uniq.begin(), uniq.end());
+    uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+    int32_t *ulist = (int32_t *)scratch(c, "batch_ulist", sizeof(int32_t) * (B + 1));
+    APSPIE_HIP(hipMemcpyAsync(ulist, uniq.data(), sizeof(int32_t) * uniq.size(),
+                              hipMemcpyHostToDevice, c->stream));
+    const int grid = fill_grid(c, kBlock, (int)opt(c, "batch_blocks_per_cu", 4));
+    if (!SSSP) {
+        fill_i32(c, (int32_t *)vis, 0, words);
+        fill_i32(c, depth, -1, (int64_t)nsrc * m);
+    } else {
+        fill_i32(c, dist, __builtin_bit_cast(int32_t, APSPIE_SSSP_INF), (int64_t)m * B);
+    }
+    fill_i32(c, (int32_t *)fa, 0, words);
+    fill_i32(c, (int32_t *)fb, 0, words);
+    fill_i32(c, stamp, -1, m);
+    fill_i32(c, (int32_t *)fs.ctl, 0, 32);
+    int32_t *lvl = (int32_t *)(fs.ctl + 12);
+    fill_i32(c, lvl, 1, 1);
+    k_batch_seed<W><<<1, 256, 0, c->stream>>>(srcs, nsrc, vis, fa, depth, m, dist);
+    k_build<kBlock, 1, kTile, true><<<1, kBlock, 0, c->stream>>>(
+        SeedOp{}, g->rowptr, ulist, (int32_t)uniq.size(), nullptr, fs.f[0], fs.ctl + 0);
+    APSPIE_LAUNCH_CHECK();
+
+    // level l: expand f[l&1] (bits in fa/fb alternating), clear its rows
+    auto level = [&](int l) {
+        const int cur = l & 1;
+        uint32_t *front = cur ? fb : fa, *next = cur ? fa : fb;
+        if (SSSP)
+            k_expand<kBlock, kTile, true, true, true><<<grid, kBlock, 0, c->stream>>>(
+                SsspBatchOp<W>{front, next, stamp, dist, g->val, lvl}, g->rowptr, g->col,
+                (int32_t)g->nnz, fs.f[cur], fs.ctl + (l % 3), fs.f[cur ^ 1],
+                fs.ctl + ((l + 1) % 3), fs.ctl + ((l + 2) % 3), fs.ctl + 4);
+        else
+            k_expand<kBlock, kTile, true, true, true><<<grid, kBlock, 0, c->stream>>>(
+                BfsBatchOp<W>{front, vis, next, stamp, depth, m, lvl, nsrc}, g->rowptr, g->col,
+                (int32_t)g->nnz, fs.f[cur], fs.ctl + (l % 3), fs.f[cur ^ 1],
+                fs.ctl + ((l + 1) % 3), fs.ctl + ((l + 2) % 3), fs.ctl + 4);
+        k_clear_rows<W><<<grid, kBlock, 0, c->stream>>>(fs.f[cur], fs.ctl + (l % 3), front, lvl);
+    };
+
+    // chunks of K levels captured once and replayed: K is a multiple of 6 so
+    // the counter ring (period 3) and the buffer parity (period 2) realign;
+    // level numbers come from the device counter, not from kernel arguments
+    const int K = 6 * (int)std::max<int64_t>(1, ceil_div(opt(c, "bfs_levels_per_sync", 8), 6));
+    const bool use_graph = opt(c, "batch_graph", 1) != 0;
+    hipGraphExec_t exec = nullptr;
+    int l = 0;
+    uint64_t *h = (uint64_t *)c->pinned;
+    KernelTimer kt(c);
+    for (int chunk = 0;; ++chunk) {
+        kt.begin();
+        if (use_graph && exec) {
+            APSPIE_HIP(hipGraphLaunch(exec, c->stream));
+            l += K;
+        } else if (use_graph) {
+            hipGraph_t graph;
+            APSPIE_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < K; ++k) level(l + k);
+            APSPIE_HIP(hipStreamEndCapture(c->stream, &graph));
+            APSPIE_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            APSPIE_HIP(hipGraphDestroy(graph));
+            APSPIE_HIP(hipGraphLaunch(exec, c->stream));
+            l += K;
+        } else {
+            for (int k = 0; k < K; ++k) level(l + k);
+            l += K;
+        }
+        kt.end();
+        APSPIE_LAUNCH_CHECK();
+        APSPIE_HIP(hipMemcpyAsync(h, fs.ctl + (l % 3), sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                  c->stream));
+        APSPIE_HIP(hipMemcpyAsync(h + 1, fs.ctl + 4, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                  c->stream));
+        sync(c);
+        c->stats.host_syncs++;
+        if (cnt_entries(h[0]) == 0) break;
+        if (l > 4 * (int64_t)m + 64) {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            throw Error{APSPIE_ERR_DEVICE, "batched traversal did not converge"};
+        }
+    }
+    if (exec) APSPIE_HIP(hipGraphExecDestroy(exec));
+    c->stats.edges_traversed += (int64_t)h[1];
+    c->stats.vertices_reached += (int64_t)h[2];
+    c->stats.iterations = std::max(c->stats.iterations, (int32_t)h[3]);
+    kt.collect();
+    if (SSSP) {
+        dim3 grd((unsigned)ceil_div(m, 32), (unsigned)ceil_div(nsrc, 32));
+        k_transpose_dist<<<grd, 256, 0, c->stream>>>(dist, m, B, nsrc, dist_out);
+        APSPIE_LAUNCH_CHECK();
+    }
+}
+
+template <bool SSSP>
+void batch_run(Ctx *c, const Graph *g, const int32_t *src, int32_t nsrc, int32_t *depth,
+               float *dist) {
+    for (int32_t s0 = 0; s0 < nsrc; s0 += kMaxBatch) {
+        const int32_t k = std::min(kMaxBatch, nsrc - s0);
+        int32_t *d = depth ? depth + (int64_t)s0 * g->m : nullptr;
+        float *f = dist ? dist + (int64_t)s0 * g->m : nullptr;
+        if (k <= 32) batch_pass<1, SSSP>(c, g, src + s0, k, d, f);
+        else if (k <= 128) batch_pass<4, SSSP>(c, g, src + s0, k, d, f);
+        else batch_pass<16, SSSP>(c, g, src + s0, k, d, f);
+    }
+}
+
+}  // namespace
+
+void bfs_batch_run(Ctx *c, const Graph *g, const int32_t *src, int32_t nsrc, int32_t *depth) {
+    batch_run<false>(c, g, src, nsrc, depth, nullptr);
+}
+
+void sssp_batch_run(Ctx *c, const Graph *g, const int32_t *src, int32_t nsrc, float *dist) {
+    batch_run<true>(c, g, src, nsrc, nullptr, dist);
+}
+
+}  // namespace apspie

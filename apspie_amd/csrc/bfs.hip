@@ -92,7 +92,7 @@ void bfs_levels(Ctx *c, const Graph *g, int32_t source, int32_t *depth) {
     c->stats.edges_traversed = (int64_t)h[1];
     c->stats.vertices_reached = (int64_t)h[2];
     c->stats.iterations = (int32_t)h[3];
-    kt.collect();
+    kt.collect((int64_t)h[3]);
 }
 
 }  // namespace

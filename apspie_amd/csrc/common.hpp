@@ -60,7 +60,7 @@ struct Graph {
     int64_t bytes = 0;
     // derived, built lazily and cached with the graph
     int32_t *lowcnt = nullptr;  // per row: number of cols < row (TC)
-    int32_t max_degree = -1;
+    std::mutex derived_mu;      // guards lazily built derived arrays
 };
 
 struct Ctx {
@@ -136,6 +136,7 @@ void graph_from_coo_dev(Ctx *c, Graph *g, const int32_t *row, const int32_t *col
 void graph_transpose_dev(Ctx *c, const Graph *g, Graph *t);
 void mt19937_priorities(Ctx *c, uint32_t seed, int32_t m, uint32_t *out_dev);
 void fill_i32(Ctx *c, int32_t *p, int32_t v, int64_t n);
+void graph_prepare_tc(Ctx *c, Graph *g);
 
 }  // namespace apspie
 

@@ -20,6 +20,7 @@ static const char *kOptionNames[] = {
     "tc_small_row",      // rows with |L_r| <= this use a wave + LDS hash
     "tc_block_row",      // rows with |L_r| <= this use a block + LDS hash
     "batch_blocks_per_cu",
+    "batch_graph",       // 1 = replay batched level chunks from a captured hipGraph
     "profiling",         // 1 = time the dominant kernel with hipEvents
     nullptr};
 
@@ -36,6 +37,7 @@ static int64_t option_default(const std::string &n) {
     if (n == "tc_small_row") return 256;
     if (n == "tc_block_row") return 4096;
     if (n == "batch_blocks_per_cu") return 4;
+    if (n == "batch_graph") return 1;
     if (n == "profiling") return 0;
     return -1;
 }
@@ -130,16 +132,6 @@ void fill_i32(Ctx *c, int32_t *p, int32_t v, int64_t n) {
     APSPIE_LAUNCH_CHECK();
 }
 
-// per-row maximum degree (cached on the graph)
-__global__ void k_max_degree(const int32_t *rowptr, int32_t m, uint32_t *out) {
-    uint32_t best = 0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
-         i += (int64_t)gridDim.x * blockDim.x)
-        best = max(best, (uint32_t)(rowptr[i + 1] - rowptr[i]));
-    best = dev::wave_max_u32(best);
-    if (dev::lane_id() == 0) atomicMax(out, best);
-}
-
 static void validate_csr_host(const apspie_csr *h) {
     APSPIE_CHECK(h && h->rowptr && (h->col || h->nnz == 0), APSPIE_ERR_INVALID, "null CSR");
     APSPIE_CHECK(h->m >= 0 && h->n >= 0 && h->nnz >= 0, APSPIE_ERR_INVALID, "negative size");
@@ -158,6 +150,7 @@ static void check_graph(Ctx *c, const Graph *g) {
     APSPIE_CHECK(g->device == c->device, APSPIE_ERR_INVALID, "graph lives on another device");
 }
 
+// device time of one algorithm call, bracketed by events on the context stream
 template <class F>
 static void timed(Ctx *c, F &&f) {
     c->stats = apspie_stats{};
@@ -259,7 +252,10 @@ int apspie_ctx_set_option(apspie_ctx *c, const char *name, int64_t v) {
         return APSPIE_ERR_INVALID;
     }
     std::string n(name);
-    if (n == "bfs_tile" && (v < 256 || v > 8192 || (v & (v - 1)))) return APSPIE_ERR_INVALID;
+    if (n == "bfs_tile" && v != 1024 && v != 2048 && v != 4096) return APSPIE_ERR_INVALID;
+    if ((n == "bfs_blocks_per_cu" || n == "batch_blocks_per_cu") && (v < 1 || v > 32))
+        return APSPIE_ERR_INVALID;
+    if (n == "bfs_levels_per_sync" && (v < 1 || v > 4096)) return APSPIE_ERR_INVALID;
     if (n == "profiling") c->profiling = v != 0;
     c->options[n] = v;
     return APSPIE_OK;
@@ -284,6 +280,45 @@ int apspie_ctx_trim(apspie_ctx *c) {
 
 const char *apspie_last_error(apspie_ctx *c) { return c ? c->last_error.c_str() : "no context"; }
 
+int apspie_dev_alloc(apspie_ctx *c, int64_t bytes, void **ptr) {
+    if (!c || !ptr || bytes < 0) return APSPIE_ERR_INVALID;
+    *ptr = nullptr;
+    return guard(c, [&] {
+        set_device(c);
+        *ptr = dev_alloc(c, (size_t)bytes);
+    });
+}
+
+int apspie_dev_free(apspie_ctx *c, void *ptr, int64_t bytes) {
+    if (!c || bytes < 0) return APSPIE_ERR_INVALID;
+    return guard(c, [&] {
+        set_device(c);
+        sync(c);
+        dev_free(c, ptr, (size_t)bytes);
+    });
+}
+
+int apspie_memcpy(apspie_ctx *c, void *dst, const void *src, int64_t bytes, int kind) {
+    if (!c || bytes < 0 || (bytes && (!dst || !src)) || kind < 0 || kind > 2)
+        return APSPIE_ERR_INVALID;
+    return guard(c, [&] {
+        set_device(c);
+        const hipMemcpyKind k = kind == 0   ? hipMemcpyHostToDevice
+                                : kind == 1 ? hipMemcpyDeviceToHost
+                                            : hipMemcpyDeviceToDevice;
+        if (bytes) APSPIE_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, k, c->stream));
+        sync(c);
+    });
+}
+
+int apspie_ctx_sync(apspie_ctx *c) {
+    if (!c) return APSPIE_ERR_INVALID;
+    return guard(c, [&] {
+        set_device(c);
+        sync(c);
+    });
+}
+
 int apspie_last_stats(apspie_ctx *c, apspie_stats *out) {
     if (!c || !out) return APSPIE_ERR_INVALID;
     *out = c->stats;
@@ -304,13 +339,14 @@ int apspie_graph_upload(apspie_ctx *c, const apspie_csr *h, apspie_graph **out) 
         g->n = h->n;
         g->nnz = h->nnz;
         g->owns = true;
+        const bool has_val = h->val != nullptr;  // a weighted graph, even if edgeless
         size_t rb = sizeof(int32_t) * (h->m + 1), cb = sizeof(int32_t) * h->nnz,
-               vb = h->val ? sizeof(float) * h->nnz : 0;
+               vb = has_val ? sizeof(float) * h->nnz : 0;
         g->rowptr = (int32_t *)dev_alloc(c, rb);
         g->col = (int32_t *)dev_alloc(c, cb);
-        if (vb) g->val = (float *)dev_alloc(c, vb);
+        if (has_val) g->val = (float *)dev_alloc(c, vb);
         g->bytes = (int64_t)(std::max<size_t>(rb, 256) + std::max<size_t>(cb, 256) +
-                             (vb ? std::max<size_t>(vb, 256) : 0));
+                             (has_val ? std::max<size_t>(vb, 256) : 0));
         copy_to_device(c, g->rowptr, h->rowptr, rb, APSPIE_MEM_HOST);
         copy_to_device(c, g->col, h->col, cb, APSPIE_MEM_HOST);
         if (vb) copy_to_device(c, g->val, h->val, vb, APSPIE_MEM_HOST);
@@ -407,10 +443,7 @@ int apspie_graph_destroy(apspie_graph *g) {
             if (g->val) (void)hipFree(g->val);
         }
     }
-    if (g->lowcnt) {
-        if (c) dev_free(c, g->lowcnt, sizeof(int32_t) * std::max<int32_t>(g->m, 1));
-        else (void)hipFree(g->lowcnt);
-    }
+    if (g->lowcnt) (void)hipFree(g->lowcnt);
     delete g;
     return APSPIE_OK;
 }

@@ -18,6 +18,14 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 
 __device__ __forceinline__ int popc64(uint64_t x) { return __popcll(x); }
 
+// Orders LDS/global accesses of the lanes of one wave (a wave-level barrier
+// that is also a memory fence, so the compiler waits for outstanding LDS ops).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Inclusive wave-wide scan (+) of 32-bit values.  Uses the DPP row shifts
 // within each 16-lane row and row broadcasts across rows, the CDNA idiom.
 __device__ __forceinline__ uint32_t wave_inclusive_add(uint32_t x) {

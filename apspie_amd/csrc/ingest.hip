@@ -1,0 +1,198 @@
+// Graph ingestion on the device: COO -> CSR (optionally symmetrized,
+// deduplicated and loop-free) and CSR transpose.
+//
+// Reference: makeSymmetric (util.cuh:571-643: mirror, qsort by (row,col),
+// dedup), buildMatrix (matrix.cpp:55-111: counting sort by row that keeps the
+// input order inside a row), cusparseXcoo2csr (util.cuh:456-471) and
+// cusparseScsr2csc (util.cuh:473-511).  Ordering is done with the stable
+// radix sort: sorting by column and then by row orders entries by
+// (row, column) with ties in input order, so a duplicate keeps the value of
+// its first occurrence; the reference leaves self-loops in and moves values
+// out of step with indices during dedup (util.cuh:606-641) -- here loops are
+// dropped and values travel with their entry.
+#include "common.hpp"
+#include "device.hpp"
+#include "primitives.hpp"
+
+namespace apspie {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ void k_mirror(const int32_t *row, const int32_t *col, int64_t nnz, int32_t *r2,
+                         int32_t *c2, int32_t m, int32_t n, int32_t *bad) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t r = row[i], c = col[i];
+        if (r < 0 || r >= m || c < 0 || c >= n) bad[0] = 1;
+        r2[i] = r;
+        c2[i] = c;
+        r2[nnz + i] = c;
+        c2[nnz + i] = r;
+    }
+}
+
+__global__ void k_copy_check(const int32_t *row, const int32_t *col, int64_t nnz, int32_t *r2,
+                             int32_t *c2, int32_t m, int32_t n, int32_t *bad) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t r = row[i], c = col[i];
+        if (r < 0 || r >= m || c < 0 || c >= n) bad[0] = 1;
+        r2[i] = r;
+        c2[i] = c;
+    }
+}
+
+__global__ void k_iota_key(const int32_t *src, int64_t n, uint32_t *keys, uint32_t *idx) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        idx[i] = (uint32_t)i;
+        keys[i] = (uint32_t)src[i];
+    }
+}
+
+__global__ void k_gather_key(const int32_t *src, const uint32_t *idx, int64_t n, uint32_t *keys) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        keys[i] = (uint32_t)src[idx[i]];
+}
+
+// keep sorted entry e unless it is a self-loop or repeats entry e-1
+__global__ void k_keep(const int32_t *row, const int32_t *col, const uint32_t *idx, int64_t n,
+                       int dedup, int32_t *keep) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t a = idx[e];
+        int k = 1;
+        if (dedup) {
+            if (row[a] == col[a]) k = 0;
+            if (e > 0) {
+                const uint32_t b = idx[e - 1];
+                if (row[b] == row[a] && col[b] == col[a]) k = 0;
+            }
+        }
+        keep[e] = k;
+    }
+}
+
+__global__ void k_emit(const int32_t *row, const int32_t *col, const float *val,
+                       const uint32_t *idx, const int32_t *kept, int64_t k, int64_t nnz_in,
+                       int32_t *ocol, float *oval, int32_t *rowcount) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < k;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t a = idx[kept ? kept[i] : i];
+        ocol[i] = col[a];
+        if (oval) oval[i] = val ? val[a < nnz_in ? a : a - nnz_in] : 1.0f;
+        atomicAdd(rowcount + row[a], 1);
+    }
+}
+
+// row id of every CSR entry (binary search over rowptr)
+__global__ void k_entry_rows(const int32_t *rowptr, int32_t m, int64_t nnz, int32_t *rows) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        int32_t lo = 0, hi = m;  // largest r with rowptr[r] <= e
+        while (hi - lo > 1) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (rowptr[mid] <= e) lo = mid; else hi = mid;
+        }
+        rows[e] = lo;
+    }
+}
+
+void finish_rowptr(Ctx *c, Graph *g, int32_t *rowcount) {
+    fill_i32(c, rowcount + g->m, 0, 1);
+    const int64_t tot = scan_exclusive_i32(c, rowcount, g->rowptr, g->m + 1);
+    APSPIE_CHECK(tot == g->nnz, APSPIE_ERR_DEVICE, "row counts do not add up");
+}
+
+}  // namespace
+
+void graph_from_coo_dev(Ctx *c, Graph *g, const int32_t *row, const int32_t *col,
+                        const float *val, int64_t nnz, int symmetrize) {
+    const int grid = fill_grid(c, kBlock, 4);
+    const int64_t E = symmetrize ? 2 * nnz : nnz;
+    const int64_t Ea = std::max<int64_t>(E, 1);
+    int32_t *r2 = (int32_t *)scratch(c, "ing_r", sizeof(int32_t) * Ea);
+    int32_t *c2 = (int32_t *)scratch(c, "ing_c", sizeof(int32_t) * Ea);
+    uint32_t *keys = (uint32_t *)scratch(c, "ing_k", sizeof(uint32_t) * Ea);
+    uint32_t *idx = (uint32_t *)scratch(c, "ing_i", sizeof(uint32_t) * Ea);
+    int32_t *flag = (int32_t *)scratch(c, "ing_flag", sizeof(int32_t) * Ea);
+    int32_t *kept = (int32_t *)scratch(c, "ing_kept", sizeof(int32_t) * Ea);
+    int32_t *bad = (int32_t *)scratch(c, "ing_bad", 256);
+    fill_i32(c, bad, 0, 1);
+    if (nnz > 0) {
+        if (symmetrize)
+            k_mirror<<<grid, kBlock, 0, c->stream>>>(row, col, nnz, r2, c2, g->m, g->n, bad);
+        else
+            k_copy_check<<<grid, kBlock, 0, c->stream>>>(row, col, nnz, r2, c2, g->m, g->n, bad);
+        APSPIE_LAUNCH_CHECK();
+    }
+    int32_t hbad = 0;
+    APSPIE_HIP(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    APSPIE_CHECK(hbad == 0, APSPIE_ERR_INVALID, "COO index out of range");
+    int64_t k = E;
+    if (E > 0) {
+        if (symmetrize) {  // order by (row, col): sort by col, then stably by row
+            k_iota_key<<<grid, kBlock, 0, c->stream>>>(c2, E, keys, idx);
+            radix_sort_pairs(c, keys, idx, E, bits_for(g->n));
+            k_gather_key<<<grid, kBlock, 0, c->stream>>>(r2, idx, E, keys);
+        } else {           // buildMatrix: by row only, input order inside a row
+            k_iota_key<<<grid, kBlock, 0, c->stream>>>(r2, E, keys, idx);
+        }
+        radix_sort_pairs(c, keys, idx, E, bits_for(g->m));
+        k_keep<<<grid, kBlock, 0, c->stream>>>(r2, c2, idx, E, symmetrize, flag);
+        APSPIE_LAUNCH_CHECK();
+        k = symmetrize ? compact_flags(c, flag, E, kept) : E;
+    }
+    g->nnz = k;
+    g->rowptr = (int32_t *)dev_alloc(c, sizeof(int32_t) * (g->m + 1));
+    g->col = (int32_t *)dev_alloc(c, sizeof(int32_t) * std::max<int64_t>(k, 1));
+    g->val = (float *)dev_alloc(c, sizeof(float) * std::max<int64_t>(k, 1));
+    g->bytes = (int64_t)(sizeof(int32_t) * (g->m + 1) + 8 * std::max<int64_t>(k, 1));
+    int32_t *rowcount = (int32_t *)scratch(c, "ing_rowcount", sizeof(int32_t) * (g->m + 1));
+    fill_i32(c, rowcount, 0, g->m + 1);
+    if (k > 0) {
+        k_emit<<<grid, kBlock, 0, c->stream>>>(r2, c2, val, idx, symmetrize ? kept : nullptr, k,
+                                               nnz, g->col, g->val, rowcount);
+        APSPIE_LAUNCH_CHECK();
+    }
+    finish_rowptr(c, g, rowcount);
+}
+
+void graph_transpose_dev(Ctx *c, const Graph *g, Graph *t) {
+    const int grid = fill_grid(c, kBlock, 4);
+    const int64_t E = g->nnz, Ea = std::max<int64_t>(E, 1);
+    t->owner = c;
+    t->device = c->device;
+    t->m = g->n;
+    t->n = g->m;
+    t->nnz = E;
+    t->owns = true;
+    int32_t *rows = (int32_t *)scratch(c, "tr_rows", sizeof(int32_t) * Ea);
+    uint32_t *keys = (uint32_t *)scratch(c, "tr_k", sizeof(uint32_t) * Ea);
+    uint32_t *idx = (uint32_t *)scratch(c, "tr_i", sizeof(uint32_t) * Ea);
+    if (E > 0) {
+        k_entry_rows<<<grid, kBlock, 0, c->stream>>>(g->rowptr, g->m, E, rows);
+        k_iota_key<<<grid, kBlock, 0, c->stream>>>(g->col, E, keys, idx);
+        APSPIE_LAUNCH_CHECK();
+        radix_sort_pairs(c, keys, idx, E, bits_for(g->n));  // stable: rows stay ascending
+    }
+    t->rowptr = (int32_t *)dev_alloc(c, sizeof(int32_t) * (t->m + 1));
+    t->col = (int32_t *)dev_alloc(c, sizeof(int32_t) * Ea);
+    t->val = g->val ? (float *)dev_alloc(c, sizeof(float) * Ea) : nullptr;
+    t->bytes = (int64_t)(sizeof(int32_t) * (t->m + 1) + (g->val ? 8 : 4) * Ea);
+    int32_t *rowcount = (int32_t *)scratch(c, "tr_rowcount", sizeof(int32_t) * (t->m + 1));
+    fill_i32(c, rowcount, 0, t->m + 1);
+    if (E > 0) {
+        // new row = old column, new column = old row
+        k_emit<<<grid, kBlock, 0, c->stream>>>(g->col, rows, g->val, idx, nullptr, E, E, t->col,
+                                               t->val, rowcount);
+        APSPIE_LAUNCH_CHECK();
+    }
+    finish_rowptr(c, t, rowcount);
+}
+
+}  // namespace apspie

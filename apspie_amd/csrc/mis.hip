@@ -1,0 +1,148 @@
+// Maximal independent set by Luby rounds under the max-priority semiring.
+//
+// Semantics follow spmspvMis() (spmspvMis.cuh:149-344): every round, each
+// undecided candidate compares its priority with the max priority over its
+// undecided neighbours (the max-priority SpMSpV y = A^T x, x = candidate
+// priorities, :277-289); candidates that beat all neighbours join
+// (updateMis :63-68) and their neighbours leave (updateNeighbor :70-78).
+// Priorities are uint32 (default: mt19937 seeded 5489, the default seed of
+// mt19937ar.cpp:111-112) with ties broken to the lower vertex id, i.e. the
+// key (prio << 32) | (0x7fffffff - v): a total order, so the rounds converge
+// to the greedy MIS in descending key order, whatever the schedule.  The
+// reference's threshold schedule (minimum -= delta) only changes the number
+// of rounds, not the set, and is not used.
+//
+// Per round (no host round trip): k_expand marks candidates that lose to an
+// undecided neighbour (pull over the candidate's row, no atomics), k_join
+// admits the winners, k_expand removes candidates adjacent to a winner, and
+// k_build compacts the survivors into the next candidate frontier.
+#include <random>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "frontier.hpp"
+#include "timing.hpp"
+
+namespace apspie {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kTile = 2048;
+
+__device__ __forceinline__ uint64_t mis_key(const uint32_t *prio, int32_t v) {
+    return (uint64_t(prio[v]) << 32) | uint64_t(0x7fffffffu - (uint32_t)v);
+}
+
+// candidate u loses if some undecided neighbour has a larger key
+struct LoseOp {
+    const uint32_t *prio;
+    const int32_t *state;
+    int32_t *lose;
+    int32_t round;
+    __device__ __forceinline__ bool edge(int32_t u, int32_t, int32_t v) const {
+        if (v == u || lose[u] == round) return false;
+        if (state[v] != 0 && mis_key(prio, v) > mis_key(prio, u)) lose[u] = round;
+        return false;
+    }
+};
+
+// candidate u that did not join leaves if a neighbour joined
+struct RemoveOp {
+    int32_t *state;
+    __device__ __forceinline__ bool edge(int32_t u, int32_t, int32_t v) const {
+        if (v != u && state[u] == -1 && state[v] == 1) state[u] = 0;
+        return false;
+    }
+};
+
+struct UndecidedOp {
+    const int32_t *state;
+    __device__ __forceinline__ bool keep(int32_t v) const { return state[v] == -1; }
+};
+
+__global__ void k_mis_init(const int32_t *rowptr, int32_t m, int32_t *state, int32_t *lose,
+                           uint64_t *ctl) {
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < m;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        // a vertex without neighbours is in every maximal independent set
+        state[v] = rowptr[v + 1] == rowptr[v] ? 1 : -1;
+        lose[v] = -1;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 16) ctl[threadIdx.x] = 0;
+}
+
+__global__ void k_mis_join(FrontierBufs cur, const uint64_t *cnt, const int32_t *lose,
+                           int32_t round, int32_t *state) {
+    const int32_t F = (int32_t)cnt_entries(*cnt);
+    for (int32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < F; j += gridDim.x * blockDim.x) {
+        const int32_t u = cur.v[j];
+        if (lose[u] != round) state[u] = 1;
+    }
+}
+
+}  // namespace
+
+void mis_run(Ctx *c, const Graph *g, const uint32_t *prio, int32_t *state) {
+    if (g->m == 0) return;
+    FrontierSet fs = frontier_alloc(c, "mis", g->m, g->nnz, kTile, true);
+    int32_t *lose = (int32_t *)scratch(c, "mis_lose", sizeof(int32_t) * g->m);
+    const int grid = fill_grid(c, kBlock, 4);
+    k_mis_init<<<grid, kBlock, 0, c->stream>>>(g->rowptr, g->m, state, lose, fs.ctl);
+    APSPIE_LAUNCH_CHECK();
+    // round 0 candidates: every undecided vertex (degree > 0)
+    k_build<kBlock, 8, kTile, true><<<grid, kBlock, 0, c->stream>>>(
+        UndecidedOp{state}, g->rowptr, nullptr, g->m, nullptr, fs.f[0], fs.ctl + 0);
+    APSPIE_LAUNCH_CHECK();
+    KernelTimer kt(c);
+    int round = 0;
+    int chunk = 4;
+    for (;;) {
+        for (int k = 0; k < chunk; ++k, ++round) {
+            const int cur = round & 1;
+            uint64_t *cnt = fs.ctl + (round % 3), *nxt = fs.ctl + ((round + 1) % 3);
+            kt.begin();
+            // lose marks; this launch also clears the next round's counter
+            k_expand<kBlock, kTile, true, false, false><<<grid, kBlock, 0, c->stream>>>(
+                LoseOp{prio, state, lose, round}, g->rowptr, g->col, (int32_t)g->nnz, fs.f[cur],
+                cnt, fs.f[cur ^ 1], nullptr, nxt, fs.ctl + 4);
+            kt.end();
+            k_mis_join<<<grid, kBlock, 0, c->stream>>>(fs.f[cur], cnt, lose, round, state);
+            k_expand<kBlock, kTile, true, false, false><<<grid, kBlock, 0, c->stream>>>(
+                RemoveOp{state}, g->rowptr, g->col, (int32_t)g->nnz, fs.f[cur], cnt,
+                fs.f[cur ^ 1], nullptr, nullptr, nullptr);
+            k_build<kBlock, 8, kTile, true><<<grid, kBlock, 0, c->stream>>>(
+                UndecidedOp{state}, g->rowptr, fs.f[cur].v, 0, cnt, fs.f[cur ^ 1], nxt);
+            APSPIE_LAUNCH_CHECK();
+        }
+        uint64_t *h = (uint64_t *)c->pinned;
+        APSPIE_HIP(hipMemcpyAsync(h, fs.ctl + (round % 3), sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                  c->stream));
+        APSPIE_HIP(hipMemcpyAsync(h + 1, fs.ctl + 4, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                  c->stream));
+        sync(c);
+        c->stats.host_syncs++;
+        if (cnt_entries(h[0]) == 0) break;
+        // the undecided vertex with the largest key joins every round
+        APSPIE_CHECK(round <= g->m + 1, APSPIE_ERR_DEVICE, "MIS did not converge");
+        chunk = std::min(chunk * 2, 64);
+    }
+    uint64_t *h = (uint64_t *)c->pinned;
+    c->stats.edges_traversed = (int64_t)h[1];
+    c->stats.vertices_reached = (int64_t)h[2];
+    c->stats.iterations = (int32_t)h[3];
+    kt.collect();
+}
+
+// mt19937(seed) stream, one draw per vertex: generated on the host (the
+// generator is sequential by definition) and copied once.
+void mt19937_priorities(Ctx *c, uint32_t seed, int32_t m, uint32_t *out) {
+    std::vector<uint32_t> h((size_t)std::max(m, 1));
+    std::mt19937 gen(seed);
+    for (int32_t i = 0; i < m; ++i) h[i] = gen();
+    APSPIE_HIP(hipMemcpyAsync(out, h.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice,
+                              c->stream));
+    sync(c);
+}
+
+}  // namespace apspie

@@ -1,0 +1,281 @@
+// Continuous-batching query server over HBM-resident graphs.
+//
+// No reference counterpart (the reference answers one query per process).
+// Graphs are admitted into a resident pool against an HBM budget.  Queries
+// (BFS, SSSP, TC) enter one FIFO.  `slots` worker threads each own a context
+// (its own HIP stream and scratch); a free worker takes the oldest query and
+// coalesces every queued query of the same graph and kind into one launch:
+// up to max_batch sources go through the batched multi-source traversal, and
+// all queued TC queries on a graph share one count.  Queries that arrive while
+// other batches run are picked up as soon as a worker frees, so their work
+// overlaps on the device with the batches in flight.  A batch that does not
+// fit the worker's share of the budget is split in halves; a single query
+// that does not fit fails with APSPIE_ERR_BUDGET.  Results land in host
+// memory; latency is measured from submit to completion on the host clock.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "common.hpp"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+    int64_t ticket = 0;
+    int32_t graph = 0;
+    int kind = 0;
+    int32_t source = 0;
+    Clock::time_point t_submit, t_done;
+    std::vector<uint8_t> result;
+    int status = APSPIE_OK;
+    bool done = false;
+};
+
+struct Resident {
+    apspie_graph *g = nullptr;
+    int32_t m = 0;
+    int64_t bytes = 0;
+};
+
+}  // namespace
+
+struct apspie_server {
+    int device = 0;
+    apspie_server_config cfg{};
+    apspie_ctx *admin = nullptr;
+    std::vector<Resident> graphs;
+    int64_t graph_bytes = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<Request>> queue;
+    std::unordered_map<int64_t, std::shared_ptr<Request>> live;
+    std::vector<std::thread> workers;
+    std::vector<apspie_ctx *> ctxs;
+    int64_t next_ticket = 1;
+    int busy = 0;
+    bool stop = false;
+};
+
+namespace {
+
+int64_t result_bytes(const apspie_server *s, int kind, int32_t graph) {
+    return kind == APSPIE_Q_TC ? (int64_t)sizeof(uint64_t) : 4 * (int64_t)s->graphs[graph].m;
+}
+
+// run one coalesced batch; on a budget failure split it
+void execute(apspie_server *s, apspie_ctx *ctx, std::vector<std::shared_ptr<Request>> &batch) {
+    const Resident &R = s->graphs[batch[0]->graph];
+    const int kind = batch[0]->kind;
+    const size_t n = batch.size();
+    int st = APSPIE_OK;
+    if (kind == APSPIE_Q_TC) {
+        uint64_t total = 0;
+        st = apspie_tc(ctx, R.g, &total, nullptr, APSPIE_MEM_HOST);
+        for (auto &r : batch) {
+            r->result.resize(sizeof total);
+            std::memcpy(r->result.data(), &total, sizeof total);
+        }
+    } else if (n == 1) {
+        auto &r = batch[0];
+        r->result.resize(4 * (size_t)R.m);
+        st = kind == APSPIE_Q_BFS
+                 ? apspie_bfs(ctx, R.g, r->source, (int32_t *)r->result.data(), APSPIE_MEM_HOST)
+                 : apspie_sssp(ctx, R.g, r->source, (float *)r->result.data(), APSPIE_MEM_HOST);
+    } else {
+        std::vector<int32_t> src(n);
+        for (size_t i = 0; i < n; ++i) src[i] = batch[i]->source;
+        std::vector<uint8_t> out(4 * (size_t)R.m * n);
+        st = kind == APSPIE_Q_BFS
+                 ? apspie_bfs_batch(ctx, R.g, src.data(), (int32_t)n, (int32_t *)out.data(),
+                                    APSPIE_MEM_HOST)
+                 : apspie_sssp_batch(ctx, R.g, src.data(), (int32_t)n, (float *)out.data(),
+                                     APSPIE_MEM_HOST);
+        if (st == APSPIE_ERR_BUDGET || st == APSPIE_ERR_OOM) {
+            apspie_ctx_trim(ctx);
+            std::vector<std::shared_ptr<Request>> lo(batch.begin(), batch.begin() + n / 2),
+                hi(batch.begin() + n / 2, batch.end());
+            execute(s, ctx, lo);
+            execute(s, ctx, hi);
+            return;
+        }
+        if (st == APSPIE_OK)
+            for (size_t i = 0; i < n; ++i)
+                batch[i]->result.assign(out.begin() + 4 * (size_t)R.m * i,
+                                        out.begin() + 4 * (size_t)R.m * (i + 1));
+    }
+    const auto now = Clock::now();
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (auto &r : batch) {
+        r->status = st;
+        r->t_done = now;
+        r->done = true;
+    }
+    s->cv_done.notify_all();
+}
+
+void worker(apspie_server *s, int slot) {
+    apspie_ctx *ctx = s->ctxs[slot];
+    for (;;) {
+        std::vector<std::shared_ptr<Request>> batch;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
+            if (s->stop && s->queue.empty()) return;
+            auto head = s->queue.front();
+            s->queue.pop_front();
+            batch.push_back(head);
+            const int cap = head->kind == APSPIE_Q_TC ? INT32_MAX : std::max(1, s->cfg.max_batch);
+            for (auto it = s->queue.begin(); it != s->queue.end() && (int)batch.size() < cap;) {
+                if ((*it)->graph == head->graph && (*it)->kind == head->kind) {
+                    batch.push_back(*it);
+                    it = s->queue.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+            ++s->busy;
+        }
+        execute(s, ctx, batch);
+        std::lock_guard<std::mutex> lk(s->mu);
+        --s->busy;
+        s->cv_done.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int apspie_server_create(int device, const apspie_server_config *cfg, apspie_server **out) {
+    if (!cfg || !out || cfg->slots < 1 || cfg->slots > 16 || cfg->max_batch < 1 ||
+        cfg->hbm_budget < 0)
+        return APSPIE_ERR_INVALID;
+    *out = nullptr;
+    auto *s = new apspie_server();
+    s->device = device;
+    s->cfg = *cfg;
+    int st = apspie_ctx_create(device, &s->admin);
+    if (st != APSPIE_OK) {
+        delete s;
+        return st;
+    }
+    const int64_t scratch_share =
+        cfg->hbm_budget > 0 ? (int64_t)(0.4 * (double)cfg->hbm_budget) / cfg->slots : 0;
+    for (int i = 0; i < cfg->slots; ++i) {
+        apspie_ctx *c = nullptr;
+        st = apspie_ctx_create(device, &c);
+        if (st != APSPIE_OK) break;
+        apspie_ctx_set_hbm_budget(c, scratch_share);
+        s->ctxs.push_back(c);
+    }
+    if (st != APSPIE_OK) {
+        for (auto *c : s->ctxs) apspie_ctx_destroy(c);
+        apspie_ctx_destroy(s->admin);
+        delete s;
+        return st;
+    }
+    for (int i = 0; i < cfg->slots; ++i) s->workers.emplace_back(worker, s, i);
+    *out = s;
+    return APSPIE_OK;
+}
+
+int apspie_server_destroy(apspie_server *s) {
+    if (!s) return APSPIE_ERR_INVALID;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (auto &t : s->workers) t.join();
+    for (auto &r : s->graphs) apspie_graph_destroy(r.g);
+    for (auto *c : s->ctxs) apspie_ctx_destroy(c);
+    apspie_ctx_destroy(s->admin);
+    delete s;
+    return APSPIE_OK;
+}
+
+int apspie_server_add_graph(apspie_server *s, const apspie_csr *host, int32_t *graph_id) {
+    if (!s || !host || !graph_id) return APSPIE_ERR_INVALID;
+    const int64_t bytes = 4 * ((int64_t)host->m + 1) + (host->val ? 8 : 4) * host->nnz;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        // resident graphs may take 60% of the budget; the rest is worker scratch
+        if (s->cfg.hbm_budget > 0 && s->graph_bytes + bytes > (int64_t)(0.6 * s->cfg.hbm_budget))
+            return APSPIE_ERR_BUDGET;
+    }
+    apspie_graph *g = nullptr;
+    int st = apspie_graph_upload(s->admin, host, &g);
+    if (st != APSPIE_OK) return st;
+    st = apspie::guard(s->admin, [&] { apspie::graph_prepare_tc(s->admin, g); });
+    if (st != APSPIE_OK && st != APSPIE_ERR_INVALID) {  // unsorted rows: TC fails later
+        apspie_graph_destroy(g);
+        return st;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->graphs.push_back({g, host->m, bytes});
+    s->graph_bytes += bytes;
+    *graph_id = (int32_t)s->graphs.size() - 1;
+    return APSPIE_OK;
+}
+
+int apspie_server_submit(apspie_server *s, int32_t graph_id, int kind, int32_t source,
+                         int64_t *ticket) {
+    if (!s || !ticket || kind < APSPIE_Q_BFS || kind > APSPIE_Q_TC) return APSPIE_ERR_INVALID;
+    auto r = std::make_shared<Request>();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (graph_id < 0 || graph_id >= (int32_t)s->graphs.size()) return APSPIE_ERR_INVALID;
+        if (kind != APSPIE_Q_TC && (source < 0 || source >= s->graphs[graph_id].m))
+            return APSPIE_ERR_INVALID;
+        if (s->cfg.queue_capacity > 0 && (int)s->queue.size() >= s->cfg.queue_capacity)
+            return APSPIE_ERR_BUSY;
+        r->ticket = s->next_ticket++;
+        r->graph = graph_id;
+        r->kind = kind;
+        r->source = source;
+        r->t_submit = Clock::now();
+        s->queue.push_back(r);
+        s->live[r->ticket] = r;
+        *ticket = r->ticket;
+    }
+    s->cv_work.notify_one();
+    return APSPIE_OK;
+}
+
+int apspie_server_wait(apspie_server *s, int64_t ticket, void *out, int64_t out_bytes,
+                       double *latency_ms) {
+    if (!s) return APSPIE_ERR_INVALID;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->live.find(ticket);
+    if (it == s->live.end()) return APSPIE_ERR_INVALID;
+    auto r = it->second;
+    s->cv_done.wait(lk, [&] { return r->done; });
+    s->live.erase(ticket);
+    if (latency_ms)
+        *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+    if (r->status != APSPIE_OK) return r->status;
+    const int64_t need = result_bytes(s, r->kind, r->graph);
+    if (out) {
+        if (out_bytes < need) return APSPIE_ERR_INVALID;
+        std::memcpy(out, r->result.data(), (size_t)need);
+    }
+    return APSPIE_OK;
+}
+
+int apspie_server_drain(apspie_server *s) {
+    if (!s) return APSPIE_ERR_INVALID;
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->queue.empty() && s->busy == 0; });
+    return APSPIE_OK;
+}
+
+}  // extern "C"

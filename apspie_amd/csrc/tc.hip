@@ -1,0 +1,321 @@
+// Triangle counting as the masked sparse product L . L o L.
+//
+// Reference: the TC driver builds the strictly-lower and strictly-upper
+// operands (buildLower, test/tc/main.cu:39-55), multiplies them with
+// cuSPARSE (spgemm(), tc.cuh:62-165), intersects the product with the mask
+// (ewiseMultTc, tc.cuh:168-199) and reduces the per-column counts in float
+// (test/tc/main.cu:249).  Here the product is never materialised: for every
+// row r of L (the columns of row r below r, a prefix of the sorted row)
+//     per_row[r] = sum_{k in L_r} |L_r cap L_k|
+// = triangles whose largest vertex is r.  Rows are binned by |L_r|:
+//   * small rows: one wave per row, L_r in a 256-slot LDS hash table;
+//   * medium rows: one block per row, 4096-slot LDS table;
+//   * large rows: one block per row, table in global memory (spill).
+// The wedges (k, j), j in L_k, of a row are enumerated load-balanced: an
+// exclusive scan of |L_k| over k in L_r (the symbolic row count of L.L) maps
+// each lane to its wedge; each wedge is one probe of the hash table.  Counts
+// are integers: per_row int32, the total an exact uint64.
+#include <algorithm>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "timing.hpp"
+
+namespace apspie {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kSmallRow = 128;     // wave bin: |L_r| <= 128, table 256
+constexpr int kSmallTable = 256;
+constexpr int kMediumRow = 2048;   // block bin: |L_r| <= 2048, table 4096
+constexpr int kMediumTable = 4096;
+
+__device__ __forceinline__ uint32_t hash32(uint32_t x) { return x * 2654435769u; }
+
+// lowcnt[r] = number of entries of row r below the diagonal (rows sorted)
+__global__ void k_lowcnt(const int32_t *rowptr, const int32_t *col, int32_t m, int32_t *lowcnt) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < m;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        int32_t lo = rowptr[r], hi = rowptr[r + 1];
+        const int32_t b = lo;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (col[mid] < (int32_t)r) lo = mid + 1; else hi = mid;
+        }
+        lowcnt[r] = lo - b;
+    }
+}
+
+// rows with sorted columns? flag[0] stays 0 if all are
+__global__ void k_check_sorted(const int32_t *rowptr, const int32_t *col, int32_t m,
+                               int32_t *flag) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < m;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        for (int32_t e = rowptr[r] + 1; e < rowptr[r + 1]; ++e)
+            if (col[e - 1] >= col[e]) {
+                flag[0] = 1;
+                break;
+            }
+    }
+}
+
+// bins: [0] small list, [1] medium, [2] large; ctl[0..2] counts, ctl[3] max |L_r|
+__global__ void k_tc_bin(const int32_t *lowcnt, int32_t m, int32_t *small, int32_t *medium,
+                         int32_t *large, uint32_t *ctl, int32_t *per_row) {
+    for (int64_t r0 = blockIdx.x * (int64_t)blockDim.x; r0 < m; r0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = r0 + threadIdx.x;
+        int32_t n = r < m ? lowcnt[r] : 0;
+        int bin = n < 2 ? -1 : (n <= kSmallRow ? 0 : (n <= kMediumRow ? 1 : 2));
+        if (r < m && bin < 0 && per_row) per_row[r] = 0;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const uint64_t mask = __ballot(bin == b);
+            if (!mask) continue;
+            const int leader = __ffsll((unsigned long long)mask) - 1;
+            uint32_t base = 0;
+            if (dev::lane_id() == leader) base = dev::atomic_add_dev(ctl + b, (uint32_t)__popcll(mask));
+            base = __shfl(base, leader, 64);
+            if (bin == b) {
+                const uint32_t slot = base + __popcll(mask & dev::lanemask_lt());
+                int32_t *list = b == 0 ? small : (b == 1 ? medium : large);
+                list[slot] = (int32_t)r;
+            }
+        }
+        if (bin == 2) atomicMax(ctl + 3, (uint32_t)n);
+    }
+}
+
+__device__ __forceinline__ void table_insert(int32_t *table, uint32_t mask, int32_t key) {
+    uint32_t h = hash32((uint32_t)key) & mask;
+    for (;;) {
+        const int32_t prev = atomicCAS(table + h, -1, key);
+        if (prev == -1 || prev == key) return;
+        h = (h + 1) & mask;
+    }
+}
+
+__device__ __forceinline__ bool table_has(const int32_t *table, uint32_t mask, int32_t key) {
+    uint32_t h = hash32((uint32_t)key) & mask;
+    for (;;) {
+        const int32_t t = table[h];
+        if (t == key) return true;
+        if (t == -1) return false;
+        h = (h + 1) & mask;
+    }
+}
+
+// one wave per small row
+__global__ __launch_bounds__(kBlock) void k_tc_small(const int32_t *rowptr, const int32_t *col, int32_t nnz,
+                                                     const int32_t *lowcnt, const int32_t *rows,
+                                                     const uint32_t *cnt, int32_t *per_row,
+                                                     uint64_t *total, uint64_t *wedges) {
+    constexpr int NW = kBlock / 64;
+    __shared__ int32_t s_table[NW][kSmallTable];
+    __shared__ int32_t s_pre[NW][kSmallRow + 1];
+    __shared__ int32_t s_rs[NW][kSmallRow];
+    const int w = threadIdx.x / 64, lane = dev::lane_id();
+    const int32_t nrows = (int32_t)cnt[0];
+    uint64_t my_tri = 0, my_wedges = 0;
+    for (int32_t i = blockIdx.x * NW + w; i < nrows; i += gridDim.x * NW) {
+        const int32_t r = rows[i];
+        const int32_t n = lowcnt[r];
+        const int32_t base = rowptr[r];
+        for (int s = lane; s < kSmallTable; s += 64) s_table[w][s] = -1;
+        dev::wave_sync();
+        uint32_t carry = 0;
+        for (int c0 = 0; c0 < n; c0 += 64) {
+            const int idx = c0 + lane;
+            uint32_t lc = 0;
+            if (idx < n) {
+                const int32_t k = col[base + idx];
+                table_insert(s_table[w], kSmallTable - 1, k);
+                s_rs[w][idx] = rowptr[k];
+                lc = (uint32_t)lowcnt[k];
+            }
+            const uint32_t inc = dev::wave_inclusive_add(lc);
+            if (idx < n) s_pre[w][idx] = (int32_t)(carry + inc - lc);
+            carry += dev::wave_read_lane(inc, 63);
+        }
+        if (lane == 0) s_pre[w][n] = (int32_t)carry;
+        dev::wave_sync();
+        const int32_t W = (int32_t)carry;
+        uint32_t c = 0;
+        for (int32_t x = lane; x < W; x += 64) {
+            const int j = dev::upper_bound_m1(s_pre[w], 0, n, x);
+            const int32_t e = s_rs[w][j] + (x - s_pre[w][j]);
+            if ((uint32_t)e >= (uint32_t)nnz) continue;  // guard only
+            c += table_has(s_table[w], kSmallTable - 1, col[e]) ? 1u : 0u;
+        }
+        const uint64_t rc = dev::wave_sum_u64(c);
+        if (lane == 0) per_row[r] = (int32_t)rc;
+        my_tri += rc;
+        my_wedges += (uint64_t)W;
+        dev::wave_sync();
+    }
+    if (lane == 0 && (my_tri || my_wedges)) {
+        dev::atomic_add_dev(total, my_tri);
+        dev::atomic_add_dev(wedges, my_wedges);
+    }
+}
+
+// one block per medium / large row; TABLE > 0: LDS table of that size,
+// TABLE == 0: per-block region of global scratch (spill)
+template <int TABLE>
+__global__ __launch_bounds__(kBlock) void k_tc_block(const int32_t *rowptr, const int32_t *col, int32_t nnz,
+                                                     const int32_t *lowcnt, const int32_t *rows,
+                                                     const uint32_t *cnt, int32_t *per_row,
+                                                     uint64_t *total, uint64_t *wedges,
+                                                     int32_t *g_table, int32_t *g_pre,
+                                                     int32_t *g_rs, int64_t g_cap) {
+    constexpr int LDS_T = TABLE > 0 ? TABLE : 1;
+    constexpr int LDS_R = TABLE > 0 ? TABLE / 2 : 1;
+    __shared__ int32_t s_table[LDS_T];
+    __shared__ int32_t s_pre[LDS_R + 1];
+    __shared__ int32_t s_rs[LDS_R];
+    __shared__ uint32_t s_scan[kBlock / 64 + 1];
+    __shared__ uint64_t s_red[kBlock / 64];
+    int32_t *table = TABLE > 0 ? s_table : g_table + (int64_t)blockIdx.x * g_cap;
+    int32_t *pre = TABLE > 0 ? s_pre : g_pre + (int64_t)blockIdx.x * (g_cap / 2 + 1);
+    int32_t *rsk = TABLE > 0 ? s_rs : g_rs + (int64_t)blockIdx.x * (g_cap / 2);
+    const int32_t nrows = (int32_t)cnt[0];
+    uint64_t my_tri = 0, my_wedges = 0;
+    for (int32_t i = blockIdx.x; i < nrows; i += gridDim.x) {
+        const int32_t r = rows[i];
+        const int32_t n = lowcnt[r];
+        const int32_t base = rowptr[r];
+        uint32_t tsize = 1;
+        while (tsize < 2u * (uint32_t)n) tsize <<= 1;
+        if (TABLE > 0) tsize = TABLE;
+        const uint32_t mask = tsize - 1;
+        for (uint32_t s = threadIdx.x; s < tsize; s += kBlock) table[s] = -1;
+        __syncthreads();
+        uint32_t carry = 0;
+        for (int c0 = 0; c0 < n; c0 += kBlock) {
+            const int idx = c0 + threadIdx.x;
+            uint32_t lc = 0;
+            if (idx < n) {
+                const int32_t k = col[base + idx];
+                table_insert(table, mask, k);
+                rsk[idx] = rowptr[k];
+                lc = (uint32_t)lowcnt[k];
+            }
+            uint32_t tot;
+            const uint32_t ex = dev::block_exclusive_add32<kBlock>(lc, s_scan, tot);
+            if (idx < n) pre[idx] = (int32_t)(carry + ex);
+            carry += tot;
+        }
+        if (threadIdx.x == 0) pre[n] = (int32_t)carry;
+        __syncthreads();
+        const int32_t W = (int32_t)carry;
+        uint32_t c = 0;
+        for (int32_t x = threadIdx.x; x < W; x += kBlock) {
+            const int j = dev::upper_bound_m1(pre, 0, n, x);
+            const int32_t e = rsk[j] + (x - pre[j]);
+            if ((uint32_t)e >= (uint32_t)nnz) continue;  // guard only
+            c += table_has(table, mask, col[e]) ? 1u : 0u;
+        }
+        const uint64_t wc = dev::wave_sum_u64(c);
+        if (dev::lane_id() == 0) s_red[threadIdx.x / 64] = wc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t rc = 0;
+            for (int q = 0; q < kBlock / 64; ++q) rc += s_red[q];
+            per_row[r] = (int32_t)rc;
+            my_tri += rc;
+            my_wedges += (uint64_t)W;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && (my_tri || my_wedges)) {
+        dev::atomic_add_dev(total, my_tri);
+        dev::atomic_add_dev(wedges, my_wedges);
+    }
+}
+
+}  // namespace
+
+// Per-row count of entries below the diagonal, cached with the graph (built
+// once, under the graph's lock, by whichever context needs it first).
+void graph_prepare_tc(Ctx *c, Graph *g) {
+    std::lock_guard<std::mutex> lock(g->derived_mu);
+    if (g->lowcnt || g->m == 0) return;
+    const int32_t m = g->m;
+    const int grid = fill_grid(c, kBlock, 4);
+    uint32_t *ctl = (uint32_t *)scratch(c, "tc_prep", 64);
+    fill_i32(c, (int32_t *)ctl, 0, 16);
+    {
+        k_check_sorted<<<grid, kBlock, 0, c->stream>>>(g->rowptr, g->col, m, (int32_t *)ctl + 4);
+        uint32_t flag = 0;
+        APSPIE_HIP(hipMemcpyAsync(&flag, ctl + 4, 4, hipMemcpyDeviceToHost, c->stream));
+        sync(c);
+        APSPIE_CHECK(flag == 0, APSPIE_ERR_INVALID, "TC needs sorted, duplicate-free rows");
+        // owned by the graph (freed with it), not charged to this context
+        int32_t *lc = nullptr;
+        APSPIE_HIP(hipMalloc(&lc, sizeof(int32_t) * std::max(m, 1)));
+        k_lowcnt<<<grid, kBlock, 0, c->stream>>>(g->rowptr, g->col, m, lc);
+        APSPIE_LAUNCH_CHECK();
+        sync(c);
+        g->lowcnt = lc;
+    }
+}
+
+void tc_run(Ctx *c, const Graph *g0, uint64_t *total_out, int32_t *per_row) {
+    Graph *g = const_cast<Graph *>(g0);
+    *total_out = 0;
+    const int32_t m = g->m;
+    if (m == 0) return;
+    graph_prepare_tc(c, g);
+    const int grid = fill_grid(c, kBlock, 4);
+    uint32_t *ctl = (uint32_t *)scratch(c, "tc_ctl", 64);
+    uint64_t *acc = (uint64_t *)(ctl + 8);  // [0] triangles, [1] wedges
+    fill_i32(c, (int32_t *)ctl, 0, 16);
+    int32_t *per = per_row ? per_row : (int32_t *)scratch(c, "tc_rows", sizeof(int32_t) * m);
+    int32_t *lists = (int32_t *)scratch(c, "tc_lists", sizeof(int32_t) * 3 * (size_t)m);
+    k_tc_bin<<<grid, kBlock, 0, c->stream>>>(g->lowcnt, m, lists, lists + m, lists + 2 * (size_t)m,
+                                            ctl, per);
+    APSPIE_LAUNCH_CHECK();
+    uint32_t h[4];
+    APSPIE_HIP(hipMemcpyAsync(h, ctl, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    c->stats.host_syncs++;
+    KernelTimer kt(c);
+    if (h[0]) {
+        const int blocks = (int)std::min<int64_t>(ceil_div(h[0], kBlock / 64), 8 * c->cus);
+        kt.begin();
+        k_tc_small<<<blocks, kBlock, 0, c->stream>>>(g->rowptr, g->col, (int32_t)g->nnz, g->lowcnt, lists, ctl + 0,
+                                                     per, acc, acc + 1);
+        kt.end();
+        APSPIE_LAUNCH_CHECK();
+    }
+    if (h[1]) {
+        const int blocks = (int)std::min<int64_t>(h[1], 4 * c->cus);
+        k_tc_block<kMediumTable><<<blocks, kBlock, 0, c->stream>>>(
+            g->rowptr, g->col, (int32_t)g->nnz, g->lowcnt, lists + m, ctl + 1, per, acc, acc + 1, nullptr, nullptr,
+            nullptr, 0);
+        APSPIE_LAUNCH_CHECK();
+    }
+    if (h[2]) {
+        int64_t cap = 1;
+        while (cap < 2 * (int64_t)h[3]) cap <<= 1;
+        const int blocks = (int)std::min<int64_t>(h[2], 2 * c->cus);
+        int32_t *gt = (int32_t *)scratch(c, "tc_gtable", sizeof(int32_t) * cap * blocks);
+        int32_t *gp = (int32_t *)scratch(c, "tc_gpre", sizeof(int32_t) * (cap / 2 + 1) * blocks);
+        int32_t *gr = (int32_t *)scratch(c, "tc_grs", sizeof(int32_t) * (cap / 2) * blocks);
+        k_tc_block<0><<<blocks, kBlock, 0, c->stream>>>(g->rowptr, g->col, (int32_t)g->nnz, g->lowcnt,
+                                                        lists + 2 * (size_t)m, ctl + 2, per, acc,
+                                                        acc + 1, gt, gp, gr, cap);
+        APSPIE_LAUNCH_CHECK();
+    }
+    uint64_t res[2];
+    APSPIE_HIP(hipMemcpyAsync(res, acc, sizeof res, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    c->stats.host_syncs++;
+    *total_out = res[0];
+    c->stats.edges_traversed = (int64_t)res[1];  // wedges probed
+    c->stats.vertices_reached = (int64_t)h[0] + h[1] + h[2];
+    c->stats.iterations = 1;
+    kt.collect();
+}
+
+}  // namespace apspie

@@ -29,18 +29,23 @@ class KernelTimer {
         if (!on_) return;
         APSPIE_HIP(hipEventRecord(ev_.back().second, c_->stream));
     }
-    // sums the recorded intervals into the context stats (after a sync)
-    void collect() {
+    // sums the first `limit` recorded intervals (all if negative) into the
+    // context stats; launches past the limit are the early-exit ones issued
+    // after the work ran out
+    void collect(int64_t limit = -1) {
         if (!on_) return;
         double total = 0;
+        int64_t n = 0;
         for (auto &p : ev_) {
+            if (limit >= 0 && n >= limit) break;
             APSPIE_HIP(hipEventSynchronize(p.second));
             float ms = 0;
             APSPIE_HIP(hipEventElapsedTime(&ms, p.first, p.second));
             total += ms;
+            ++n;
         }
         c_->stats.kernel_ms += total;
-        c_->stats.kernel_launches += (int64_t)ev_.size();
+        c_->stats.kernel_launches += n;
     }
 
   private:

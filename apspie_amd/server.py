@@ -1,0 +1,70 @@
+"""Python front end of the continuous-batching query server (apspie_server_*
+in include/apspie.h)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import Q_BFS, Q_SSSP, Q_TC, _Csr, _ServerCfg, _check, _p, lib
+
+__all__ = ["Server"]
+
+
+class Server:
+    """Resident graph pool + request queue + `slots` worker streams."""
+
+    def __init__(self, device=0, slots=2, max_batch=64, hbm_budget=0, queue_capacity=0):
+        self._L = lib()
+        cfg = _ServerCfg(max_batch, slots, int(hbm_budget), queue_capacity)
+        h = C.c_void_p()
+        _check(self._L.apspie_server_create(device, C.byref(cfg), C.byref(h)))
+        self.h = h
+        self._keep = []  # host arrays must outlive the upload call only
+
+    def close(self):
+        if self.h:
+            self._L.apspie_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_graph(self, rowptr, col, val=None):
+        rowptr = np.ascontiguousarray(rowptr, np.int32)
+        col = np.ascontiguousarray(col, np.int32)
+        val = None if val is None else np.ascontiguousarray(val, np.float32)
+        m = len(rowptr) - 1
+        csr = _Csr(m, m, len(col), _p(rowptr).value, _p(col).value if len(col) else None,
+                   None if val is None else _p(val).value)
+        gid = C.c_int32()
+        _check(self._L.apspie_server_add_graph(self.h, C.byref(csr), C.byref(gid)))
+        return gid.value
+
+    def submit(self, graph_id, kind, source=0):
+        t = C.c_int64()
+        _check(self._L.apspie_server_submit(self.h, graph_id, kind, int(source), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket, kind, m):
+        """Returns (result, latency_ms): int32[m] depths, float32[m] distances or an int."""
+        lat = C.c_double()
+        if kind == Q_TC:
+            out = C.c_uint64()
+            _check(self._L.apspie_server_wait(self.h, ticket, C.byref(out), 8, C.byref(lat)))
+            return int(out.value), lat.value
+        out = np.empty(m, np.int32 if kind == Q_BFS else np.float32)
+        _check(self._L.apspie_server_wait(self.h, ticket, _p(out), out.nbytes, C.byref(lat)))
+        return out, lat.value
+
+    def drain(self):
+        _check(self._L.apspie_server_drain(self.h))

@@ -1,0 +1,195 @@
+"""Benchmark of the hot path: single-source BFS (Boolean SpMSpV + visited
+mask) on R-MAT scale 22 (4,194,304 vertices, edge factor 16, a/b/c/d =
+.57/.19/.19/.05, seed 1, symmetric, int32 CSR), 64 non-isolated sources from
+mt19937(3), each run separately with the graph resident in HBM.
+
+A step is one BFS from one source (the 64 sources are cycled).  The line
+printed is the whole-job GTEPS: sum over steps of the edges traversed
+(sum of out-degrees of the reached vertices) / timed wall time.
+
+    python bench.py [--gpus 1] [--steps 64] [--warmup 4] [--no-extra]
+
+Extra fields (SSSP GTEPS, batched/served throughput) are measured after the
+main line's timed region unless --no-extra is given.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+import apspie_amd as A  # noqa: E402
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def sources(rowptr, k, seed=3):
+    """k distinct vertices with degree > 0: v = draw % m over std::mt19937(seed)."""
+    bg = np.random.MT19937()
+    bg._legacy_seeding(seed)
+    draws = bg.random_raw(4096 * k).astype(np.uint64)
+    m = len(rowptr) - 1
+    deg = np.diff(rowptr)
+    out, seen = [], set()
+    for d in draws.tolist():
+        v = d % m
+        if deg[v] > 0 and v not in seen:
+            seen.add(v)
+            out.append(int(v))
+            if len(out) == k:
+                return out
+    raise RuntimeError("not enough sources")
+
+
+
+
+def read_traffic(path: Path):
+    """HBM bytes per launch of the expansion kernel, from the committed
+    rocprofv3 --pmc summary (line 'traffic_bytes_per_launch: N')."""
+    if not path.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", path.read_text())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(rowptr, col, srcs, budget_s=15.0):
+    """The oracle's restatement of SimpleReferenceBfs (testBfs.cpp:6-72), one
+    core, on a bounded sample of the same sources."""
+    from oracle import oracle as O
+    rp64 = rowptr.astype(np.int64)
+    m = len(rowptr) - 1
+    deg = np.diff(rowptr)
+    edges, t_tot, n = 0, 0.0, 0
+    for s in srcs:
+        t0 = time.perf_counter()
+        d, _ = O.bfs(m, rp64, col, s)
+        t_tot += time.perf_counter() - t0
+        edges += int(deg[d >= 0].sum())
+        n += 1
+        if t_tot > budget_s:
+            break
+    return {"value": edges / t_tot / 1e9, "unit": "GTEPS", "cores": 1, "kind": "port",
+            "sample": f"oracle deque BFS (SimpleReferenceBfs restated) from the first {n} of "
+                      f"the 64 sources on the same RMAT-22 graph, {t_tot:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sharded "
+              "design is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    t0 = time.time()
+    rowptr, col = A.gen_rmat(args.scale)
+    m, nnz = len(rowptr) - 1, len(col)
+    log(f"rmat{args.scale}: m={m} nnz={nnz} built in {time.time() - t0:.1f}s")
+    srcs = sources(rowptr, 64)
+
+    ctx = A.Context(0)
+    G = ctx.upload(rowptr, col)
+    out = ctx.dev_alloc(4 * m)
+    optr = out.ptr
+    deg = np.diff(rowptr)
+
+    for i in range(args.warmup):
+        ctx.bfs(G, srcs[i % 64], out=optr, mem=A.DEVICE)
+    ctx.sync()
+
+    # ---- timed region: K single-source BFS runs
+    edges = 0
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        ctx.bfs(G, srcs[i % 64], out=optr, mem=A.DEVICE)
+        edges += ctx.stats()["edges_traversed"]
+    ctx.sync()
+    elapsed = time.perf_counter() - t_start
+    gteps = edges / elapsed / 1e9
+
+    # ---- dominant kernel (the level expansion), timed with HIP events on the
+    # context stream, over the same sources, outside the timed region
+    ctx.set_option("profiling", 1)
+    k_ms, k_launch, alg_bytes = 0.0, 0, 0.0
+    for i in range(min(args.steps, 64)):
+        s = srcs[i % 64]
+        ctx.bfs(G, s, out=optr, mem=A.DEVICE)
+        st = ctx.stats()
+        k_ms += st["kernel_ms"]
+        k_launch += st["kernel_launches"]
+        # 8 B per traversed edge (column index + visited probe), 20 B per
+        # reached vertex (row-offset pair, depth write, frontier write+read)
+        alg_bytes += 8.0 * st["edges_traversed"] + 20.0 * st["vertices_reached"]
+    ctx.set_option("profiling", 0)
+    achieved = alg_bytes / (k_ms / 1e3)
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                "traffic": read_traffic(ROOT / "profiles" / "bfs_expand_rmat22_pmc.md"),
+                "kernel": "k_expand<BfsOp> (one BFS level)",
+                "avg_launch_us": 1e3 * k_ms / max(k_launch, 1),
+                "alg_bytes_per_launch": alg_bytes / max(k_launch, 1)}
+
+    rec = {"metric": "GTEPS (BFS, RMAT-22, single source, 64 sources)", "value": gteps,
+           "unit": "GTEPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+           "data": "synthetic (R-MAT generator, seed 1)",
+           "config": {"workload": f"bfs_rmat{args.scale}_64src", "scale": args.scale,
+                      "edge_factor": 16, "abcd": [0.57, 0.19, 0.19, 0.05], "vertices": m,
+                      "nnz": nnz, "sources": 64, "source_rng": "mt19937(3) % m, degree > 0"},
+           "roofline": roofline,
+           "graph500_gteps": gteps / 2}
+
+    if not args.no_extra:
+        rec["extra"] = extras(ctx, G, rowptr, col, srcs, optr)
+    if not args.no_cpu:
+        rec["cpu_baseline"] = cpu_baseline(rowptr, col, srcs)
+    else:
+        rec["cpu_baseline"] = None
+    G.close()
+    ctx.close()
+    print(json.dumps(rec), flush=True)
+
+
+def extras(ctx, G, rowptr, col, srcs, optr):
+    """SSSP GTEPS on the same graph and the batched form on a lattice; each
+    bounded to a few seconds."""
+    ex = {}
+    w = A.gen_weights(rowptr, col)
+    Gw = ctx.upload(rowptr, col, w)
+    fout = ctx.dev_alloc(4 * (len(rowptr) - 1))
+    edges, relaxed, t = 0, 0, 0.0
+    for s in srcs[:16]:
+        ctx.bfs(G, s, out=optr, mem=A.DEVICE)  # same reached set -> E_r
+        edges += ctx.stats()["edges_traversed"]
+        ctx.sssp(Gw, s, out=fout.ptr, mem=A.DEVICE)
+        st = ctx.stats()
+        relaxed += st["edges_traversed"]
+        t += st["elapsed_ms"]
+    ex["sssp_rmat22"] = {"gteps": edges / (t / 1e3) / 1e9, "ms_per_source": t / 16,
+                         "sources": 16, "expanded_edges_per_traversed": relaxed / edges}
+    Gw.close()
+    return ex
+
+
+if __name__ == "__main__":
+    main()

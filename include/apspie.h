@@ -104,6 +104,17 @@ const char *apspie_last_error(apspie_ctx *ctx);
 const char *apspie_status_str(int status);
 int apspie_abi_version(void);
 
+/* Device memory for callers without their own HIP plumbing: replaces the
+ * drivers' cudaMalloc / cudaMemcpy / cudaFree of the CSR and result arrays
+ * (test/bfs/main.cu:105-120, 147-152).  kind: 0 host->device,
+ * 1 device->host, 2 device->device.  Memory from apspie_dev_alloc counts
+ * against the context's HBM budget. */
+int apspie_dev_alloc(apspie_ctx *ctx, int64_t bytes, void **ptr);
+int apspie_dev_free(apspie_ctx *ctx, void *ptr, int64_t bytes);
+int apspie_memcpy(apspie_ctx *ctx, void *dst, const void *src, int64_t bytes, int kind);
+/* Wait for all work queued on the context's stream. */
+int apspie_ctx_sync(apspie_ctx *ctx);
+
 /* Counters from the last algorithm call on this context. */
 typedef struct {
     int64_t edges_traversed; /* sum of out-degrees of expanded frontier vertices */

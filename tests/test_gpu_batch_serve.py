@@ -1,0 +1,135 @@
+"""Batched multi-source traversal, the autotuner and the query server on the
+GPU.  Every batched / served result must equal the single-source answer of
+the oracle for that source, alone and in the mix."""
+import numpy as np
+import pytest
+
+import apspie_amd as A
+from apspie_amd import server as S
+from oracle import oracle as O
+from tests import graphs as T
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    with A.Context(0) as c:
+        yield c
+
+
+def oracle_sssp(g, s):
+    d = O.sssp(g.m, g.rowptr, g.col, g.w, s)
+    return np.where(d == -1, A.SSSP_INF, d).astype(np.float32)
+
+
+@pytest.mark.parametrize("nsrc", [1, 5, 32, 33, 100, 128, 200])
+@pytest.mark.parametrize("key", ["rmat12", "lattice64"])
+def test_bfs_batch_vs_oracle(ctx, key, nsrc):
+    g = T.rmat(12) if key == "rmat12" else T.lattice(64)
+    G = ctx.upload(g.rowptr32, g.col, g.w)
+    src = T.sources(g, nsrc)
+    if nsrc > 3:
+        src[3] = src[1]  # a repeated source
+    got = ctx.bfs_batch(G, src)
+    for i, s in enumerate(src):
+        want, _ = O.bfs(g.m, g.rowptr, g.col, s)
+        assert np.array_equal(got[i], want), (key, nsrc, i, s)
+    G.close()
+
+
+@pytest.mark.parametrize("nsrc", [1, 7, 32, 64, 150])
+@pytest.mark.parametrize("key", ["rmat11", "lattice32"])
+def test_sssp_batch_vs_oracle(ctx, key, nsrc):
+    g = T.rmat(11) if key == "rmat11" else T.lattice(32)
+    G = ctx.upload(g.rowptr32, g.col, g.w)
+    src = T.sources(g, nsrc)
+    got = ctx.sssp_batch(G, src)
+    for i, s in enumerate(src):
+        assert np.array_equal(got[i], oracle_sssp(g, s)), (key, nsrc, i, s)
+    G.close()
+
+
+@pytest.mark.parametrize("graph_on,lps", [(0, 6), (1, 6), (1, 48)])
+def test_batch_graph_replay_matches(ctx, graph_on, lps):
+    g = T.lattice(64)  # ~100 levels: many chunk replays
+    ctx.set_option("batch_graph", graph_on)
+    ctx.set_option("bfs_levels_per_sync", lps)
+    try:
+        G = ctx.upload(g.rowptr32, g.col, g.w)
+        src = T.sources(g, 40)
+        got = ctx.bfs_batch(G, src)
+        gs = ctx.sssp_batch(G, src[:9])
+        for i, s in enumerate(src):
+            assert np.array_equal(got[i], O.bfs(g.m, g.rowptr, g.col, s)[0])
+        for i, s in enumerate(src[:9]):
+            assert np.array_equal(gs[i], oracle_sssp(g, s))
+        G.close()
+    finally:
+        ctx.set_option("batch_graph", 1)
+        ctx.set_option("bfs_levels_per_sync", 8)
+
+
+def test_batch_many_sources_split(ctx):
+    g = T.rmat(10)
+    G = ctx.upload(g.rowptr32, g.col)
+    src = (T.sources(g, 300) * 2)[:600]  # > 512: two passes
+    got = ctx.bfs_batch(G, src)
+    for i in (0, 299, 511, 512, 599):
+        assert np.array_equal(got[i], O.bfs(g.m, g.rowptr, g.col, src[i])[0])
+    G.close()
+
+
+def test_autotune_sets_valid_options():
+    g = T.rmat(12)
+    with A.Context(0) as c:
+        G = c.upload(g.rowptr32, g.col)
+        c.autotune(G, 0)
+        assert c.get_option("bfs_tile") in (1024, 2048, 4096)
+        assert c.get_option("bfs_blocks_per_cu") in (2, 4, 8)
+        c.autotune(G, 32)
+        assert c.get_option("batch_blocks_per_cu") in (2, 4, 8)
+        src = T.sources(g, 4)
+        for s in src:
+            assert np.array_equal(c.bfs(G, s), O.bfs(g.m, g.rowptr, g.col, s)[0])
+        G.close()
+
+
+def test_server_mixed_stream_equals_alone():
+    graphs = [T.rmat(12), T.lattice(64), T.fixture("chesapeake")]
+    with S.Server(slots=3, max_batch=16, hbm_budget=1 << 30) as srv:
+        ids = [srv.add_graph(g.rowptr32, g.col, g.w) for g in graphs]
+        rng = np.random.default_rng(0)
+        reqs = []
+        for _ in range(200):
+            gi = int(rng.integers(0, len(graphs)))
+            r = rng.random()
+            kind = A.Q_BFS if r < 0.6 else (A.Q_SSSP if r < 0.9 else A.Q_TC)
+            src = int(rng.choice(np.nonzero(graphs[gi].deg)[0]))
+            reqs.append((gi, kind, src, srv.submit(ids[gi], kind, src)))
+        lat = []
+        for gi, kind, src, t in reqs:
+            res, ms = srv.wait(t, kind, graphs[gi].m)
+            lat.append(ms)
+            g = graphs[gi]
+            if kind == A.Q_BFS:
+                assert np.array_equal(res, O.bfs(g.m, g.rowptr, g.col, src)[0])
+            elif kind == A.Q_SSSP:
+                assert np.array_equal(res, oracle_sssp(g, src))
+            else:
+                assert res == O.tc(g.m, g.rowptr, g.col)[0]
+        assert min(lat) > 0
+        srv.drain()
+
+
+def test_server_budget_and_bad_requests():
+    g = T.rmat(12)
+    with S.Server(slots=1, max_batch=4, hbm_budget=100_000) as srv:
+        with pytest.raises(A.ApspieError) as e:
+            srv.add_graph(g.rowptr32, g.col)
+        assert e.value.status == -4
+        gi = srv.add_graph(T.fixture("test_bc").rowptr32, T.fixture("test_bc").col)
+        with pytest.raises(A.ApspieError):
+            srv.submit(gi, A.Q_BFS, 99)
+        with pytest.raises(A.ApspieError):
+            srv.submit(gi + 5, A.Q_BFS, 0)

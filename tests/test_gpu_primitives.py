@@ -1,0 +1,204 @@
+"""The SpMSpV pipeline primitives and the general hash SpGEMM on the GPU:
+scan, stable radix sort, SpMSpV under the four semirings on both execution
+paths, SpGEMM, and device-side graph ingestion / transpose, each against the
+oracle or a closed form."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import apspie_amd as A
+from oracle import oracle as O
+from tests import graphs as T
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    with A.Context(0) as c:
+        yield c
+
+
+@pytest.mark.parametrize("n", [0, 1, 5, 2047, 2048, 2049, 100_000, 3_000_001])
+def test_scan_exclusive(ctx, n):
+    rng = np.random.default_rng(n)
+    x = rng.integers(0, 100, n).astype(np.int32)
+    out, tot = ctx.scan_exclusive(x)
+    want = np.concatenate([[0], np.cumsum(x, dtype=np.int64)[:-1]]) if n else np.zeros(0)
+    assert np.array_equal(out, want.astype(np.int32)) and tot == int(x.sum())
+
+
+@pytest.mark.parametrize("n,bits", [(1, 8), (100, 8), (5000, 13), (1_000_003, 22), (300_000, 32)])
+def test_radix_sort_stable(ctx, n, bits):
+    rng = np.random.default_rng(bits)
+    hi = (1 << bits) if bits < 32 else (1 << 32)
+    keys = rng.integers(0, min(hi, 5000 if n > 1000 else hi), n, dtype=np.uint64).astype(np.uint32)
+    vals = np.arange(n, dtype=np.uint32)
+    k, v = ctx.sort_pairs(keys, vals, bits)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(k, keys[order]) and np.array_equal(v, vals[order])
+    k2, _ = ctx.sort_pairs(keys, None, bits)
+    assert np.array_equal(k2, keys[order])
+
+
+SEMIRINGS = [A.SR_PLUS_TIMES, A.SR_BOOL, A.SR_MIN_PLUS, A.SR_MAX_FIRST]
+
+
+@pytest.mark.parametrize("path", [A.PATH_SORT, A.PATH_ATOMIC, A.PATH_AUTO])
+@pytest.mark.parametrize("sr", SEMIRINGS)
+def test_spmspv_vs_oracle(ctx, sr, path):
+    rng = np.random.default_rng(sr * 7 + path)
+    for g in (T.rmat(12), T.fixture("chesapeake"), T.lattice(32)):
+        a = rng.integers(1, 9, len(g.col)).astype(np.float32)
+        G = ctx.upload(g.rowptr32, g.col, a)
+        for nx in (1, 17, g.m // 3):
+            x_ind = np.sort(rng.choice(g.m, nx, replace=False)).astype(np.int32)
+            x_val = rng.integers(-20, 50, nx).astype(np.float32)
+            mask = (rng.random(g.m) < 0.25).astype(np.int32)
+            for mk in (None, mask):
+                yi, yv = ctx.spmspv(G, sr, x_ind, x_val, mk, path)
+                wi, wv = O.spmspv(g.m, g.rowptr, g.col, a, sr, x_ind, x_val, mk)
+                assert np.array_equal(yi, wi) and np.array_equal(yv, wv), (g, sr, path, nx)
+        G.close()
+
+
+def test_spmspv_sort_path_is_sequential_fold(ctx):
+    """Non-integer (+,x): the sort path folds each run in expansion order,
+    so it is bit-identical to the oracle's sequential fold."""
+    g = T.rmat(11)
+    rng = np.random.default_rng(0)
+    a = rng.random(len(g.col)).astype(np.float32)
+    G = ctx.upload(g.rowptr32, g.col, a)
+    x_ind = np.sort(rng.choice(g.m, 300, replace=False)).astype(np.int32)
+    x_val = rng.random(300).astype(np.float32)
+    yi, yv = ctx.spmspv(G, A.SR_PLUS_TIMES, x_ind, x_val, None, A.PATH_SORT)
+    wi, wv = O.spmspv(g.m, g.rowptr, g.col, a, A.SR_PLUS_TIMES, x_ind, x_val)
+    assert np.array_equal(yi, wi) and np.array_equal(yv, wv)
+    yi, yv = ctx.spmspv(G, A.SR_PLUS_TIMES, x_ind, x_val, None, A.PATH_ATOMIC)
+    assert np.array_equal(yi, wi) and np.allclose(yv, wv, rtol=1e-5, atol=1e-6)
+    G.close()
+
+
+def test_spmspv_empty_and_zero_degree(ctx):
+    g = T.random_graph(500, 0.002, 9)
+    G = ctx.upload(g.rowptr32, g.col, np.ones(len(g.col), np.float32))
+    zero = np.nonzero(g.deg == 0)[0][:5].astype(np.int32)
+    yi, yv = ctx.spmspv(G, A.SR_BOOL, zero)
+    assert len(yi) == 0
+    yi, yv = ctx.spmspv(G, A.SR_BOOL, np.zeros(0, np.int32))
+    assert len(yi) == 0
+    G.close()
+
+
+def _scipy_csr(g_or_mat):
+    return g_or_mat
+
+
+@pytest.mark.parametrize("case", ["rect", "square_int", "rmat_sq", "hub"])
+def test_spgemm_vs_oracle(ctx, case):
+    rng = np.random.default_rng(3)
+    if case == "rect":
+        Am = sp.random(300, 200, 0.05, random_state=1, format="csr")
+        Bm = sp.random(200, 400, 0.05, random_state=2, format="csr")
+    elif case == "square_int":
+        Am = sp.random(1000, 1000, 0.01, random_state=3, format="csr")
+        Bm = Am.T.tocsr()
+    elif case == "rmat_sq":
+        g = T.rmat(10)
+        Am = sp.csr_matrix((np.ones(len(g.col)), g.col, g.rowptr), shape=(g.m, g.m))
+        Bm = Am
+    else:  # one row with a huge product count (global spill tier)
+        Am = sp.lil_matrix((50, 6000))
+        Am[0, :] = 1
+        Am[1, :40] = 1
+        Am = Am.tocsr()
+        Bm = sp.random(6000, 7000, 0.002, random_state=5, format="csr")
+    for M in (Am, Bm):
+        M.sort_indices()
+        M.data = rng.integers(1, 5, M.nnz).astype(np.float64)
+    Ga = ctx.upload(Am.indptr.astype(np.int32), Am.indices.astype(np.int32),
+                    Am.data.astype(np.float32), n=Am.shape[1])
+    Gb = ctx.upload(Bm.indptr.astype(np.int32), Bm.indices.astype(np.int32),
+                    Bm.data.astype(np.float32), n=Bm.shape[1])
+    Gc = ctx.spgemm(Ga, Gb)
+    rp, cc, vv = Gc.download(values=True)
+    wrp, wc, wv = O.spgemm(Am.shape[0], Am.shape[1], Bm.shape[1], Am.indptr.astype(np.int64),
+                           Am.indices.astype(np.int32), Am.data.astype(np.float32),
+                           Bm.indptr.astype(np.int64), Bm.indices.astype(np.int32),
+                           Bm.data.astype(np.float32))
+    assert np.array_equal(rp, wrp.astype(np.int32))
+    assert np.array_equal(cc, wc) and np.array_equal(vv, wv)
+    for G in (Ga, Gb, Gc):
+        G.close()
+
+
+@pytest.mark.parametrize("name", T.FIXTURES)
+def test_graph_from_coo_matches_host_build(ctx, name):
+    path = T.GOLDEN_DIR / "mtx" / f"{name}.mtx"
+    m, n, row, col, val = A.read_mtx(path)
+    nv = max(m, n)
+    G = ctx.from_coo(nv, row, col, val, symmetrize=True)
+    rp, cc, _ = G.download()
+    g = T.fixture(name)
+    assert np.array_equal(rp, g.rowptr32) and np.array_equal(cc, g.col)
+    G.close()
+
+
+def test_graph_from_coo_values_and_buildmatrix_order(ctx):
+    row = np.array([0, 1, 2, 1, 0, 2], np.int32)
+    col = np.array([1, 0, 2, 2, 1, 0], np.int32)
+    val = np.array([3, 7, 9, 4, 8, 5], np.float32)
+    G = ctx.from_coo(3, row, col, val, symmetrize=True)
+    rp, cc, vv = G.download(values=True)
+    r2, c2, v2 = O.symmetrize(np.concatenate([row, np.zeros(6, np.int32)]),
+                              np.concatenate([col, np.zeros(6, np.int32)]),
+                              np.concatenate([val, np.zeros(6, np.float32)]), 6)
+    wrp, wc, wv = O.coo_to_csr(3, r2, c2, v2)
+    assert np.array_equal(rp, wrp.astype(np.int32)) and np.array_equal(cc, wc)
+    assert np.array_equal(vv, wv)
+    G.close()
+    # no symmetrize: buildMatrix keeps input order inside a row, loops and repeats
+    G = ctx.from_coo(3, row, col, val, symmetrize=False)
+    rp, cc, vv = G.download(values=True)
+    wrp, wc, wv = O.coo_to_csr(3, row, col, val)
+    assert np.array_equal(rp, wrp.astype(np.int32)) and np.array_equal(cc, wc)
+    assert np.array_equal(vv, wv)
+    G.close()
+
+
+def test_graph_from_coo_large(ctx):
+    s, d = O.rmat_edges(14)
+    G = ctx.from_coo(1 << 14, s, d, None, symmetrize=True)
+    rp, cc, _ = G.download()
+    g = T.rmat(14)
+    assert np.array_equal(rp, g.rowptr32) and np.array_equal(cc, g.col)
+    G.close()
+
+
+def test_transpose(ctx):
+    Mx = sp.random(500, 300, 0.02, random_state=4, format="csr")
+    Mx.sort_indices()
+    Mx.data = np.arange(1, Mx.nnz + 1, dtype=np.float64)
+    G = ctx.upload(Mx.indptr.astype(np.int32), Mx.indices.astype(np.int32),
+                   Mx.data.astype(np.float32), n=300)
+    T_ = G.transpose()
+    rp, cc, vv = T_.download(values=True)
+    Wt = Mx.T.tocsr()
+    Wt.sort_indices()
+    assert np.array_equal(rp, Wt.indptr) and np.array_equal(cc, Wt.indices)
+    assert np.array_equal(vv, Wt.data.astype(np.float32))
+    G.close()
+    T_.close()
+
+
+def test_hbm_budget_is_enforced(ctx):
+    g = T.rmat(12)
+    with A.Context(0) as c2:
+        c2.set_hbm_budget(1000)
+        with pytest.raises(A.ApspieError) as e:
+            c2.upload(g.rowptr32, g.col)
+        assert e.value.status == -4
+        c2.set_hbm_budget(0)
+        G = c2.upload(g.rowptr32, g.col)
+        assert c2.hbm_used() >= G.hbm_bytes > 0
+        G.close()

@@ -1,0 +1,159 @@
+"""SSSP, MIS and triangle counting on the GPU through the C ABI, bit-exact
+against the oracle (testSssp.cpp Dijkstra restated, greedy MIS closed form,
+ewiseMultTc semantics) and the golden fixtures."""
+import numpy as np
+import pytest
+
+import apspie_amd as A
+from oracle import oracle as O
+from tests import graphs as T
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    with A.Context(0) as c:
+        yield c
+
+
+def to_ref_sentinel(d):
+    """oracle marks unreached with -1 (testSssp.cpp:49); the GPU path with
+    the reference sentinel 1.70141e38 (sssp.cuh:48)."""
+    return np.where(d == -1, A.SSSP_INF, d).astype(np.float32)
+
+
+# ---------------------------------------------------------------------- SSSP
+@pytest.mark.parametrize("name", T.FIXTURES)
+def test_sssp_fixtures(ctx, name):
+    g = T.fixture(name)
+    G = ctx.upload(g.rowptr32, g.col, g.w)
+    got = ctx.sssp(G, 0)
+    want = to_ref_sentinel(np.array(T.GOLD["fixtures"][name]["sssp"], np.float32))
+    assert np.array_equal(got, want)
+    G.close()
+
+
+@pytest.mark.parametrize("key", ["rmat10", "rmat14", "lattice32", "lattice64"])
+def test_sssp_synthetic_vs_oracle(ctx, key):
+    g = T.rmat(int(key[4:])) if key.startswith("rmat") else T.lattice(int(key[7:]))
+    G = ctx.upload(g.rowptr32, g.col, g.w)
+    for src in [T.GOLD[key]["source"]] + T.sources(g, 4):
+        want = to_ref_sentinel(O.sssp(g.m, g.rowptr, g.col, g.w, src))
+        got = ctx.sssp(G, src)
+        assert np.array_equal(got, want), (key, src)
+    G.close()
+
+
+def test_sssp_edge_cases(ctx):
+    g = T.path_graph(200)
+    w = np.full(len(g.col), 3, np.float32)
+    G = ctx.upload(g.rowptr32, g.col, w)
+    assert ctx.sssp(G, 0).tolist() == [3.0 * i for i in range(200)]
+    G.close()
+    g = T.empty_graph(5)
+    G = ctx.upload(g.rowptr32, g.col, np.zeros(0, np.float32))
+    got = ctx.sssp(G, 2)
+    assert got[2] == 0 and np.all(np.delete(got, 2) == A.SSSP_INF)
+    G.close()
+    g = T.fixture("test_bc")
+    G = ctx.upload(g.rowptr32, g.col)  # no weights
+    with pytest.raises(A.ApspieError):
+        ctx.sssp(G, 0)
+    G.close()
+
+
+# ----------------------------------------------------------------------- MIS
+@pytest.mark.parametrize("name", T.FIXTURES)
+def test_mis_fixtures(ctx, name):
+    g = T.fixture(name)
+    G = ctx.upload(g.rowptr32, g.col)
+    got = ctx.mis(G)  # default priorities: mt19937(5489)
+    assert got.tolist() == T.GOLD["fixtures"][name]["mis"]
+    G.close()
+
+
+@pytest.mark.parametrize("key", ["rmat10", "rmat14", "lattice64"])
+def test_mis_synthetic_vs_oracle(ctx, key):
+    g = T.rmat(int(key[4:])) if key.startswith("rmat") else T.lattice(int(key[7:]))
+    G = ctx.upload(g.rowptr32, g.col)
+    prio = O.mis_priorities(g.m)
+    want = O.mis_greedy(g.m, g.rowptr, g.col, prio)
+    assert np.array_equal(ctx.mis(G), want)
+    assert np.array_equal(ctx.mis(G, prio), want)
+    # other priorities, including heavy ties (tie -> lower id wins)
+    for seed in (7, 11):
+        p = O.mt_stream(seed, g.m)
+        assert np.array_equal(ctx.mis(G, p), O.mis_greedy(g.m, g.rowptr, g.col, p))
+    p = (O.mt_stream(3, g.m) % 4).astype(np.uint32)
+    assert np.array_equal(ctx.mis(G, p), O.mis_greedy(g.m, g.rowptr, g.col, p))
+    G.close()
+
+
+def test_mis_properties_and_isolated(ctx):
+    g = T.random_graph(3000, 0.001, 5)  # has isolated vertices
+    G = ctx.upload(g.rowptr32, g.col)
+    s = ctx.mis(G)
+    rows = np.repeat(np.arange(g.m), np.diff(g.rowptr))
+    assert set(np.unique(s).tolist()) <= {0, 1}
+    assert not np.any((s[rows] == 1) & (s[g.col] == 1))
+    cov = np.zeros(g.m, bool)
+    cov[g.col[s[rows] == 1]] = True
+    assert np.all((s == 1) | cov)
+    assert np.all(s[g.deg == 0] == 1)
+    G.close()
+
+
+# ------------------------------------------------------------------------ TC
+@pytest.mark.parametrize("name", T.FIXTURES)
+def test_tc_fixtures(ctx, name):
+    g = T.fixture(name)
+    G = ctx.upload(g.rowptr32, g.col)
+    total, rows = ctx.tc(G)
+    fx = T.GOLD["fixtures"][name]
+    assert total == fx["triangles"] == fx["nx_triangles"]
+    assert rows.tolist() == fx["tc_rows"]
+    G.close()
+
+
+@pytest.mark.parametrize("key", ["rmat10", "rmat14", "lattice64"])
+def test_tc_synthetic_vs_oracle(ctx, key):
+    g = T.rmat(int(key[4:])) if key.startswith("rmat") else T.lattice(int(key[7:]))
+    G = ctx.upload(g.rowptr32, g.col)
+    want_total, want_rows = O.tc(g.m, g.rowptr, g.col)
+    total, rows = ctx.tc(G)
+    assert total == want_total == T.GOLD[key]["triangles"]
+    assert np.array_equal(rows, want_rows)
+    t2, _ = ctx.tc(G, per_row=False)
+    assert t2 == total
+    G.close()
+
+
+def test_tc_all_bins():
+    """Dense blocks push rows into the block and global-spill tiers."""
+    from itertools import combinations
+    for n in (40, 200, 2500, 5000):
+        u, v = zip(*combinations(range(n), 2)) if n <= 200 else (None, None)
+        if n <= 200:
+            rp, c = O.edges_to_csr(n, np.array(u, np.int32), np.array(v, np.int32))
+        else:  # the last 300 vertices form a clique adjacent to every other vertex
+            uu = np.repeat(np.arange(n - 300, n), n - 300)
+            vv = np.tile(np.arange(n - 300), 300)
+            hu, hv = zip(*combinations(range(n - 300, n), 2))
+            uu = np.concatenate([uu, hu])
+            vv = np.concatenate([vv, hv])
+            rp, c = O.edges_to_csr(n, uu.astype(np.int32), vv.astype(np.int32))
+        with A.Context(0) as ctx:
+            G = ctx.upload(rp.astype(np.int32), c)
+            total, rows = ctx.tc(G)
+            wt, wr = O.tc(n, rp, c)
+            assert total == wt and np.array_equal(rows, wr), n
+
+
+def test_tc_rejects_unsorted_rows(ctx):
+    rp = np.array([0, 2, 3, 4], np.int32)
+    col = np.array([2, 1, 0, 0], np.int32)  # row 0 unsorted
+    G = ctx.upload(rp, col)
+    with pytest.raises(A.ApspieError):
+        ctx.tc(G)
+    G.close()
