@@ -60,9 +60,24 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, OUT)
+    build_driver()
     if verbose:
-        print(f"built {OUT} from {len(srcs)} sources")
+        print(f"built {OUT} from {len(srcs)} sources, and {DRIVER}")
     return OUT
+
+
+DRIVER = ROOT / "tests" / "cpp" / "mirror_driver"
+
+
+def build_driver() -> Path:
+    """The C++ driver that exercises include/apspie.hpp (host code only)."""
+    src = DRIVER.with_suffix(".cpp")
+    cmd = ["g++", "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(src), "-o", str(DRIVER),
+           f"-L{PKG}", "-lapspie", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../../apspie_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return DRIVER
 
 
 def clean():

@@ -3,10 +3,25 @@
 // Semantics follow bfs() (bfs.cuh:32-125) / spmspvBfs() (spmspvBfs.cuh:96-243):
 // depth[src] = 0, depth[v] = level of first discovery, -1 if unreached.  The
 // reference runs expand -> sort -> reduce -> scatter -> addResult per level
-// with host round trips; here one kernel per level does the whole SpMSpV:
-// load-balanced expansion of the frontier, visited filter (Boolean semiring
-// plus mask: the first writer of a bit wins), depth write and the append of
-// the discovered vertices into the next frontier with their degree offsets.
+// with host round trips; here one kernel per level does the whole SpMSpV.
+//
+// Two directions of the same product y = A^T x (Boolean semiring, masked by
+// the visited set):
+//   push (SpMSpV, "top-down"): load-balanced expansion of the frontier's
+//     rows; a fetch-or on the visited bitmap decides the first writer of each
+//     bit, which writes the depth and appends the vertex to the next frontier.
+//   pull (masked SpMV, "bottom-up", the dense-vector direction the reference
+//     has as SpmvCsrBinary, bfs.cuh:11-14): every unvisited vertex scans its
+//     row until it finds a frontier vertex.  A wave owns 64 consecutive
+//     vertices, i.e. two bitmap words, so the visited and next-frontier
+//     bitmaps are written without atomics.
+// Every level leaves the next frontier in both forms (compacted queue with
+// degree offsets for push, bitmap for pull), and both kernels of a level read
+// the same device counters to pick the direction (Beamer's rule: pull when
+// the frontier's edges exceed the unexplored edges / alpha while growing,
+// push again when the frontier falls under m / beta while shrinking), so no
+// host round trip is needed per level.  The result does not depend on the
+// direction: a vertex's depth is its level of first discovery either way.
 #include "common.hpp"
 #include "device.hpp"
 #include "frontier.hpp"
@@ -17,34 +32,88 @@ namespace apspie {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr uint32_t kPush = 0, kPull = 1;
+
+// ctl layout (uint64 words): [0..2] frontier counters (ring), [4] edges
+// traversed, [5] entries expanded, [6] levels, [7] edges scanned by pull,
+// [8..9] (mode << 32 | entries) per level parity, [10..11] explored edges
+// before the level, per level parity.
+struct Ctl {
+    uint64_t *w;
+    __device__ uint64_t *cnt(int l) const { return w + (l % 3); }
+    __device__ uint64_t *slot(int l) const { return w + 8 + (l & 1); }
+    __device__ uint64_t *expl(int l) const { return w + 10 + (l & 1); }
+};
+
+struct DoParams {
+    int32_t direction;  // 0 push only, 1 direction-optimising, 2 pull whenever possible
+    int32_t alpha, beta;
+    int64_t nnz;
+    int32_t m;
+};
+
+// Direction of level l, computed identically by every block of both kernels
+// from values written by the previous level.
+__device__ __forceinline__ uint32_t decide(const Ctl &c, int l, const DoParams &p, int32_t F,
+                                           int32_t E) {
+    if (p.direction == 0) return kPush;
+    if (p.direction == 2) return l == 0 ? kPush : kPull;
+    const uint64_t prev = *c.slot(l + 1);
+    const uint32_t pmode = uint32_t(prev >> 32), pF = uint32_t(prev);
+    const int64_t unexplored = p.nnz - (int64_t)*c.expl(l + 1) - E;
+    if (pmode == kPush)
+        return ((int64_t)E * p.alpha > unexplored && (uint32_t)F > pF) ? kPull : kPush;
+    return ((int64_t)F * p.beta < p.m && (uint32_t)F < pF) ? kPush : kPull;
+}
+
+// bookkeeping done once per level by block 0 of the active kernel
+__device__ __forceinline__ void level_books(const Ctl &c, int l, uint32_t mode, int32_t F,
+                                            int32_t E) {
+    *c.cnt(l + 2) = 0;
+    *c.slot(l) = (uint64_t(mode) << 32) | uint32_t(F);
+    *c.expl(l) = *c.expl(l + 1) + (uint64_t)E;
+    c.w[4] += (uint64_t)E;
+    c.w[5] += (uint64_t)F;
+    c.w[6] += 1;
+}
+
+__device__ __forceinline__ void clear_slice(uint32_t *p, int64_t words) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < words;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
 
 struct BfsOp {
-    uint32_t *bitmap;
+    uint32_t *vis;
     int32_t *depth;
+    uint32_t *fb_next;  // next-frontier bitmap (null when only pushing)
     int32_t nd;
     __device__ __forceinline__ bool edge(int32_t, int32_t, int32_t v) const {
-        uint32_t *w = bitmap + (v >> 5);
+        uint32_t *w = vis + (v >> 5);
         const uint32_t bit = 1u << (v & 31);
         if (*w & bit) return false;  // seen (possibly stale: the atomic decides)
         const uint32_t old = dev::atomic_or_dev(w, bit);
         if (old & bit) return false;
         depth[v] = nd;
+        if (fb_next)
+            __hip_atomic_fetch_or(fb_next + (v >> 5), bit, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
         return true;
     }
 };
 
 template <int TILE>
-__global__ void k_bfs_seed(const int32_t *rowptr, int32_t src, int32_t *depth,
-                           uint32_t *bitmap, FrontierBufs f0, uint64_t *ctl) {
+__global__ void k_bfs_seed(const int32_t *rowptr, int32_t src, int32_t *depth, uint32_t *vis,
+                           uint32_t *fb0, FrontierBufs f0, uint64_t *ctl) {
     const int32_t rs = rowptr[src];
     const int32_t d = rowptr[src + 1] - rs;
     if (threadIdx.x == 0) {
         depth[src] = 0;
-        bitmap[src >> 5] = 1u << (src & 31);
+        vis[src >> 5] = 1u << (src & 31);
+        if (fb0) fb0[src >> 5] = 1u << (src & 31);
+        for (int i = 0; i < 16; ++i) ctl[i] = 0;
         ctl[0] = d > 0 ? ((uint64_t(1) << 32) | uint64_t(d)) : 0;
-        ctl[1] = 0;
-        ctl[2] = 0;
-        for (int i = 4; i < 12; ++i) ctl[i] = 0;
+        ctl[9] = uint64_t(kPush) << 32;  // "level -1": pushing, empty
         f0.start[0] = rs;
         f0.off[0] = 0;
         if (f0.v) f0.v[0] = src;
@@ -52,16 +121,118 @@ __global__ void k_bfs_seed(const int32_t *rowptr, int32_t src, int32_t *depth,
     for (int32_t t = threadIdx.x; (int64_t)t * TILE < d; t += blockDim.x) f0.tfirst[t] = 0;
 }
 
+// push level of the direction-optimising loop
+template <int TILE>
+__global__ __launch_bounds__(kBlock) void k_bfs_push(const int32_t *__restrict__ rowptr,
+                                                     const int32_t *__restrict__ col,
+                                                     int32_t *depth, uint32_t *vis,
+                                                     uint32_t *fb_next, uint32_t *fb_clear,
+                                                     FrontierBufs cur, FrontierBufs nxt,
+                                                     uint64_t *ctlw, int32_t level, DoParams p) {
+    const Ctl c{ctlw};
+    const uint64_t cc = *c.cnt(level);
+    const int32_t F = (int32_t)cnt_entries(cc), E = (int32_t)cnt_edges(cc);
+    if (F == 0) {
+        // keep the counter ring clean after the last level: the entry two
+        // slots ahead still holds an old frontier and would revive it
+        if (blockIdx.x == 0 && threadIdx.x == 0) *c.cnt(level + 2) = 0;
+        return;
+    }
+    const uint32_t mode = decide(c, level, p, F, E);
+    if (mode != kPush) return;
+    __syncthreads();  // every thread has read the shared state before block 0 writes
+    if (blockIdx.x == 0 && threadIdx.x == 0) level_books(c, level, mode, F, E);
+    if (fb_clear) clear_slice(fb_clear, p.m / 32 + 1);
+    BfsOp op{vis, depth, fb_next, level + 1};
+    expand_tiles<kBlock, TILE, false, false, true>(op, rowptr, col, (int32_t)p.nnz, cur, F, E,
+                                                   nxt, c.cnt(level + 1));
+}
+
+// pull level: one thread per vertex, 64 consecutive vertices per wave
+template <int TILE>
+__global__ __launch_bounds__(kBlock) void k_bfs_pull(const int32_t *__restrict__ rowptr,
+                                                     const int32_t *__restrict__ col,
+                                                     int32_t *depth, uint32_t *vis,
+                                                     const uint32_t *__restrict__ fb_cur,
+                                                     uint32_t *fb_next, uint32_t *fb_clear,
+                                                     FrontierBufs nxt, uint64_t *ctlw,
+                                                     int32_t level, DoParams p) {
+    __shared__ uint64_t s_scan[kBlock / 64 + 1];
+    __shared__ uint64_t s_base;
+    __shared__ uint64_t s_scanned[kBlock / 64];
+    const Ctl c{ctlw};
+    const uint64_t cc = *c.cnt(level);
+    const int32_t F = (int32_t)cnt_entries(cc), E = (int32_t)cnt_edges(cc);
+    if (F == 0) return;
+    const uint32_t mode = decide(c, level, p, F, E);
+    if (mode != kPull) return;
+    __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x == 0) level_books(c, level, mode, F, E);
+    clear_slice(fb_clear, p.m / 32 + 1);
+    const int lane = dev::lane_id();
+    uint64_t scanned = 0;
+    for (int64_t base = (int64_t)blockIdx.x * kBlock; base < p.m;
+         base += (int64_t)gridDim.x * kBlock) {
+        const int32_t v = (int32_t)(base + threadIdx.x);
+        int32_t cv[1] = {v}, crs[1] = {0}, cd[1] = {0};
+        bool hit = false;
+        if (v < p.m && !((vis[v >> 5] >> (v & 31)) & 1u)) {
+            const int32_t rs = rowptr[v], re = rowptr[v + 1];
+            int32_t e = rs;
+            for (; e < re; ++e) {
+                const int32_t u = col[e];
+                if ((fb_cur[u >> 5] >> (u & 31)) & 1u) {
+                    hit = true;
+                    break;
+                }
+            }
+            scanned += (uint64_t)(hit ? e - rs + 1 : re - rs);
+            if (hit) {
+                depth[v] = level + 1;
+                crs[0] = rs;
+                cd[0] = re - rs;
+            }
+        }
+        const uint64_t bm = __ballot(hit);
+        const int64_t wbase = (base + (threadIdx.x & ~63)) >> 5;  // first word of this wave
+        if (lane == 0 && (uint32_t)bm) {
+            fb_next[wbase] = (uint32_t)bm;
+            vis[wbase] |= (uint32_t)bm;
+        }
+        if (lane == 32 && (uint32_t)(bm >> 32)) {
+            fb_next[wbase + 1] = (uint32_t)(bm >> 32);
+            vis[wbase + 1] |= (uint32_t)(bm >> 32);
+        }
+        produce<kBlock, 1, TILE, false>(cv, crs, cd, nxt, c.cnt(level + 1), s_scan, &s_base);
+    }
+    const uint64_t ws = dev::wave_sum_u64(scanned);
+    if (lane == 0) s_scanned[threadIdx.x / 64] = ws;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (int q = 0; q < kBlock / 64; ++q) t += s_scanned[q];
+        if (t) dev::atomic_add_dev(ctlw + 7, t);
+    }
+}
+
 template <int TILE>
 void bfs_levels(Ctx *c, const Graph *g, int32_t source, int32_t *depth) {
+    const int direction = (int)opt(c, "bfs_direction", 1);
+    const bool do_bfs = direction != 0;
     FrontierSet fs = frontier_alloc(c, "bfs", g->m, g->nnz, TILE, false);
-    uint32_t *bitmap = (uint32_t *)scratch(c, "bfs_bitmap", sizeof(uint32_t) * (g->m / 32 + 1));
+    const int64_t words = g->m / 32 + 1;
+    uint32_t *vis = (uint32_t *)scratch(c, "bfs_bitmap", sizeof(uint32_t) * words);
+    uint32_t *fb = do_bfs ? (uint32_t *)scratch(c, "bfs_fb", sizeof(uint32_t) * 3 * words) : nullptr;
     fill_i32(c, depth, -1, g->m);
-    fill_i32(c, (int32_t *)bitmap, 0, g->m / 32 + 1);
-    k_bfs_seed<TILE><<<1, 256, 0, c->stream>>>(g->rowptr, source, depth, bitmap, fs.f[0], fs.ctl);
+    fill_i32(c, (int32_t *)vis, 0, words);
+    if (do_bfs) fill_i32(c, (int32_t *)fb, 0, 3 * words);
+    k_bfs_seed<TILE><<<1, 256, 0, c->stream>>>(g->rowptr, source, depth, vis, fb, fs.f[0], fs.ctl);
     APSPIE_LAUNCH_CHECK();
 
     const int grid = fill_grid(c, kBlock, (int)opt(c, "bfs_blocks_per_cu", 4));
+    const int pull_grid = (int)std::min<int64_t>(ceil_div(g->m, kBlock), 8 * c->cus);
+    const DoParams dp{direction, (int32_t)opt(c, "bfs_alpha", 14), (int32_t)opt(c, "bfs_beta", 24),
+                      g->nnz, g->m};
     int chunk = (int)std::max<int64_t>(1, opt(c, "bfs_levels_per_sync", 8));
     KernelTimer kt(c);
     int level = 0;
@@ -69,18 +240,28 @@ void bfs_levels(Ctx *c, const Graph *g, int32_t source, int32_t *depth) {
         for (int k = 0; k < chunk; ++k, ++level) {
             const int cur = level & 1;
             kt.begin();
-            k_expand<kBlock, TILE, false, false, true>
-                <<<grid, kBlock, 0, c->stream>>>(BfsOp{bitmap, depth, level + 1}, g->rowptr, g->col,
-                                                 (int32_t)g->nnz, fs.f[cur], fs.ctl + (level % 3), fs.f[cur ^ 1],
-                                                 fs.ctl + ((level + 1) % 3),
-                                                 fs.ctl + ((level + 2) % 3), fs.ctl + 4);
+            if (!do_bfs) {
+                k_expand<kBlock, TILE, false, false, true><<<grid, kBlock, 0, c->stream>>>(
+                    BfsOp{vis, depth, nullptr, level + 1}, g->rowptr, g->col, (int32_t)g->nnz,
+                    fs.f[cur], fs.ctl + (level % 3), fs.f[cur ^ 1], fs.ctl + ((level + 1) % 3),
+                    fs.ctl + ((level + 2) % 3), fs.ctl + 4);
+            } else {
+                uint32_t *fcur = fb + (level % 3) * words, *fnext = fb + ((level + 1) % 3) * words,
+                         *fclr = fb + ((level + 2) % 3) * words;
+                k_bfs_push<TILE><<<grid, kBlock, 0, c->stream>>>(g->rowptr, g->col, depth, vis,
+                                                                 fnext, fclr, fs.f[cur],
+                                                                 fs.f[cur ^ 1], fs.ctl, level, dp);
+                k_bfs_pull<TILE><<<pull_grid, kBlock, 0, c->stream>>>(
+                    g->rowptr, g->col, depth, vis, fcur, fnext, fclr, fs.f[cur ^ 1], fs.ctl,
+                    level, dp);
+            }
             kt.end();
             APSPIE_LAUNCH_CHECK();
         }
         uint64_t *h = (uint64_t *)c->pinned;
         APSPIE_HIP(hipMemcpyAsync(h, fs.ctl + (level % 3), sizeof(uint64_t), hipMemcpyDeviceToHost,
                                   c->stream));
-        APSPIE_HIP(hipMemcpyAsync(h + 1, fs.ctl + 4, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+        APSPIE_HIP(hipMemcpyAsync(h + 1, fs.ctl + 4, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost,
                                   c->stream));
         sync(c);
         c->stats.host_syncs++;

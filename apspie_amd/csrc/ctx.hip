@@ -28,7 +28,7 @@ static int64_t option_default(const std::string &n) {
     if (n == "bfs_tile") return 2048;
     if (n == "bfs_blocks_per_cu") return 4;
     if (n == "bfs_levels_per_sync") return 8;
-    if (n == "bfs_direction") return 0;
+    if (n == "bfs_direction") return 1;
     if (n == "bfs_alpha") return 14;
     if (n == "bfs_beta") return 24;
     if (n == "sssp_delta") return 0;
@@ -455,6 +455,17 @@ int apspie_graph_info(const apspie_graph *g, int32_t *m, int32_t *n, int64_t *nn
     if (n) *n = g->n;
     if (nnz) *nnz = g->nnz;
     if (bytes) *bytes = g->bytes;
+    return APSPIE_OK;
+}
+
+int apspie_graph_view(const apspie_graph *g, apspie_csr *v) {
+    if (!g || !v) return APSPIE_ERR_INVALID;
+    v->m = g->m;
+    v->n = g->n;
+    v->nnz = g->nnz;
+    v->rowptr = g->rowptr;
+    v->col = g->col;
+    v->val = g->val;
     return APSPIE_OK;
 }
 

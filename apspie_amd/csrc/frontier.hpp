@@ -85,31 +85,19 @@ __device__ __forceinline__ void produce(const int32_t (&cv)[K], const int32_t (&
 //       per expanded edge (u -> v, e = CSR position); returns true if v is
 //       claimed for the next frontier (each v at most once per level).
 // When PRODUCE is false nothing is appended.
+// The tile loop of the expansion, for kernels that add their own prologue.
+// F / E: entries and expanded edges of `cur` (F > 0).
 template <int BLOCK, int TILE, bool STORE_V_IN, bool STORE_V_OUT, bool PRODUCE, class Op>
-__global__ __launch_bounds__(BLOCK) void k_expand(Op op, const int32_t *__restrict__ rowptr,
-                                                  const int32_t *__restrict__ col, int32_t nnz,
-                                                  FrontierBufs cur, const uint64_t *cnt_cur,
-                                                  FrontierBufs nxt, uint64_t *cnt_next,
-                                                  uint64_t *cnt_clear, uint64_t *stats) {
+__device__ __forceinline__ void expand_tiles(Op &op, const int32_t *__restrict__ rowptr,
+                                             const int32_t *__restrict__ col, int32_t nnz,
+                                             FrontierBufs cur, int32_t F, int32_t E,
+                                             FrontierBufs nxt, uint64_t *cnt_next) {
     constexpr int K = TILE / BLOCK;
     __shared__ int32_t s_off[TILE + 2];
     __shared__ int32_t s_start[TILE + 2];
     __shared__ int32_t s_v[STORE_V_IN ? TILE + 2 : 1];
     __shared__ uint64_t s_scan[BLOCK / 64 + 1];
     __shared__ uint64_t s_base;
-
-    const uint64_t cc = *cnt_cur;
-    const int32_t F = (int32_t)cnt_entries(cc);
-    const int32_t E = (int32_t)cnt_edges(cc);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        if (cnt_clear) *cnt_clear = 0;
-        if (stats && F > 0) {
-            stats[0] += (uint64_t)E;  // edges traversed
-            stats[1] += (uint64_t)F;  // entries expanded
-            stats[2] += 1;            // levels with work
-        }
-    }
-    if (F == 0) return;
     const int32_t ntiles = (int32_t)((E + TILE - 1) / TILE);
     for (int32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         // entries owning this tile; every degree is >= 1, so at most TILE + 1.
@@ -149,6 +137,28 @@ __global__ __launch_bounds__(BLOCK) void k_expand(Op op, const int32_t *__restri
             produce<BLOCK, K, TILE, STORE_V_OUT>(cv, crs, cd, nxt, cnt_next, s_scan, &s_base);
         __syncthreads();  // s_off / s_start are overwritten by the next tile
     }
+}
+
+template <int BLOCK, int TILE, bool STORE_V_IN, bool STORE_V_OUT, bool PRODUCE, class Op>
+__global__ __launch_bounds__(BLOCK) void k_expand(Op op, const int32_t *__restrict__ rowptr,
+                                                  const int32_t *__restrict__ col, int32_t nnz,
+                                                  FrontierBufs cur, const uint64_t *cnt_cur,
+                                                  FrontierBufs nxt, uint64_t *cnt_next,
+                                                  uint64_t *cnt_clear, uint64_t *stats) {
+    const uint64_t cc = *cnt_cur;
+    const int32_t F = (int32_t)cnt_entries(cc);
+    const int32_t E = (int32_t)cnt_edges(cc);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (cnt_clear) *cnt_clear = 0;
+        if (stats && F > 0) {
+            stats[0] += (uint64_t)E;  // edges traversed
+            stats[1] += (uint64_t)F;  // entries expanded
+            stats[2] += 1;            // levels with work
+        }
+    }
+    if (F == 0) return;
+    expand_tiles<BLOCK, TILE, STORE_V_IN, STORE_V_OUT, PRODUCE>(op, rowptr, col, nnz, cur, F, E,
+                                                                 nxt, cnt_next);
 }
 
 // Build a frontier from a vertex predicate over [0, n) or from a vertex

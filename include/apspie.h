@@ -143,6 +143,9 @@ int apspie_graph_from_coo(apspie_ctx *ctx, const apspie_coo *coo, int mem, int s
 int apspie_graph_destroy(apspie_graph *g);
 int apspie_graph_info(const apspie_graph *g, int32_t *m, int32_t *n, int64_t *nnz,
                       int64_t *hbm_bytes);
+/* Device addresses of a resident graph's arrays (the d_cscColPtr /
+ * d_cscRowInd / d_cscVal members of d_matrix, matrix.hpp:25-28). */
+int apspie_graph_view(const apspie_graph *g, apspie_csr *device_view);
 /* Copy the CSR back to host arrays (rowptr m+1, col nnz, val nnz or NULL). */
 int apspie_graph_download(apspie_ctx *ctx, const apspie_graph *g, int32_t *rowptr,
                           int32_t *col, float *val);

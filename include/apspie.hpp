@@ -1,0 +1,244 @@
+// apspie.hpp -- header-only C++ layer over the C ABI (apspie.h) that keeps the
+// reference's entry points: the same names, argument order and meaning as
+// apspie's bfs() / spmspvBfs() / sssp() / spmspvMis() / spgemm() /
+// buildMatrix() / d_matrix, with the mgpu::CudaContext& argument replaced by
+// apspie::Context&.  Existing callers (the test/<alg>/main.cu drivers) switch
+// by including this header instead of the .cuh files.
+//
+// Differences a caller can see:
+//   * the level / round loops run to convergence; the reference's `depth`
+//     argument (a CPU-computed level count, test/bfs/main.cu:129-130) is
+//     accepted and ignored -- with the CPU depth the outputs are the same;
+//   * sssp() uses the real weights in every round (the reference switches to
+//     unit weights after round 1, sssp.cuh:83,91);
+//   * spmspvMis() takes the priorities as given (any positive float order, or
+//     uint32) with ties broken to the lower id; `delta` (the reference's
+//     candidate threshold schedule) changes no result and is ignored;
+//   * errors throw apspie::Error (status + message) instead of exit(); define
+//     APSPIE_EXIT_ON_ERROR to print and exit(EXIT_FAILURE) like CUDA_SAFE_CALL
+//     (mXv.cuh:12-27).
+#ifndef APSPIE_HPP
+#define APSPIE_HPP
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <stdexcept>
+#include <string>
+
+#include "apspie.h"
+
+namespace apspie {
+
+class Error : public std::runtime_error {
+  public:
+    Error(int status, const std::string &msg) : std::runtime_error(msg), status(status) {}
+    int status;
+};
+
+inline void check(int st, apspie_ctx *ctx, const char *what) {
+    if (st == APSPIE_OK) return;
+    std::string msg = std::string(what) + ": " + apspie_status_str(st);
+    if (ctx) {
+        const char *d = apspie_last_error(ctx);
+        if (d && *d) msg += std::string(" (") + d + ")";
+    }
+#ifdef APSPIE_EXIT_ON_ERROR
+    std::fprintf(stderr, "apspie error in %s\n", msg.c_str());
+    std::exit(EXIT_FAILURE);
+#else
+    throw Error(st, msg);
+#endif
+}
+
+// Replaces mgpu::CudaContext (one device, one stream).
+class Context {
+  public:
+    explicit Context(int device = 0) { check(apspie_ctx_create(device, &h_), nullptr, "context"); }
+    ~Context() { if (h_) apspie_ctx_destroy(h_); }
+    Context(const Context &) = delete;
+    Context &operator=(const Context &) = delete;
+    apspie_ctx *get() const { return h_; }
+
+  private:
+    apspie_ctx *h_ = nullptr;
+};
+
+// mgpu::CreateCudaDevice(0) (test/bfs/main.cu:27)
+inline Context *CreateDevice(int device) { return new Context(device); }
+
+// A non-owning graph view over caller-owned device CSR arrays.
+class GraphView {
+  public:
+    GraphView(Context &ctx, int m, int n, int64_t nnz, const int *d_rowptr, const int *d_col,
+              const float *d_val) {
+        apspie_csr c{m, n, nnz, d_rowptr, d_col, d_val};
+        check(apspie_graph_wrap(ctx.get(), &c, &g_), ctx.get(), "graph_wrap");
+    }
+    ~GraphView() { if (g_) apspie_graph_destroy(g_); }
+    GraphView(const GraphView &) = delete;
+    GraphView &operator=(const GraphView &) = delete;
+    const apspie_graph *get() const { return g_; }
+
+  private:
+    apspie_graph *g_ = nullptr;
+};
+
+// d_matrix (matrix.hpp:6-37) in its CSR use: device arrays owned by a
+// resident graph, plus the host row offsets as the reference keeps them.
+struct d_matrix {
+    int m = 0, n = 0, nnz = 0;
+    int *h_cscColPtr = nullptr;  // host row offsets (m+1)
+    int *d_cscColPtr = nullptr;  // device row offsets
+    int *d_cscRowInd = nullptr;  // device column indices
+    float *d_cscVal = nullptr;   // device values
+    apspie_graph *g = nullptr;
+};
+
+inline void matrix_new(d_matrix *A, int m, int n) {
+    A->m = m;
+    A->n = n;
+    A->nnz = 0;
+    A->h_cscColPtr = static_cast<int *>(std::calloc((size_t)m + 1, sizeof(int)));
+}
+
+inline void matrix_delete(d_matrix *A) {
+    if (A->g) apspie_graph_destroy(A->g);
+    std::free(A->h_cscColPtr);
+    *A = d_matrix{};
+}
+
+inline void attach(d_matrix *A, apspie_graph *g, Context &ctx) {
+    if (A->g) apspie_graph_destroy(A->g);
+    A->g = g;
+    apspie_csr v;
+    check(apspie_graph_view(g, &v), ctx.get(), "graph_view");
+    A->m = v.m;
+    A->n = v.n;
+    A->nnz = (int)v.nnz;
+    A->d_cscColPtr = const_cast<int *>(v.rowptr);
+    A->d_cscRowInd = const_cast<int *>(v.col);
+    A->d_cscVal = const_cast<float *>(v.val);
+    std::free(A->h_cscColPtr);
+    A->h_cscColPtr = static_cast<int *>(std::malloc(((size_t)A->m + 1) * sizeof(int)));
+    check(apspie_graph_download(ctx.get(), g, A->h_cscColPtr, nullptr, nullptr), ctx.get(),
+          "graph_download");
+}
+
+// buildMatrix (matrix.cpp:55-111): host COO -> device CSR by a counting sort
+// on the row index that keeps the input order inside a row.
+template <typename T>
+void buildMatrix(d_matrix *A, int numEdge, const int *h_cooRowInd, const int *h_cooColInd,
+                 const T *h_cooVal, Context &ctx) {
+    static_assert(sizeof(T) == sizeof(float), "values are FP32");
+    apspie_coo coo{A->m, A->n, numEdge, h_cooRowInd, h_cooColInd,
+                   reinterpret_cast<const float *>(h_cooVal)};
+    apspie_graph *g = nullptr;
+    check(apspie_graph_from_coo(ctx.get(), &coo, APSPIE_MEM_HOST, 0, &g), ctx.get(), "buildMatrix");
+    attach(A, g, ctx);
+}
+
+// makeSymmetric (util.cuh:571-643) + buildMatrix: mirror, sort, dedup, drop
+// self-loops, CSR with sorted rows.
+template <typename T>
+void buildSymmetricMatrix(d_matrix *A, int numEdge, const int *h_cooRowInd,
+                          const int *h_cooColInd, const T *h_cooVal, Context &ctx) {
+    apspie_coo coo{A->m, A->n, numEdge, h_cooRowInd, h_cooColInd,
+                   reinterpret_cast<const float *>(h_cooVal)};
+    apspie_graph *g = nullptr;
+    check(apspie_graph_from_coo(ctx.get(), &coo, APSPIE_MEM_HOST, 1, &g), ctx.get(),
+          "makeSymmetric");
+    attach(A, g, ctx);
+}
+
+// csr2csc (util.cuh:473-511)
+inline void csr2csc(d_matrix *out, const d_matrix *A, Context &ctx) {
+    apspie_graph *t = nullptr;
+    check(apspie_graph_transpose(ctx.get(), A->g, &t), ctx.get(), "csr2csc");
+    attach(out, t, ctx);
+}
+
+// bfs<T> (bfs.cuh:32-33): d_cscValA is unused by the reference too.
+template <typename T>
+void bfs(int vertex, int edge, int m, const T * /*d_cscValA*/, const int *d_cscColPtrA,
+         const int *d_cscRowIndA, int *d_bfsResult, int /*depth*/, Context &ctx) {
+    GraphView g(ctx, m, m, edge, d_cscColPtrA, d_cscRowIndA, nullptr);
+    check(apspie_bfs(ctx.get(), g.get(), vertex, d_bfsResult, APSPIE_MEM_DEVICE), ctx.get(), "bfs");
+}
+
+// spmspvBfs (spmspvBfs.cuh:96): the keys-only variant; same result.
+inline void spmspvBfs(int vertex, int edge, int m, const int * /*h_csrRowPtr*/,
+                      const int *d_csrRowPtr, const int *d_csrColInd, int *d_bfsResult,
+                      int /*depth*/, Context &ctx) {
+    GraphView g(ctx, m, m, edge, d_csrRowPtr, d_csrColInd, nullptr);
+    check(apspie_bfs(ctx.get(), g.get(), vertex, d_bfsResult, APSPIE_MEM_DEVICE), ctx.get(),
+          "spmspvBfs");
+}
+
+// sssp<V> (sssp.cuh:27-28); unreached = 1.70141e+38 as in the reference.
+template <typename V>
+void sssp(int vertex, int edge, int m, const V *d_csrValA, const int *d_csrRowPtrA,
+          const int *d_csrColIndA, V *d_ssspResult, int /*depth*/, Context &ctx) {
+    static_assert(sizeof(V) == sizeof(float), "weights and distances are FP32");
+    GraphView g(ctx, m, m, edge, d_csrRowPtrA, d_csrColIndA,
+                reinterpret_cast<const float *>(d_csrValA));
+    check(apspie_sssp(ctx.get(), g.get(), vertex, reinterpret_cast<float *>(d_ssspResult),
+                      APSPIE_MEM_DEVICE),
+          ctx.get(), "sssp");
+}
+
+// spmspvMis<V> (spmspvMis.cuh:149-150): d_randVec holds one priority per
+// vertex on the device; positive floats order like their bit patterns, so
+// they are used as uint32 keys directly.  d_misResult: 1 in the set, 0 not.
+template <typename V>
+void spmspvMis(int edge, int m, const int * /*h_csrRowPtr*/, const int *d_csrRowPtr,
+               const int *d_csrColInd, const V *d_randVec, int *d_misResult, float /*delta*/,
+               Context &ctx) {
+    static_assert(sizeof(V) == sizeof(uint32_t), "priorities are 32-bit");
+    GraphView g(ctx, m, m, edge, d_csrRowPtr, d_csrColInd, nullptr);
+    check(apspie_mis(ctx.get(), g.get(), reinterpret_cast<const uint32_t *>(d_randVec),
+                     APSPIE_MEM_DEVICE, d_misResult, APSPIE_MEM_DEVICE),
+          ctx.get(), "spmspvMis");
+}
+
+// Triangle counting, the composition of the TC driver (test/tc/main.cu:
+// 171-175 buildLower, 243 spgemm, 246 ewiseMultTc, 249 Reduce): per-row
+// counts (triangles whose largest vertex is the row) into d_perRow (device,
+// may be null) and the exact total as the return value.
+inline uint64_t tc(int edge, int m, const int *d_csrRowPtr, const int *d_csrColInd,
+                   int *d_perRow, Context &ctx) {
+    GraphView g(ctx, m, m, edge, d_csrRowPtr, d_csrColInd, nullptr);
+    uint64_t total = 0;
+    check(apspie_tc(ctx.get(), g.get(), &total, d_perRow, APSPIE_MEM_DEVICE), ctx.get(), "tc");
+    return total;
+}
+
+// spgemm (spgemm.cuh:132): C = A * B.  partSize / partNum (the reference's
+// 2D tiling of C) are accepted and ignored.
+inline void spgemm(d_matrix *C, d_matrix *A, d_matrix *B, int /*partSize*/, int /*partNum*/,
+                   Context &ctx) {
+    apspie_graph *c = nullptr;
+    check(apspie_spgemm(ctx.get(), A->g, B->g, &c), ctx.get(), "spgemm");
+    attach(C, c, ctx);
+}
+
+// mXv (mXv.cuh:249-250), sparse in / sparse out: y = A^T x under (+,x) on
+// device arrays; returns nnz(y) (the reference's in/out `nnz`).
+template <typename T>
+int mXv(const int *d_xInd, const T *d_xVal, int edge, int m, int nnz_x, const T *d_cscVal,
+        const int *d_cscColPtr, const int *d_cscRowInd, int *d_resultInd, T *d_resultVal,
+        Context &ctx, int semiring = APSPIE_SR_PLUS_TIMES) {
+    GraphView g(ctx, m, m, edge, d_cscColPtr, d_cscRowInd,
+                reinterpret_cast<const float *>(d_cscVal));
+    int64_t ny = 0;
+    check(apspie_spmspv(ctx.get(), g.get(), semiring, nnz_x, d_xInd,
+                        reinterpret_cast<const float *>(d_xVal), nullptr, &ny, d_resultInd,
+                        reinterpret_cast<float *>(d_resultVal), APSPIE_MEM_DEVICE,
+                        APSPIE_PATH_AUTO),
+          ctx.get(), "mXv");
+    return (int)ny;
+}
+
+}  // namespace apspie
+
+#endif

@@ -52,6 +52,30 @@ def test_bfs_synthetic_vs_oracle(ctx, key, tile):
         ctx.set_option("bfs_tile", 2048)
 
 
+@pytest.mark.parametrize("direction", [0, 1, 2])
+@pytest.mark.parametrize("key", ["rmat10", "rmat14", "lattice64", "test_cc", "star"])
+def test_bfs_directions_vs_oracle(ctx, key, direction):
+    """push only (0), direction-optimising (1) and pull whenever possible (2)
+    give the same depths."""
+    if key == "star":
+        g = T.star_graph(3000)
+    elif key == "test_cc":
+        g = T.fixture("test_cc")
+    else:
+        g = T.rmat(int(key[4:])) if key.startswith("rmat") else T.lattice(int(key[7:]))
+    ctx.set_option("bfs_direction", direction)
+    G = ctx.upload(g.rowptr32, g.col)
+    try:
+        for src in T.sources(g, 4, seed=17):
+            want, _ = O.bfs(g.m, g.rowptr, g.col, src)
+            assert np.array_equal(ctx.bfs(G, src), want), (key, direction, src)
+            reached = want >= 0
+            assert ctx.stats()["edges_traversed"] == int(g.deg[reached].sum())
+    finally:
+        G.close()
+        ctx.set_option("bfs_direction", 1)
+
+
 def test_bfs_edge_cases(ctx):
     # star: one hub expands everything in one level; path: one vertex per level
     for g, src in [(T.star_graph(5000), 0), (T.star_graph(5000), 17), (T.path_graph(300), 0),
