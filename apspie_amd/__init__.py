@@ -53,7 +53,7 @@ class Stats(C.Structure):
     _fields_ = [("edges_traversed", C.c_int64), ("vertices_reached", C.c_int64),
                 ("iterations", C.c_int32), ("host_syncs", C.c_int32),
                 ("elapsed_ms", C.c_double), ("kernel_ms", C.c_double),
-                ("kernel_launches", C.c_int64)]
+                ("kernel_launches", C.c_int64), ("edges_checked", C.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

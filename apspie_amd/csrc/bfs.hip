@@ -33,6 +33,9 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr uint32_t kPush = 0, kPull = 1;
+constexpr int32_t kPullLight = 32;  // rows up to this length are scanned by one lane
+constexpr int kPullPer = 8;         // vertices per thread per pull pass
+constexpr int kPullIlp = 8;         // row entries a lane loads at once
 
 // ctl layout (uint64 words): [0..2] frontier counters (ring), [4] edges
 // traversed, [5] entries expanded, [6] levels, [7] edges scanned by pull,
@@ -121,14 +124,35 @@ __global__ void k_bfs_seed(const int32_t *rowptr, int32_t src, int32_t *depth, u
     for (int32_t t = threadIdx.x; (int64_t)t * TILE < d; t += blockDim.x) f0.tfirst[t] = 0;
 }
 
-// push level of the direction-optimising loop
+// Vertices without neighbours start out "visited" (their depth stays -1):
+// pull levels then skip them after one bitmap read.  One thread per vertex,
+// a wave writes its two bitmap words from a ballot.
+__global__ void k_vis_init(const int32_t *rowptr, int32_t m, uint32_t *vis) {
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < m;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = base + threadIdx.x;
+        const bool iso = v < m && rowptr[v] == rowptr[v + 1];
+        const uint64_t bm = __ballot(iso);
+        const int64_t w = (base + (threadIdx.x & ~63)) >> 5;
+        if (dev::lane_id() == 0) vis[w] = (uint32_t)bm;
+        if (dev::lane_id() == 32 && (w + 1) * 32 < m + 32) vis[w + 1] = (uint32_t)(bm >> 32);
+    }
+}
+
+// One level of the direction-optimising loop: every block reads the same
+// counters, picks the same direction and runs either the push expansion or
+// the pull sweep.
+// The level is latency-bound (a chain of dependent loads per wave), so all
+// 8 blocks per CU of the default grid must be resident at once: 8 waves per
+// SIMD, i.e. at most 64 VGPRs.
 template <int TILE>
-__global__ __launch_bounds__(kBlock) void k_bfs_push(const int32_t *__restrict__ rowptr,
-                                                     const int32_t *__restrict__ col,
-                                                     int32_t *depth, uint32_t *vis,
-                                                     uint32_t *fb_next, uint32_t *fb_clear,
-                                                     FrontierBufs cur, FrontierBufs nxt,
-                                                     uint64_t *ctlw, int32_t level, DoParams p) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_bfs_level(
+    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col, int32_t *depth,
+    uint32_t *vis, const uint32_t *__restrict__ fb_cur, uint32_t *fb_next, uint32_t *fb_clear,
+    FrontierBufs cur, FrontierBufs nxt, uint64_t *ctlw, int32_t level, DoParams p) {
+    __shared__ uint64_t s_scan[kBlock / 64 + 1];
+    __shared__ uint64_t s_base;
+    __shared__ uint64_t s_scanned[kBlock / 64];
     const Ctl c{ctlw};
     const uint64_t cc = *c.cnt(level);
     const int32_t F = (int32_t)cnt_entries(cc), E = (int32_t)cnt_edges(cc);
@@ -139,71 +163,155 @@ __global__ __launch_bounds__(kBlock) void k_bfs_push(const int32_t *__restrict__
         return;
     }
     const uint32_t mode = decide(c, level, p, F, E);
-    if (mode != kPush) return;
     __syncthreads();  // every thread has read the shared state before block 0 writes
-    if (blockIdx.x == 0 && threadIdx.x == 0) level_books(c, level, mode, F, E);
-    if (fb_clear) clear_slice(fb_clear, p.m / 32 + 1);
-    BfsOp op{vis, depth, fb_next, level + 1};
-    expand_tiles<kBlock, TILE, false, false, true>(op, rowptr, col, (int32_t)p.nnz, cur, F, E,
-                                                   nxt, c.cnt(level + 1));
-}
-
-// pull level: one thread per vertex, 64 consecutive vertices per wave
-template <int TILE>
-__global__ __launch_bounds__(kBlock) void k_bfs_pull(const int32_t *__restrict__ rowptr,
-                                                     const int32_t *__restrict__ col,
-                                                     int32_t *depth, uint32_t *vis,
-                                                     const uint32_t *__restrict__ fb_cur,
-                                                     uint32_t *fb_next, uint32_t *fb_clear,
-                                                     FrontierBufs nxt, uint64_t *ctlw,
-                                                     int32_t level, DoParams p) {
-    __shared__ uint64_t s_scan[kBlock / 64 + 1];
-    __shared__ uint64_t s_base;
-    __shared__ uint64_t s_scanned[kBlock / 64];
-    const Ctl c{ctlw};
-    const uint64_t cc = *c.cnt(level);
-    const int32_t F = (int32_t)cnt_entries(cc), E = (int32_t)cnt_edges(cc);
-    if (F == 0) return;
-    const uint32_t mode = decide(c, level, p, F, E);
-    if (mode != kPull) return;
-    __syncthreads();
-    if (blockIdx.x == 0 && threadIdx.x == 0) level_books(c, level, mode, F, E);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        level_books(c, level, mode, F, E);
+        if (mode == kPush) c.w[12] += (uint64_t)E;  // entries read by push levels
+    }
     clear_slice(fb_clear, p.m / 32 + 1);
+    if (mode == kPush) {
+        BfsOp op{vis, depth, fb_next, level + 1};
+        expand_tiles<kBlock, TILE, false, false, true>(op, rowptr, col, (int32_t)p.nnz, cur, F,
+                                                       E, nxt, c.cnt(level + 1));
+        return;
+    }
+    // pull: each thread takes kPullPer vertices of a pass; for every k the
+    // block's 256 threads cover 256 consecutive vertices, so a wave owns 64
+    // consecutive vertices (two bitmap words) and writes them without atomics.
+    // All loads of a pass are issued before any of them is needed, and the
+    // claims of the pass are appended with one block-wide reservation.
     const int lane = dev::lane_id();
     uint64_t scanned = 0;
-    for (int64_t base = (int64_t)blockIdx.x * kBlock; base < p.m;
-         base += (int64_t)gridDim.x * kBlock) {
-        const int32_t v = (int32_t)(base + threadIdx.x);
-        int32_t cv[1] = {v}, crs[1] = {0}, cd[1] = {0};
-        bool hit = false;
-        if (v < p.m && !((vis[v >> 5] >> (v & 31)) & 1u)) {
-            const int32_t rs = rowptr[v], re = rowptr[v + 1];
-            int32_t e = rs;
-            for (; e < re; ++e) {
-                const int32_t u = col[e];
-                if ((fb_cur[u >> 5] >> (u & 31)) & 1u) {
-                    hit = true;
-                    break;
+    for (int64_t pbase = (int64_t)blockIdx.x * kBlock * kPullPer; pbase < p.m;
+         pbase += (int64_t)gridDim.x * kBlock * kPullPer) {
+        int32_t cv[kPullPer], crs[kPullPer], cd[kPullPer];
+        int32_t rs[kPullPer], re[kPullPer];
+        bool open[kPullPer], hit[kPullPer];
+#pragma unroll
+        for (int k = 0; k < kPullPer; ++k) {
+            const int64_t v = pbase + (int64_t)k * kBlock + threadIdx.x;
+            cv[k] = (int32_t)v;
+            crs[k] = 0;
+            cd[k] = 0;
+            hit[k] = false;
+            open[k] = v < p.m && !((vis[v >> 5] >> (v & 31)) & 1u);
+        }
+#pragma unroll
+        for (int k = 0; k < kPullPer; ++k) {
+            rs[k] = open[k] ? rowptr[cv[k]] : 0;
+            re[k] = open[k] ? rowptr[cv[k] + 1] : 0;
+        }
+        // first entry of every open row (the lowest-id neighbour: on R-MAT
+        // usually a hub, and most hits happen here)
+        int32_t u0[kPullPer];
+#pragma unroll
+        for (int k = 0; k < kPullPer; ++k) u0[k] = open[k] && re[k] > rs[k] ? col[rs[k]] : -1;
+#pragma unroll
+        for (int k = 0; k < kPullPer; ++k) {
+            if (u0[k] >= 0 && ((fb_cur[u0[k] >> 5] >> (u0[k] & 31)) & 1u)) hit[k] = true;
+            if (open[k]) scanned += (uint64_t)min(re[k] - rs[k], 1);
+        }
+        // the next kPullLight - 1 entries of every row still open: each lane
+        // walks its own rows one after another, eight loads in flight, so a
+        // wave costs the longest lane's total rather than the sum over k of
+        // the longest row of each k
+        uint32_t pend = 0, hitm = 0, longm = 0;
+#pragma unroll
+        for (int k = 0; k < kPullPer; ++k)
+            if (open[k] && !hit[k] && re[k] - rs[k] > 1) pend |= 1u << k;
+        {
+            int32_t e = 0, lim = 0, rend = 0;
+            uint32_t kbit = 0;
+            for (;;) {
+                if (e >= lim) {
+                    if (!pend) break;
+                    const int kk = __ffs(pend) - 1;
+                    pend &= pend - 1;
+                    kbit = 1u << kk;
+                    int32_t r0 = 0;
+#pragma unroll
+                    for (int k = 0; k < kPullPer; ++k)
+                        if (k == kk) {
+                            r0 = rs[k];
+                            rend = re[k];
+                        }
+                    e = r0 + 1;
+                    lim = min(rend, r0 + kPullLight);
+                }
+                int32_t u[kPullIlp];
+#pragma unroll
+                for (int q = 0; q < kPullIlp; ++q) u[q] = e + q < lim ? col[e + q] : -1;
+                uint32_t f[kPullIlp];
+#pragma unroll
+                for (int q = 0; q < kPullIlp; ++q)
+                    f[q] = u[q] >= 0 ? (fb_cur[u[q] >> 5] >> (u[q] & 31)) & 1u : 0u;
+                int at = kPullIlp;
+#pragma unroll
+                for (int q = kPullIlp - 1; q >= 0; --q)
+                    if (f[q]) at = q;
+                if (at < kPullIlp) {
+                    hitm |= kbit;
+                    scanned += (uint64_t)(at + 1);
+                    e = lim;
+                } else {
+                    scanned += (uint64_t)min(kPullIlp, lim - e);
+                    e += kPullIlp;
+                    if (e >= lim && lim < rend) longm |= kbit;
                 }
             }
-            scanned += (uint64_t)(hit ? e - rs + 1 : re - rs);
-            if (hit) {
-                depth[v] = level + 1;
-                crs[0] = rs;
-                cd[0] = re - rs;
+        }
+#pragma unroll
+        for (int k = 0; k < kPullPer; ++k) hit[k] = hit[k] || ((hitm >> k) & 1u);
+        // rows still open after kPullLight entries: the whole wave scans one
+        // row 64 entries at a time, coalesced, stopping at the first hit
+#pragma unroll
+        for (int k = 0; k < kPullPer; ++k) {
+            const int32_t lim = min(re[k], rs[k] + kPullLight);
+            uint64_t heavy = __ballot((longm >> k) & 1u);
+            while (heavy) {
+                const int h = __ffsll((unsigned long long)heavy) - 1;
+                heavy &= heavy - 1;
+                const int32_t hrs = __shfl(lim, h, 64), hre = __shfl(re[k], h, 64);
+                bool found = false;
+                for (int32_t e0 = hrs; e0 < hre; e0 += 64) {
+                    const int32_t e = e0 + lane;
+                    bool f = false;
+                    if (e < hre) {
+                        const int32_t u = col[e];
+                        f = (fb_cur[u >> 5] >> (u & 31)) & 1u;
+                    }
+                    const uint64_t fm = __ballot(f);
+                    if (fm) {
+                        found = true;
+                        if (lane == 0)
+                            scanned += (uint64_t)(e0 - hrs + __ffsll((unsigned long long)fm));
+                        break;
+                    }
+                }
+                if (!found && lane == 0) scanned += (uint64_t)(hre - hrs);
+                if (lane == h) hit[k] = found;
             }
         }
-        const uint64_t bm = __ballot(hit);
-        const int64_t wbase = (base + (threadIdx.x & ~63)) >> 5;  // first word of this wave
-        if (lane == 0 && (uint32_t)bm) {
-            fb_next[wbase] = (uint32_t)bm;
-            vis[wbase] |= (uint32_t)bm;
+#pragma unroll
+        for (int k = 0; k < kPullPer; ++k) {
+            if (hit[k]) {
+                depth[cv[k]] = level + 1;
+                crs[k] = rs[k];
+                cd[k] = re[k] - rs[k];
+            }
+            const uint64_t bm = __ballot(hit[k]);
+            const int64_t wbase = (pbase + (int64_t)k * kBlock + (threadIdx.x & ~63)) >> 5;
+            if (lane == 0 && (uint32_t)bm) {
+                fb_next[wbase] = (uint32_t)bm;
+                vis[wbase] |= (uint32_t)bm;
+            }
+            if (lane == 32 && (uint32_t)(bm >> 32)) {
+                fb_next[wbase + 1] = (uint32_t)(bm >> 32);
+                vis[wbase + 1] |= (uint32_t)(bm >> 32);
+            }
         }
-        if (lane == 32 && (uint32_t)(bm >> 32)) {
-            fb_next[wbase + 1] = (uint32_t)(bm >> 32);
-            vis[wbase + 1] |= (uint32_t)(bm >> 32);
-        }
-        produce<kBlock, 1, TILE, false>(cv, crs, cd, nxt, c.cnt(level + 1), s_scan, &s_base);
+        produce<kBlock, kPullPer, TILE, false>(cv, crs, cd, nxt, c.cnt(level + 1), s_scan,
+                                               &s_base);
     }
     const uint64_t ws = dev::wave_sum_u64(scanned);
     if (lane == 0) s_scanned[threadIdx.x / 64] = ws;
@@ -218,19 +326,24 @@ __global__ __launch_bounds__(kBlock) void k_bfs_pull(const int32_t *__restrict__
 template <int TILE>
 void bfs_levels(Ctx *c, const Graph *g, int32_t source, int32_t *depth) {
     const int direction = (int)opt(c, "bfs_direction", 1);
-    const bool do_bfs = direction != 0;
+    // pulling reads a row as the in-edges of its vertex: only valid if A == A^T
+    const bool do_bfs = direction != 0 && graph_is_symmetric(c, const_cast<Graph *>(g));
     FrontierSet fs = frontier_alloc(c, "bfs", g->m, g->nnz, TILE, false);
     const int64_t words = g->m / 32 + 1;
     uint32_t *vis = (uint32_t *)scratch(c, "bfs_bitmap", sizeof(uint32_t) * words);
     uint32_t *fb = do_bfs ? (uint32_t *)scratch(c, "bfs_fb", sizeof(uint32_t) * 3 * words) : nullptr;
     fill_i32(c, depth, -1, g->m);
-    fill_i32(c, (int32_t *)vis, 0, words);
-    if (do_bfs) fill_i32(c, (int32_t *)fb, 0, 3 * words);
+    if (do_bfs) {  // symmetric: a vertex without neighbours is never reached
+        k_vis_init<<<(int)std::min<int64_t>(ceil_div(g->m, kBlock), 16 * c->cus), kBlock, 0,
+                     c->stream>>>(g->rowptr, g->m, vis);
+        fill_i32(c, (int32_t *)fb, 0, 3 * words);
+    } else {
+        fill_i32(c, (int32_t *)vis, 0, words);
+    }
     k_bfs_seed<TILE><<<1, 256, 0, c->stream>>>(g->rowptr, source, depth, vis, fb, fs.f[0], fs.ctl);
     APSPIE_LAUNCH_CHECK();
 
-    const int grid = fill_grid(c, kBlock, (int)opt(c, "bfs_blocks_per_cu", 4));
-    const int pull_grid = (int)std::min<int64_t>(ceil_div(g->m, kBlock), 8 * c->cus);
+    const int grid = fill_grid(c, kBlock, (int)opt(c, "bfs_blocks_per_cu", 8));
     const DoParams dp{direction, (int32_t)opt(c, "bfs_alpha", 14), (int32_t)opt(c, "bfs_beta", 24),
                       g->nnz, g->m};
     int chunk = (int)std::max<int64_t>(1, opt(c, "bfs_levels_per_sync", 8));
@@ -248,12 +361,9 @@ void bfs_levels(Ctx *c, const Graph *g, int32_t source, int32_t *depth) {
             } else {
                 uint32_t *fcur = fb + (level % 3) * words, *fnext = fb + ((level + 1) % 3) * words,
                          *fclr = fb + ((level + 2) % 3) * words;
-                k_bfs_push<TILE><<<grid, kBlock, 0, c->stream>>>(g->rowptr, g->col, depth, vis,
-                                                                 fnext, fclr, fs.f[cur],
-                                                                 fs.f[cur ^ 1], fs.ctl, level, dp);
-                k_bfs_pull<TILE><<<pull_grid, kBlock, 0, c->stream>>>(
-                    g->rowptr, g->col, depth, vis, fcur, fnext, fclr, fs.f[cur ^ 1], fs.ctl,
-                    level, dp);
+                k_bfs_level<TILE><<<grid, kBlock, 0, c->stream>>>(
+                    g->rowptr, g->col, depth, vis, fcur, fnext, fclr, fs.f[cur], fs.f[cur ^ 1],
+                    fs.ctl, level, dp);
             }
             kt.end();
             APSPIE_LAUNCH_CHECK();
@@ -261,7 +371,7 @@ void bfs_levels(Ctx *c, const Graph *g, int32_t source, int32_t *depth) {
         uint64_t *h = (uint64_t *)c->pinned;
         APSPIE_HIP(hipMemcpyAsync(h, fs.ctl + (level % 3), sizeof(uint64_t), hipMemcpyDeviceToHost,
                                   c->stream));
-        APSPIE_HIP(hipMemcpyAsync(h + 1, fs.ctl + 4, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+        APSPIE_HIP(hipMemcpyAsync(h + 1, fs.ctl + 4, 9 * sizeof(uint64_t), hipMemcpyDeviceToHost,
                                   c->stream));
         sync(c);
         c->stats.host_syncs++;
@@ -269,10 +379,11 @@ void bfs_levels(Ctx *c, const Graph *g, int32_t source, int32_t *depth) {
         APSPIE_CHECK(level <= g->m + 1, APSPIE_ERR_DEVICE, "BFS did not converge");
         chunk = std::min(chunk * 2, 256);
     }
-    uint64_t *h = (uint64_t *)c->pinned;
+    uint64_t *h = (uint64_t *)c->pinned;  // h[1..9] = ctl[4..12]
     c->stats.edges_traversed = (int64_t)h[1];
     c->stats.vertices_reached = (int64_t)h[2];
     c->stats.iterations = (int32_t)h[3];
+    c->stats.edges_checked = do_bfs ? (int64_t)(h[4] + h[9]) : (int64_t)h[1];
     kt.collect((int64_t)h[3]);
 }
 

@@ -60,7 +60,8 @@ struct Graph {
     int64_t bytes = 0;
     // derived, built lazily and cached with the graph
     int32_t *lowcnt = nullptr;  // per row: number of cols < row (TC)
-    std::mutex derived_mu;      // guards lazily built derived arrays
+    int symmetric = -1;         // 1 / 0 once known (pull traversal needs A == A^T)
+    std::mutex derived_mu;      // guards lazily built derived data
 };
 
 struct Ctx {
@@ -137,6 +138,8 @@ void graph_transpose_dev(Ctx *c, const Graph *g, Graph *t);
 void mt19937_priorities(Ctx *c, uint32_t seed, int32_t m, uint32_t *out_dev);
 void fill_i32(Ctx *c, int32_t *p, int32_t v, int64_t n);
 void graph_prepare_tc(Ctx *c, Graph *g);
+// Whether the CSR equals its transpose (cached with the graph).
+bool graph_is_symmetric(Ctx *c, Graph *g);
 
 }  // namespace apspie
 

@@ -26,7 +26,7 @@ static const char *kOptionNames[] = {
 
 static int64_t option_default(const std::string &n) {
     if (n == "bfs_tile") return 2048;
-    if (n == "bfs_blocks_per_cu") return 4;
+    if (n == "bfs_blocks_per_cu") return 8;
     if (n == "bfs_levels_per_sync") return 8;
     if (n == "bfs_direction") return 1;
     if (n == "bfs_alpha") return 14;
@@ -413,6 +413,7 @@ int apspie_graph_from_coo(apspie_ctx *c, const apspie_coo *coo, int mem, int sym
             r = dr; cl = dc; v = dv;
         }
         graph_from_coo_dev(c, g, r, cl, v, coo->nnz, symmetrize);
+        if (symmetrize) g->symmetric = 1;
         sync(c);
     });
     if (st != APSPIE_OK) {

@@ -162,6 +162,37 @@ void graph_from_coo_dev(Ctx *c, Graph *g, const int32_t *row, const int32_t *col
     finish_rowptr(c, g, rowcount);
 }
 
+__global__ void k_neq(const int32_t *a, const int32_t *b, int64_t n, int32_t *flag) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (a[i] != b[i]) flag[0] = 1;
+}
+
+bool graph_is_symmetric(Ctx *c, Graph *g) {
+    std::lock_guard<std::mutex> lock(g->derived_mu);
+    if (g->symmetric >= 0) return g->symmetric == 1;
+    if (g->m != g->n) {
+        g->symmetric = 0;
+        return false;
+    }
+    Graph t;
+    graph_transpose_dev(c, g, &t);
+    int32_t *flag = (int32_t *)scratch(c, "sym_flag", 256);
+    fill_i32(c, flag, 0, 1);
+    const int grid = fill_grid(c, kBlock, 4);
+    k_neq<<<grid, kBlock, 0, c->stream>>>(g->rowptr, t.rowptr, g->m + 1, flag);
+    k_neq<<<grid, kBlock, 0, c->stream>>>(g->col, t.col, g->nnz, flag);
+    APSPIE_LAUNCH_CHECK();
+    int32_t h = 1;
+    APSPIE_HIP(hipMemcpyAsync(&h, flag, 4, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    dev_free(c, t.rowptr, sizeof(int32_t) * (t.m + 1));
+    dev_free(c, t.col, sizeof(int32_t) * std::max<int64_t>(t.nnz, 1));
+    if (t.val) dev_free(c, t.val, sizeof(float) * std::max<int64_t>(t.nnz, 1));
+    g->symmetric = h == 0 ? 1 : 0;
+    return g->symmetric == 1;
+}
+
 void graph_transpose_dev(Ctx *c, const Graph *g, Graph *t) {
     const int grid = fill_grid(c, kBlock, 4);
     const int64_t E = g->nnz, Ea = std::max<int64_t>(E, 1);

@@ -133,7 +133,11 @@ void worker(apspie_server *s, int slot) {
             auto head = s->queue.front();
             s->queue.pop_front();
             batch.push_back(head);
-            const int cap = head->kind == APSPIE_Q_TC ? INT32_MAX : std::max(1, s->cfg.max_batch);
+            // TC queries on one graph share one count; max_batch == 1 is the
+            // plain one-query-per-launch form with no coalescing at all
+            const int cap = s->cfg.max_batch <= 1     ? 1
+                            : head->kind == APSPIE_Q_TC ? INT32_MAX
+                                                        : s->cfg.max_batch;
             for (auto it = s->queue.begin(); it != s->queue.end() && (int)batch.size() < cap;) {
                 if ((*it)->graph == head->graph && (*it)->kind == head->kind) {
                     batch.push_back(*it);

@@ -188,7 +188,50 @@ def extras(ctx, G, rowptr, col, srcs, optr):
     ex["sssp_rmat22"] = {"gteps": edges / (t / 1e3) / 1e9, "ms_per_source": t / 16,
                          "sources": 16, "expanded_edges_per_traversed": relaxed / edges}
     Gw.close()
+    fout.free()
+    ex["batched_lattice2048"] = batched(ctx)
+    from apspie_amd import servebench as SB
+    log("serving pool:")
+    pool = SB.build_pool(log=log)
+    ex["served"] = SB.serve_benchmark(pool, n_queries=240, log=log)
     return ex
+
+
+def batched(ctx):
+    """Batched multi-source BFS / SSSP on the 2048^2 lattice (10% of edges
+    removed, 0.1% long-range edges, integer weights 1..255), sources uniform
+    over the largest component, against the plain per-source loop."""
+    rp, col = A.gen_lattice(2048)
+    w = A.gen_weights(rp, col)
+    G = ctx.upload(rp, col, w)
+    m = len(rp) - 1
+    # largest component = vertices reached from a high-degree vertex of it
+    depth = ctx.bfs(G, int(np.argmax(np.diff(rp))))
+    comp = np.nonzero(depth >= 0)[0]
+    rng = np.random.default_rng(4)
+    srcs = [int(x) for x in rng.choice(comp, 512, replace=False)]
+    out = ctx.dev_alloc(4 * m)
+    res = {"largest_component": int(len(comp))}
+    for kind, fn, fb in (("bfs", ctx.bfs, ctx.bfs_batch), ("sssp", ctx.sssp, ctx.sssp_batch)):
+        n_plain = 8
+        t0 = time.perf_counter()
+        for s in srcs[:n_plain]:
+            fn(G, s, out=out.ptr, mem=A.DEVICE)
+        plain_ms = (time.perf_counter() - t0) * 1e3 / n_plain
+        res[f"{kind}_plain_ms_per_query"] = plain_ms
+        for B in ((32, 128, 512) if kind == "bfs" else (32, 128)):
+            buf = ctx.dev_alloc(4 * B * m)
+            t0 = time.perf_counter()
+            fb(G, srcs[:B], out=buf.ptr, mem=A.DEVICE)
+            ms = (time.perf_counter() - t0) * 1e3
+            buf.free()
+            res[f"{kind}_batch{B}_ms_per_query"] = ms / B
+            res[f"{kind}_batch{B}_speedup"] = plain_ms / (ms / B)
+            log(f"  lattice2048 {kind} batch {B}: {ms / B:.3f} ms/query vs plain "
+                f"{plain_ms:.3f} ({plain_ms / (ms / B):.1f}x)")
+    out.free()
+    G.close()
+    return res
 
 
 if __name__ == "__main__":

@@ -124,6 +124,8 @@ typedef struct {
     double elapsed_ms;       /* device time of the call (hipEvent) */
     double kernel_ms;        /* summed time of the dominant kernel (profiling on) */
     int64_t kernel_launches; /* launches of the dominant kernel (profiling on) */
+    int64_t edges_checked;   /* CSR entries actually read: push levels expand whole rows,
+                                pull levels stop at the first frontier neighbour */
 } apspie_stats;
 int apspie_last_stats(apspie_ctx *ctx, apspie_stats *out);
 

@@ -76,6 +76,31 @@ def test_bfs_directions_vs_oracle(ctx, key, direction):
         ctx.set_option("bfs_direction", 1)
 
 
+def test_bfs_directed_graph_falls_back_to_push(ctx):
+    """Pulling reads rows as in-edges, so a non-symmetric CSR must be
+    traversed by pushing only; sinks (out-degree 0) must still be reached."""
+    rng = np.random.default_rng(5)
+    n = 3000
+    u = rng.integers(0, n, 20000).astype(np.int32)
+    v = rng.integers(0, n, 20000).astype(np.int32)
+    keep = (u != v) & (u < 2500)  # vertices >= 2500 are sinks
+    rp, c, _ = O.coo_to_csr(n, u[keep], v[keep])
+    for r in range(n):  # sorted rows, no duplicates
+        row = np.unique(c[rp[r]:rp[r + 1]])
+        c[rp[r]:rp[r] + len(row)] = row
+    uu = np.repeat(np.arange(n), np.diff(rp))
+    key = np.unique(uu.astype(np.int64) * n + c)
+    rp, c, _ = O.coo_to_csr(n, (key // n).astype(np.int32), (key % n).astype(np.int32))
+    G = ctx.upload(rp.astype(np.int32), c)
+    for d in (0, 1, 2):
+        ctx.set_option("bfs_direction", d)
+        for src in (0, 7, 2600):
+            want, _ = O.bfs(n, rp, c, src)
+            assert np.array_equal(ctx.bfs(G, src), want), (d, src)
+    ctx.set_option("bfs_direction", 1)
+    G.close()
+
+
 def test_bfs_edge_cases(ctx):
     # star: one hub expands everything in one level; path: one vertex per level
     for g, src in [(T.star_graph(5000), 0), (T.star_graph(5000), 17), (T.path_graph(300), 0),
