@@ -104,6 +104,137 @@ struct SsspBatchOp {
     }
 };
 
+// Pull form of one batched BFS level, for symmetric CSR (a row lists the
+// vertex's in-neighbours): a group of W lanes owns vertex v, lane w its source
+// word w.  The lane ORs word w of its neighbours' frontier rows (coalesced:
+// the group reads a W-word row at once), keeps the bits not yet visited, and
+// writes the visited / next words and the depths itself -- no atomics.  The
+// loads of KP vertices per thread are issued stage by stage.
+template <int W>
+__device__ __forceinline__ void batch_pull(const int32_t *__restrict__ rowptr,
+                                           const int32_t *__restrict__ col, int32_t m,
+                                           const uint32_t *__restrict__ front, uint32_t *vis,
+                                           uint32_t *next, int32_t *depth, int32_t level,
+                                           FrontierBufs nxt, uint64_t *cnt_next) {
+    constexpr int VPB = kBlock / W;  // vertices per block row
+    constexpr int KP = 4;            // block rows per pass
+    constexpr int Q = 4;             // neighbours loaded at once
+    __shared__ uint64_t s_scan[kBlock / 64 + 1];
+    __shared__ uint64_t s_base;
+    const int g = threadIdx.x / W, w = threadIdx.x % W;
+    const int lane = dev::lane_id();
+    const uint64_t gmask = W == 64 ? ~0ull : (((1ull << W) - 1) << ((lane / W) * W));
+    for (int64_t pbase = (int64_t)blockIdx.x * VPB * KP; pbase < m;
+         pbase += (int64_t)gridDim.x * VPB * KP) {
+        int32_t cv[KP], crs[KP], cd[KP], rs[KP], re[KP];
+        uint32_t uv[KP], acc[KP];
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            const int64_t v = pbase + (int64_t)k * VPB + g;
+            cv[k] = (int32_t)v;
+            crs[k] = 0;
+            cd[k] = 0;
+            acc[k] = 0;
+            const bool in = v < m;
+            uv[k] = in ? ~vis[v * W + w] : 0u;
+            rs[k] = in ? rowptr[v] : 0;
+            re[k] = in ? rowptr[v + 1] : 0;
+        }
+        // first Q neighbours of every row, then the rest row by row
+        int32_t u[KP][Q];
+#pragma unroll
+        for (int k = 0; k < KP; ++k)
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+                u[k][q] = uv[k] && rs[k] + q < re[k] ? col[rs[k] + q] : -1;
+#pragma unroll
+        for (int k = 0; k < KP; ++k)
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+                if (u[k][q] >= 0) acc[k] |= front[(int64_t)u[k][q] * W + w];
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            for (int32_t e = rs[k] + Q; e < re[k] && (acc[k] & uv[k]) != uv[k]; e += Q) {
+                int32_t uu[Q];
+#pragma unroll
+                for (int q = 0; q < Q; ++q) uu[q] = e + q < re[k] ? col[e + q] : -1;
+#pragma unroll
+                for (int q = 0; q < Q; ++q)
+                    if (uu[q] >= 0) acc[k] |= front[(int64_t)uu[q] * W + w];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            uint32_t nw = acc[k] & uv[k];
+            if (nw) {
+                const int64_t i = (int64_t)cv[k] * W + w;
+                vis[i] = ~uv[k] | nw;
+                next[i] = nw;
+                while (nw) {
+                    const int b = __ffs(nw) - 1;
+                    nw &= nw - 1;
+                    depth[(int64_t)(w * 32 + b) * m + cv[k]] = level;
+                }
+            }
+            const uint64_t bm = __ballot((acc[k] & uv[k]) != 0);
+            if (w == 0 && (bm & gmask)) {
+                crs[k] = rs[k];
+                cd[k] = re[k] - rs[k];
+            }
+        }
+        produce<kBlock, KP, kTile, true>(cv, crs, cd, nxt, cnt_next, s_scan, &s_base);
+    }
+}
+
+// One batched BFS level: push (load-balanced expansion of the union
+// frontier) or pull, chosen identically by every block from the union
+// frontier's edge count: pull when E * pull_div > nnz (pull_div 0: never).
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_batch_bfs_level(
+    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col, int32_t nnz, int32_t m,
+    BfsBatchOp<W> op, uint32_t *next_w, int32_t pull_div, FrontierBufs cur,
+    const uint64_t *cnt_cur, FrontierBufs nxt, uint64_t *cnt_next, uint64_t *cnt_clear,
+    uint64_t *stats) {
+    const uint64_t cc = *cnt_cur;
+    const int32_t F = (int32_t)cnt_entries(cc);
+    const int32_t E = (int32_t)cnt_edges(cc);
+    const bool pull = pull_div > 0 && (int64_t)E * pull_div > (int64_t)nnz;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *cnt_clear = 0;
+        if (F > 0) {
+            stats[0] += (uint64_t)E;  // union-frontier edges
+            stats[1] += (uint64_t)F;  // union-frontier entries
+            stats[2] += 1;            // levels with work
+            if (pull) stats[3] += 1;  // of which pulled
+        }
+    }
+    if (F == 0) return;
+    if (pull) {
+        batch_pull<W>(rowptr, col, m, op.front, op.vis, next_w, op.depth, *op.lvl, nxt,
+                      cnt_next);
+        return;
+    }
+    expand_tiles<kBlock, kTile, true, true, true>(op, rowptr, col, nnz, cur, F, E, nxt,
+                                                   cnt_next);
+}
+
+// visited words at the start: bits of sources beyond nsrc count as visited
+// (the last word of a batch), and with a symmetric CSR a vertex without
+// neighbours counts as visited by every source (it is never reached)
+template <int W>
+__global__ void k_batch_vis_init(const int32_t *__restrict__ rowptr, int32_t m, int32_t nsrc,
+                                 bool iso_visited, uint32_t *vis) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)m * W;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = i / W;
+        const int w = (int)(i % W);
+        const int valid = min(max(nsrc - w * 32, 0), 32);
+        uint32_t x = valid == 32 ? 0u : ~((1u << valid) - 1u);
+        if (iso_visited && rowptr[v] == rowptr[v + 1]) x = ~0u;
+        vis[i] = x;
+    }
+}
+
 // zero the frontier rows of the entries just expanded; advance the level
 template <int W>
 __global__ void k_clear_rows(FrontierBufs cur, const uint64_t *cnt, uint32_t *front,
@@ -174,8 +305,11 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
     APSPIE_HIP(hipMemcpyAsync(ulist, uniq.data(), sizeof(int32_t) * uniq.size(),
                               hipMemcpyHostToDevice, c->stream));
     const int grid = fill_grid(c, kBlock, (int)opt(c, "batch_blocks_per_cu", 4));
+    // pulling reads a row as in-edges: only for a symmetric CSR
+    const bool sym = !SSSP && graph_is_symmetric(c, const_cast<Graph *>(g));
+    const int32_t pull_div = sym ? (int32_t)opt(c, "batch_pull_div", 4) : 0;
     if (!SSSP) {
-        fill_i32(c, (int32_t *)vis, 0, words);
+        k_batch_vis_init<W><<<grid, kBlock, 0, c->stream>>>(g->rowptr, m, nsrc, sym, vis);
         fill_i32(c, depth, -1, (int64_t)nsrc * m);
     } else {
         fill_i32(c, dist, __builtin_bit_cast(int32_t, APSPIE_SSSP_INF), (int64_t)m * B);
@@ -201,10 +335,11 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
                 (int32_t)g->nnz, fs.f[cur], fs.ctl + (l % 3), fs.f[cur ^ 1],
                 fs.ctl + ((l + 1) % 3), fs.ctl + ((l + 2) % 3), fs.ctl + 4);
         else
-            k_expand<kBlock, kTile, true, true, true><<<grid, kBlock, 0, c->stream>>>(
-                BfsBatchOp<W>{front, vis, next, stamp, depth, m, lvl, nsrc}, g->rowptr, g->col,
-                (int32_t)g->nnz, fs.f[cur], fs.ctl + (l % 3), fs.f[cur ^ 1],
-                fs.ctl + ((l + 1) % 3), fs.ctl + ((l + 2) % 3), fs.ctl + 4);
+            k_batch_bfs_level<W><<<grid, kBlock, 0, c->stream>>>(
+                g->rowptr, g->col, (int32_t)g->nnz, m,
+                BfsBatchOp<W>{front, vis, next, stamp, depth, m, lvl, nsrc}, next, pull_div,
+                fs.f[cur], fs.ctl + (l % 3), fs.f[cur ^ 1], fs.ctl + ((l + 1) % 3),
+                fs.ctl + ((l + 2) % 3), fs.ctl + 4);
         k_clear_rows<W><<<grid, kBlock, 0, c->stream>>>(fs.f[cur], fs.ctl + (l % 3), front, lvl);
     };
 

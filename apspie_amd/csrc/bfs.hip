@@ -36,11 +36,13 @@ constexpr uint32_t kPush = 0, kPull = 1;
 constexpr int32_t kPullLight = 32;  // rows up to this length are scanned by one lane
 constexpr int kPullPer = 8;         // vertices per thread per pull pass
 constexpr int kPullIlp = 8;         // row entries a lane loads at once
+constexpr int32_t kSmallPush = 1 << 18;  // push levels up to this many edges prefetch rows
 
 // ctl layout (uint64 words): [0..2] frontier counters (ring), [4] edges
-// traversed, [5] entries expanded, [6] levels, [7] edges scanned by pull,
+// traversed, [5] entries expanded, [6] levels, [16 + 16 i] edges scanned by
+// pull (spread over 16 accumulators),
 // [8..9] (mode << 32 | entries) per level parity, [10..11] explored edges
-// before the level, per level parity.
+// before the level, per level parity, [12] row entries read by push levels.
 struct Ctl {
     uint64_t *w;
     __device__ uint64_t *cnt(int l) const { return w + (l % 3); }
@@ -55,15 +57,15 @@ struct DoParams {
     int32_t m;
 };
 
-// Direction of level l, computed identically by every block of both kernels
-// from values written by the previous level.
-__device__ __forceinline__ uint32_t decide(const Ctl &c, int l, const DoParams &p, int32_t F,
-                                           int32_t E) {
+// Direction of level l, computed identically by every block from values
+// written by the previous level: prev = its (mode << 32 | entries), explored =
+// frontier edges of all levels before l.
+__device__ __forceinline__ uint32_t decide(int l, const DoParams &p, uint64_t prev,
+                                           uint64_t explored, int32_t F, int32_t E) {
     if (p.direction == 0) return kPush;
     if (p.direction == 2) return l == 0 ? kPush : kPull;
-    const uint64_t prev = *c.slot(l + 1);
     const uint32_t pmode = uint32_t(prev >> 32), pF = uint32_t(prev);
-    const int64_t unexplored = p.nnz - (int64_t)*c.expl(l + 1) - E;
+    const int64_t unexplored = p.nnz - (int64_t)explored - E;
     if (pmode == kPush)
         return ((int64_t)E * p.alpha > unexplored && (uint32_t)F > pF) ? kPull : kPush;
     return ((int64_t)F * p.beta < p.m && (uint32_t)F < pF) ? kPush : kPull;
@@ -71,10 +73,10 @@ __device__ __forceinline__ uint32_t decide(const Ctl &c, int l, const DoParams &
 
 // bookkeeping done once per level by block 0 of the active kernel
 __device__ __forceinline__ void level_books(const Ctl &c, int l, uint32_t mode, int32_t F,
-                                            int32_t E) {
+                                            int32_t E, uint64_t explored) {
     *c.cnt(l + 2) = 0;
     *c.slot(l) = (uint64_t(mode) << 32) | uint32_t(F);
-    *c.expl(l) = *c.expl(l + 1) + (uint64_t)E;
+    *c.expl(l) = explored + (uint64_t)E;
     c.w[4] += (uint64_t)E;
     c.w[5] += (uint64_t)F;
     c.w[6] += 1;
@@ -105,37 +107,55 @@ struct BfsOp {
     }
 };
 
+// Everything a traversal needs before level 0, in one launch:
+//   depth[v] = -1 (0 at the source);
+//   visited bitmap = {source}, plus, when pulling is possible (symmetric CSR),
+//     every vertex without neighbours: it can never be reached, and pull
+//     levels then skip it after one bitmap read;
+//   frontier bitmaps fb0 = {source}, fb1 = {} (fb2 is cleared by level 0);
+//   the counter block and the level-0 frontier (the source's row).
+// One thread per vertex; a wave writes its two bitmap words from a ballot.
 template <int TILE>
-__global__ void k_bfs_seed(const int32_t *rowptr, int32_t src, int32_t *depth, uint32_t *vis,
-                           uint32_t *fb0, FrontierBufs f0, uint64_t *ctl) {
-    const int32_t rs = rowptr[src];
-    const int32_t d = rowptr[src + 1] - rs;
-    if (threadIdx.x == 0) {
-        depth[src] = 0;
-        vis[src >> 5] = 1u << (src & 31);
-        if (fb0) fb0[src >> 5] = 1u << (src & 31);
-        for (int i = 0; i < 16; ++i) ctl[i] = 0;
-        ctl[0] = d > 0 ? ((uint64_t(1) << 32) | uint64_t(d)) : 0;
-        ctl[9] = uint64_t(kPush) << 32;  // "level -1": pushing, empty
-        f0.start[0] = rs;
-        f0.off[0] = 0;
-        if (f0.v) f0.v[0] = src;
-    }
-    for (int32_t t = threadIdx.x; (int64_t)t * TILE < d; t += blockDim.x) f0.tfirst[t] = 0;
-}
-
-// Vertices without neighbours start out "visited" (their depth stays -1):
-// pull levels then skip them after one bitmap read.  One thread per vertex,
-// a wave writes its two bitmap words from a ballot.
-__global__ void k_vis_init(const int32_t *rowptr, int32_t m, uint32_t *vis) {
+__global__ void k_bfs_init(const int32_t *__restrict__ rowptr, int32_t m, int32_t src,
+                           bool iso_visited, int32_t *depth, uint32_t *vis, uint32_t *fb0,
+                           uint32_t *fb1, FrontierBufs f0, uint64_t *ctl) {
+    const int lane = dev::lane_id();
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < m;
          base += (int64_t)gridDim.x * blockDim.x) {
         const int64_t v = base + threadIdx.x;
-        const bool iso = v < m && rowptr[v] == rowptr[v + 1];
-        const uint64_t bm = __ballot(iso);
+        bool iso = false;
+        if (v < m) {
+            depth[v] = v == src ? 0 : -1;
+            iso = iso_visited && rowptr[v] == rowptr[v + 1];
+        }
+        const uint64_t sm = __ballot(v == src);
+        const uint64_t vm = __ballot(iso) | sm;
         const int64_t w = (base + (threadIdx.x & ~63)) >> 5;
-        if (dev::lane_id() == 0) vis[w] = (uint32_t)bm;
-        if (dev::lane_id() == 32 && (w + 1) * 32 < m + 32) vis[w + 1] = (uint32_t)(bm >> 32);
+        if (lane == 0 || lane == 32) {
+            const int64_t wi = w + (lane >> 5);
+            const int sh = lane;  // 0 or 32
+            if (wi * 32 < m) {
+                vis[wi] = (uint32_t)(vm >> sh);
+                if (fb0) {
+                    fb0[wi] = (uint32_t)(sm >> sh);
+                    fb1[wi] = 0;
+                }
+            }
+        }
+    }
+    if (blockIdx.x == 0) {
+        const int32_t rs = rowptr[src];
+        const int32_t d = rowptr[src + 1] - rs;
+        for (int i = threadIdx.x; i < kCtlWords; i += blockDim.x) ctl[i] = 0;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            ctl[0] = d > 0 ? ((uint64_t(1) << 32) | uint64_t(d)) : 0;
+            ctl[9] = uint64_t(kPush) << 32;  // "level -1": pushing, empty
+            f0.start[0] = rs;
+            f0.off[0] = 0;
+            if (f0.v) f0.v[0] = src;
+        }
+        for (int32_t t = threadIdx.x; (int64_t)t * TILE < d; t += blockDim.x) f0.tfirst[t] = 0;
     }
 }
 
@@ -154,7 +174,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     __shared__ uint64_t s_base;
     __shared__ uint64_t s_scanned[kBlock / 64];
     const Ctl c{ctlw};
+    // the three control words are loaded together (one round trip)
     const uint64_t cc = *c.cnt(level);
+    const uint64_t prev = *c.slot(level + 1), explored = *c.expl(level + 1);
     const int32_t F = (int32_t)cnt_entries(cc), E = (int32_t)cnt_edges(cc);
     if (F == 0) {
         // keep the counter ring clean after the last level: the entry two
@@ -162,17 +184,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
         if (blockIdx.x == 0 && threadIdx.x == 0) *c.cnt(level + 2) = 0;
         return;
     }
-    const uint32_t mode = decide(c, level, p, F, E);
+    const uint32_t mode = decide(level, p, prev, explored, F, E);
     __syncthreads();  // every thread has read the shared state before block 0 writes
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        level_books(c, level, mode, F, E);
+        level_books(c, level, mode, F, E, explored);
         if (mode == kPush) c.w[12] += (uint64_t)E;  // entries read by push levels
     }
     clear_slice(fb_clear, p.m / 32 + 1);
     if (mode == kPush) {
+        // small levels are latency-bound: row extents loaded with the columns
+        const bool small = E <= kSmallPush;
         BfsOp op{vis, depth, fb_next, level + 1};
         expand_tiles<kBlock, TILE, false, false, true>(op, rowptr, col, (int32_t)p.nnz, cur, F,
-                                                       E, nxt, c.cnt(level + 1));
+                                                       E, nxt, c.cnt(level + 1), small);
         return;
     }
     // pull: each thread takes kPullPer vertices of a pass; for every k the
@@ -194,12 +218,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
             crs[k] = 0;
             cd[k] = 0;
             hit[k] = false;
-            open[k] = v < p.m && !((vis[v >> 5] >> (v & 31)) & 1u);
-        }
-#pragma unroll
-        for (int k = 0; k < kPullPer; ++k) {
-            rs[k] = open[k] ? rowptr[cv[k]] : 0;
-            re[k] = open[k] ? rowptr[cv[k] + 1] : 0;
+            // row extents are loaded with the visited bits, not after them
+            // (coalesced, and one dependent round trip less)
+            const bool in = v < p.m;
+            open[k] = in && !((vis[v >> 5] >> (v & 31)) & 1u);
+            rs[k] = in ? rowptr[v] : 0;
+            re[k] = in ? rowptr[v + 1] : 0;
         }
         // first entry of every open row (the lowest-id neighbour: on R-MAT
         // usually a hub, and most hits happen here)
@@ -319,7 +343,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     if (threadIdx.x == 0) {
         uint64_t t = 0;
         for (int q = 0; q < kBlock / 64; ++q) t += s_scanned[q];
-        if (t) dev::atomic_add_dev(ctlw + 7, t);
+        if (t) dev::atomic_add_dev(spread_slot(ctlw), t);
     }
 }
 
@@ -332,15 +356,11 @@ void bfs_levels(Ctx *c, const Graph *g, int32_t source, int32_t *depth) {
     const int64_t words = g->m / 32 + 1;
     uint32_t *vis = (uint32_t *)scratch(c, "bfs_bitmap", sizeof(uint32_t) * words);
     uint32_t *fb = do_bfs ? (uint32_t *)scratch(c, "bfs_fb", sizeof(uint32_t) * 3 * words) : nullptr;
-    fill_i32(c, depth, -1, g->m);
-    if (do_bfs) {  // symmetric: a vertex without neighbours is never reached
-        k_vis_init<<<(int)std::min<int64_t>(ceil_div(g->m, kBlock), 16 * c->cus), kBlock, 0,
-                     c->stream>>>(g->rowptr, g->m, vis);
-        fill_i32(c, (int32_t *)fb, 0, 3 * words);
-    } else {
-        fill_i32(c, (int32_t *)vis, 0, words);
-    }
-    k_bfs_seed<TILE><<<1, 256, 0, c->stream>>>(g->rowptr, source, depth, vis, fb, fs.f[0], fs.ctl);
+    // symmetric: a vertex without neighbours is never reached, so it starts
+    // out visited; fb[2 * words ..] is cleared by level 0 itself
+    k_bfs_init<TILE><<<(int)std::min<int64_t>(ceil_div(g->m, kBlock), 16 * c->cus), kBlock, 0,
+                       c->stream>>>(g->rowptr, g->m, source, do_bfs, depth, vis, fb,
+                                    fb ? fb + words : nullptr, fs.f[0], fs.ctl);
     APSPIE_LAUNCH_CHECK();
 
     const int grid = fill_grid(c, kBlock, (int)opt(c, "bfs_blocks_per_cu", 8));
@@ -348,11 +368,12 @@ void bfs_levels(Ctx *c, const Graph *g, int32_t source, int32_t *depth) {
                       g->nnz, g->m};
     int chunk = (int)std::max<int64_t>(1, opt(c, "bfs_levels_per_sync", 8));
     KernelTimer kt(c);
+    uint64_t *h = (uint64_t *)c->pinned;  // copy of the control block
     int level = 0;
     for (;;) {
+        kt.begin();
         for (int k = 0; k < chunk; ++k, ++level) {
             const int cur = level & 1;
-            kt.begin();
             if (!do_bfs) {
                 k_expand<kBlock, TILE, false, false, true><<<grid, kBlock, 0, c->stream>>>(
                     BfsOp{vis, depth, nullptr, level + 1}, g->rowptr, g->col, (int32_t)g->nnz,
@@ -365,26 +386,23 @@ void bfs_levels(Ctx *c, const Graph *g, int32_t source, int32_t *depth) {
                     g->rowptr, g->col, depth, vis, fcur, fnext, fclr, fs.f[cur], fs.f[cur ^ 1],
                     fs.ctl, level, dp);
             }
-            kt.end();
             APSPIE_LAUNCH_CHECK();
         }
-        uint64_t *h = (uint64_t *)c->pinned;
-        APSPIE_HIP(hipMemcpyAsync(h, fs.ctl + (level % 3), sizeof(uint64_t), hipMemcpyDeviceToHost,
-                                  c->stream));
-        APSPIE_HIP(hipMemcpyAsync(h + 1, fs.ctl + 4, 9 * sizeof(uint64_t), hipMemcpyDeviceToHost,
-                                  c->stream));
+        kt.end(chunk);
+        // the whole control block in one copy: counters and statistics
+        APSPIE_HIP(hipMemcpyAsync(h, fs.ctl, kCtlWords * sizeof(uint64_t),
+                                  hipMemcpyDeviceToHost, c->stream));
         sync(c);
         c->stats.host_syncs++;
-        if (cnt_entries(h[0]) == 0) break;
+        if (cnt_entries(h[level % 3]) == 0) break;
         APSPIE_CHECK(level <= g->m + 1, APSPIE_ERR_DEVICE, "BFS did not converge");
         chunk = std::min(chunk * 2, 256);
     }
-    uint64_t *h = (uint64_t *)c->pinned;  // h[1..9] = ctl[4..12]
-    c->stats.edges_traversed = (int64_t)h[1];
-    c->stats.vertices_reached = (int64_t)h[2];
-    c->stats.iterations = (int32_t)h[3];
-    c->stats.edges_checked = do_bfs ? (int64_t)(h[4] + h[9]) : (int64_t)h[1];
-    kt.collect((int64_t)h[3]);
+    c->stats.edges_traversed = (int64_t)h[4];
+    c->stats.vertices_reached = (int64_t)h[5];
+    c->stats.iterations = (int32_t)h[6];
+    c->stats.edges_checked = do_bfs ? (int64_t)(spread_sum(h) + h[12]) : (int64_t)h[4];
+    kt.collect();  // every launch, the empty trailing ones included
 }
 
 }  // namespace

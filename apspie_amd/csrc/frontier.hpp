@@ -87,11 +87,17 @@ __device__ __forceinline__ void produce(const int32_t (&cv)[K], const int32_t (&
 // When PRODUCE is false nothing is appended.
 // The tile loop of the expansion, for kernels that add their own prologue.
 // F / E: entries and expanded edges of `cur` (F > 0).
+// prefetch loads the row extent of every edge's endpoint together with the
+// column instead of after a successful claim: one dependent round trip less
+// per level for 8 extra bytes per edge, worth it on small (latency-bound)
+// frontiers only.  (A runtime flag: a second instantiation would duplicate
+// the LDS arrays below.)
 template <int BLOCK, int TILE, bool STORE_V_IN, bool STORE_V_OUT, bool PRODUCE, class Op>
 __device__ __forceinline__ void expand_tiles(Op &op, const int32_t *__restrict__ rowptr,
                                              const int32_t *__restrict__ col, int32_t nnz,
                                              FrontierBufs cur, int32_t F, int32_t E,
-                                             FrontierBufs nxt, uint64_t *cnt_next) {
+                                             FrontierBufs nxt, uint64_t *cnt_next,
+                                             bool prefetch = false) {
     constexpr int K = TILE / BLOCK;
     __shared__ int32_t s_off[TILE + 2];
     __shared__ int32_t s_start[TILE + 2];
@@ -101,8 +107,9 @@ __device__ __forceinline__ void expand_tiles(Op &op, const int32_t *__restrict__
     const int32_t ntiles = (int32_t)((E + TILE - 1) / TILE);
     for (int32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         // entries owning this tile; every degree is >= 1, so at most TILE + 1.
-        // The clamps only guard memory if the frontier were inconsistent.
-        const int32_t i0 = min(max(cur.tfirst[t], 0), F - 1);
+        // The clamps only guard memory if the frontier were inconsistent.  A
+        // frontier of one tile owns it whole (no partition lookup).
+        const int32_t i0 = ntiles == 1 ? 0 : min(max(cur.tfirst[t], 0), F - 1);
         const int32_t i1 = (t + 1 < ntiles) ? min(max(cur.tfirst[t + 1], i0), F - 1) : F - 1;
         const int32_t nv = min(i1 - i0 + 1, TILE + 1);
         for (int32_t j = threadIdx.x; j < nv; j += BLOCK) {
@@ -125,11 +132,19 @@ __device__ __forceinline__ void expand_tiles(Op &op, const int32_t *__restrict__
                 if ((uint32_t)idx >= (uint32_t)nnz) continue;
                 const int32_t v = col[idx];
                 const int32_t u = STORE_V_IN ? s_v[j] : -1;
+                int32_t prs = 0, pre = 0;
+                if (PRODUCE && prefetch) {
+                    prs = rowptr[v];
+                    pre = rowptr[v + 1];
+                }
                 if (op.edge(u, idx, v) && PRODUCE) {
-                    const int32_t rs = rowptr[v];
+                    if (!prefetch) {
+                        prs = rowptr[v];
+                        pre = rowptr[v + 1];
+                    }
                     cv[k] = v;
-                    crs[k] = rs;
-                    cd[k] = rowptr[v + 1] - rs;
+                    crs[k] = prs;
+                    cd[k] = pre - prs;
                 }
             }
         }
@@ -194,11 +209,25 @@ __global__ __launch_bounds__(BLOCK) void k_build(Op op, const int32_t *__restric
     }
 }
 
+// Control block: 16 words of counters / statistics, then 16 accumulators
+// 128 bytes apart, which blocks pick by block index so that a per-block
+// statistic does not serialise thousands of atomics on one address.
+constexpr int kCtlWords = 16 + 16 * 16;
+constexpr int kSpread = 16;
+__device__ __forceinline__ uint64_t *spread_slot(uint64_t *ctl) {
+    return ctl + 16 + (blockIdx.x % kSpread) * 16;
+}
+inline uint64_t spread_sum(const uint64_t *h) {
+    uint64_t t = 0;
+    for (int i = 0; i < kSpread; ++i) t += h[16 + i * 16];
+    return t;
+}
+
 // Frontier storage sized for a graph: two ping-pong frontiers, a ring of
 // three counters and a stats block.
 struct FrontierSet {
     FrontierBufs f[2];
-    uint64_t *ctl;  // [0..2] counters, [4..11] stats
+    uint64_t *ctl;  // [0..2] counters, [4..15] stats, [16..271] spread accumulators
     int64_t cap_entries = 0, cap_tiles = 0;
 };
 
@@ -214,7 +243,7 @@ inline FrontierSet frontier_alloc(Ctx *c, const std::string &tag, int32_t m, int
         s.f[i].off = (int32_t *)scratch(c, p + "_o", sizeof(int32_t) * ent);
         s.f[i].tfirst = (int32_t *)scratch(c, p + "_t", sizeof(int32_t) * tiles);
     }
-    s.ctl = (uint64_t *)scratch(c, tag + "_ctl", sizeof(uint64_t) * 16);
+    s.ctl = (uint64_t *)scratch(c, tag + "_ctl", sizeof(uint64_t) * kCtlWords);
     s.cap_entries = ent;
     s.cap_tiles = tiles;
     return s;

@@ -75,16 +75,16 @@ void sssp_rounds(Ctx *c, const Graph *g, int32_t source, float *dist_out) {
     KernelTimer kt(c);
     int round = 0;
     for (;;) {
+        kt.begin();
         for (int k = 0; k < chunk; ++k, ++round) {
             const int cur = round & 1;
-            kt.begin();
             k_expand<kBlock, TILE, true, true, true><<<grid, kBlock, 0, c->stream>>>(
                 SsspOp{g->val, dist, stamp, round + 1}, g->rowptr, g->col, (int32_t)g->nnz,
                 fs.f[cur], fs.ctl + (round % 3), fs.f[cur ^ 1], fs.ctl + ((round + 1) % 3),
                 fs.ctl + ((round + 2) % 3), fs.ctl + 4);
-            kt.end();
             APSPIE_LAUNCH_CHECK();
         }
+        kt.end(chunk);
         uint64_t *h = (uint64_t *)c->pinned;
         APSPIE_HIP(hipMemcpyAsync(h, fs.ctl + (round % 3), sizeof(uint64_t), hipMemcpyDeviceToHost,
                                   c->stream));
@@ -101,7 +101,7 @@ void sssp_rounds(Ctx *c, const Graph *g, int32_t source, float *dist_out) {
     c->stats.edges_traversed = (int64_t)h[1];
     c->stats.vertices_reached = (int64_t)h[2];
     c->stats.iterations = (int32_t)h[3];
-    kt.collect((int64_t)h[3]);
+    kt.collect();
 }
 
 }  // namespace

@@ -17,6 +17,8 @@ class KernelTimer {
             (void)hipEventDestroy(p.second);
         }
     }
+    // an interval around `launches` back-to-back launches of the kernel (a
+    // span over several launches keeps the events from separating them)
     void begin() {
         if (!on_) return;
         std::pair<hipEvent_t, hipEvent_t> p;
@@ -25,24 +27,22 @@ class KernelTimer {
         APSPIE_HIP(hipEventRecord(p.first, c_->stream));
         ev_.push_back(p);
     }
-    void end() {
+    void end(int64_t launches = 1) {
         if (!on_) return;
         APSPIE_HIP(hipEventRecord(ev_.back().second, c_->stream));
+        n_.push_back(launches);
     }
-    // sums the first `limit` recorded intervals (all if negative) into the
-    // context stats; launches past the limit are the early-exit ones issued
-    // after the work ran out
-    void collect(int64_t limit = -1) {
+    // sums the recorded intervals and their launch counts into the stats
+    void collect() {
         if (!on_) return;
         double total = 0;
         int64_t n = 0;
-        for (auto &p : ev_) {
-            if (limit >= 0 && n >= limit) break;
-            APSPIE_HIP(hipEventSynchronize(p.second));
+        for (size_t i = 0; i < ev_.size(); ++i) {
+            APSPIE_HIP(hipEventSynchronize(ev_[i].second));
             float ms = 0;
-            APSPIE_HIP(hipEventElapsedTime(&ms, p.first, p.second));
+            APSPIE_HIP(hipEventElapsedTime(&ms, ev_[i].first, ev_[i].second));
             total += ms;
-            ++n;
+            n += n_[i];
         }
         c_->stats.kernel_ms += total;
         c_->stats.kernel_launches += n;
@@ -52,6 +52,7 @@ class KernelTimer {
     Ctx *c_;
     bool on_;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_;
+    std::vector<int64_t> n_;
 };
 
 }  // namespace apspie

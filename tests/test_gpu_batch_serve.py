@@ -38,6 +38,46 @@ def test_bfs_batch_vs_oracle(ctx, key, nsrc):
     G.close()
 
 
+@pytest.mark.parametrize("pull_div", [0, 1, 4, 1 << 20])
+@pytest.mark.parametrize("nsrc", [5, 33, 200])
+@pytest.mark.parametrize("key", ["rmat12", "lattice64", "star"])
+def test_bfs_batch_directions(ctx, key, nsrc, pull_div):
+    """Push only (0), pull on big levels (1, 4) and pull on every level after
+    the first (2^20) give the single-source depths."""
+    g = {"rmat12": lambda: T.rmat(12), "lattice64": lambda: T.lattice(64),
+         "star": lambda: T.star_graph(3000)}[key]()
+    ctx.set_option("batch_pull_div", pull_div)
+    try:
+        G = ctx.upload(g.rowptr32, g.col)
+        src = T.sources(g, nsrc, seed=23)
+        got = ctx.bfs_batch(G, src)
+        for i, s in enumerate(src):
+            assert np.array_equal(got[i], O.bfs(g.m, g.rowptr, g.col, s)[0]), (key, i, s)
+        G.close()
+    finally:
+        ctx.set_option("batch_pull_div", 4)
+
+
+def test_bfs_batch_directed_graph_pushes(ctx):
+    """A non-symmetric CSR is never pulled (rows are out-edges only)."""
+    rng = np.random.default_rng(9)
+    n = 2000
+    u = rng.integers(0, n, 12000)
+    v = rng.integers(0, n, 12000)
+    key = np.unique((u * n + v)[(u != v) & (u < 1700)])
+    rp, c, _ = O.coo_to_csr(n, (key // n).astype(np.int32), (key % n).astype(np.int32))
+    ctx.set_option("batch_pull_div", 1 << 20)
+    try:
+        G = ctx.upload(rp.astype(np.int32), c)
+        src = [0, 5, 1800, 77] * 10
+        got = ctx.bfs_batch(G, src)
+        for i, s in enumerate(src):
+            assert np.array_equal(got[i], O.bfs(n, rp, c, s)[0]), (i, s)
+        G.close()
+    finally:
+        ctx.set_option("batch_pull_div", 4)
+
+
 @pytest.mark.parametrize("nsrc", [1, 7, 32, 64, 150])
 @pytest.mark.parametrize("key", ["rmat11", "lattice32"])
 def test_sssp_batch_vs_oracle(ctx, key, nsrc):
