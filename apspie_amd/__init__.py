@@ -112,6 +112,7 @@ def _declare(L):
         "apspie_spmspv": ([vp, vp, C.c_int, i64, vp, vp, vp, C.POINTER(i64), vp, vp, C.c_int,
                            C.c_int], C.c_int),
         "apspie_spgemm": ([vp, vp, vp, pp], C.c_int),
+        "apspie_spgemm_masked": ([vp, vp, vp, vp, vp, C.POINTER(i64), C.c_int], C.c_int),
         "apspie_scan_exclusive": ([vp, vp, vp, i64, C.POINTER(i64), C.c_int], C.c_int),
         "apspie_sort_pairs": ([vp, vp, vp, i64, C.c_int, C.c_int], C.c_int),
         "apspie_bfs_batch": ([vp, vp, vp, i32, vp, C.c_int], C.c_int),
@@ -321,6 +322,15 @@ class Context:
         h = C.c_void_p()
         _check(self._L.apspie_spgemm(self.h, a.h, b.h, C.byref(h)), self.h)
         return Graph(self, h)
+
+    def spgemm_masked(self, a, b, mask):
+        """(A * B) o M on M's pattern: (values in M's CSR order, entries of M
+        with at least one product)."""
+        out = np.empty(max(mask.nnz, 1), np.float32)
+        nnz = C.c_int64()
+        _check(self._L.apspie_spgemm_masked(self.h, a.h, b.h, mask.h, _p(out), C.byref(nnz),
+                                            HOST), self.h)
+        return out[:mask.nnz], nnz.value
 
     def scan_exclusive(self, x):
         x = np.ascontiguousarray(x, np.int32)

@@ -104,6 +104,43 @@ struct SsspBatchOp {
     }
 };
 
+// Claims of a pull level are collected in an LDS queue and appended to the
+// next frontier kQueue at a time: one 64-bit reservation per flush instead of
+// one per pass (a pass covers only 256 / W * 4 vertices, and thousands of
+// reservations on one counter serialise in the L2).
+constexpr int kQueue = 2048;
+struct PullQueue {
+    int32_t v[kQueue], rs[kQueue], d[kQueue];
+    uint32_t n;
+    uint32_t scan32[kBlock / 64 + 1];
+    uint64_t scan[kBlock / 64 + 1];
+    uint64_t base;
+};
+
+// LDS of the level kernel: the push branch's expansion or the pull's queue
+union BatchLds {
+    ExpandLds<kBlock, kTile, true> ex;
+    PullQueue pq;
+};
+
+// append the queue to the next frontier (every thread of the block calls)
+__device__ __forceinline__ void queue_flush(PullQueue &q, FrontierBufs nxt, uint64_t *cnt_next) {
+    constexpr int QK = kQueue / kBlock;
+    const uint32_t n = q.n;
+    int32_t cv[QK], crs[QK], cd[QK];
+#pragma unroll
+    for (int k = 0; k < QK; ++k) {
+        const uint32_t i = threadIdx.x + k * kBlock;
+        cv[k] = i < n ? q.v[i] : 0;
+        crs[k] = i < n ? q.rs[i] : 0;
+        cd[k] = i < n ? q.d[i] : 0;
+    }
+    produce<kBlock, QK, kTile, true>(cv, crs, cd, nxt, cnt_next, q.scan, &q.base);
+    __syncthreads();  // every thread has read q.n and its entries
+    if (threadIdx.x == 0) q.n = 0;
+    __syncthreads();
+}
+
 // Pull form of one batched BFS level, for symmetric CSR (a row lists the
 // vertex's in-neighbours): a group of W lanes owns vertex v, lane w its source
 // word w.  The lane ORs word w of its neighbours' frontier rows (coalesced:
@@ -111,7 +148,7 @@ struct SsspBatchOp {
 // writes the visited / next words and the depths itself -- no atomics.  The
 // loads of KP vertices per thread are issued stage by stage.
 template <int W>
-__device__ __forceinline__ void batch_pull(const int32_t *__restrict__ rowptr,
+__device__ __forceinline__ void batch_pull(PullQueue &q, const int32_t *__restrict__ rowptr,
                                            const int32_t *__restrict__ col, int32_t m,
                                            const uint32_t *__restrict__ front, uint32_t *vis,
                                            uint32_t *next, int32_t *depth, int32_t level,
@@ -119,8 +156,8 @@ __device__ __forceinline__ void batch_pull(const int32_t *__restrict__ rowptr,
     constexpr int VPB = kBlock / W;  // vertices per block row
     constexpr int KP = 4;            // block rows per pass
     constexpr int Q = 4;             // neighbours loaded at once
-    __shared__ uint64_t s_scan[kBlock / 64 + 1];
-    __shared__ uint64_t s_base;
+    if (threadIdx.x == 0) q.n = 0;
+    __syncthreads();
     const int g = threadIdx.x / W, w = threadIdx.x % W;
     const int lane = dev::lane_id();
     const uint64_t gmask = W == 64 ? ~0ull : (((1ull << W) - 1) << ((lane / W) * W));
@@ -182,8 +219,27 @@ __device__ __forceinline__ void batch_pull(const int32_t *__restrict__ rowptr,
                 cd[k] = re[k] - rs[k];
             }
         }
-        produce<kBlock, KP, kTile, true>(cv, crs, cd, nxt, cnt_next, s_scan, &s_base);
+        // append this pass's claims to the queue (flushing it first if full)
+        uint32_t c = 0;
+#pragma unroll
+        for (int k = 0; k < KP; ++k) c += cd[k] > 0 ? 1u : 0u;
+        uint32_t total;
+        uint32_t pos = dev::block_exclusive_add32<kBlock>(c, q.scan32, total);
+        if (q.n + total > (uint32_t)kQueue) queue_flush(q, nxt, cnt_next);
+        pos += q.n;
+#pragma unroll
+        for (int k = 0; k < KP; ++k)
+            if (cd[k] > 0) {
+                q.v[pos] = cv[k];
+                q.rs[pos] = crs[k];
+                q.d[pos] = cd[k];
+                ++pos;
+            }
+        __syncthreads();  // every thread has read q.n
+        if (threadIdx.x == 0) q.n += total;
+        __syncthreads();
     }
+    if (q.n > 0) queue_flush(q, nxt, cnt_next);
 }
 
 // One batched BFS level: push (load-balanced expansion of the union
@@ -195,6 +251,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_bfs_level(
     BfsBatchOp<W> op, uint32_t *next_w, int32_t pull_div, FrontierBufs cur,
     const uint64_t *cnt_cur, FrontierBufs nxt, uint64_t *cnt_next, uint64_t *cnt_clear,
     uint64_t *stats) {
+    __shared__ BatchLds lds;
     const uint64_t cc = *cnt_cur;
     const int32_t F = (int32_t)cnt_entries(cc);
     const int32_t E = (int32_t)cnt_edges(cc);
@@ -210,11 +267,11 @@ __global__ __launch_bounds__(kBlock) void k_batch_bfs_level(
     }
     if (F == 0) return;
     if (pull) {
-        batch_pull<W>(rowptr, col, m, op.front, op.vis, next_w, op.depth, *op.lvl, nxt,
+        batch_pull<W>(lds.pq, rowptr, col, m, op.front, op.vis, next_w, op.depth, *op.lvl, nxt,
                       cnt_next);
         return;
     }
-    expand_tiles<kBlock, kTile, true, true, true>(op, rowptr, col, nnz, cur, F, E, nxt,
+    expand_tiles<kBlock, kTile, true, true, true>(lds.ex, op, rowptr, col, nnz, cur, F, E, nxt,
                                                    cnt_next);
 }
 

@@ -31,7 +31,10 @@ namespace apspie {
 
 namespace {
 
-constexpr int kBlock = 256;
+// 1024-thread blocks, two per CU: a level's appends to the next frontier
+// are one 64-bit atomic per block and pass on a single counter, which the L2
+// serialises (~10 ns each), so fewer, larger blocks shorten every level.
+constexpr int kBlock = 1024;
 constexpr uint32_t kPush = 0, kPull = 1;
 constexpr int32_t kPullLight = 32;  // rows up to this length are scanned by one lane
 constexpr int kPullPer = 8;         // vertices per thread per pull pass
@@ -162,9 +165,9 @@ __global__ void k_bfs_init(const int32_t *__restrict__ rowptr, int32_t m, int32_
 // One level of the direction-optimising loop: every block reads the same
 // counters, picks the same direction and runs either the push expansion or
 // the pull sweep.
-// The level is latency-bound (a chain of dependent loads per wave), so all
-// 8 blocks per CU of the default grid must be resident at once: 8 waves per
-// SIMD, i.e. at most 64 VGPRs.
+// The level is latency-bound (a chain of dependent loads per wave), so the
+// whole default grid (two blocks per CU) must be resident at once: 8 waves
+// per SIMD, i.e. at most 64 VGPRs.
 template <int TILE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_bfs_level(
     const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col, int32_t *depth,
@@ -363,7 +366,9 @@ void bfs_levels(Ctx *c, const Graph *g, int32_t source, int32_t *depth) {
                                     fb ? fb + words : nullptr, fs.f[0], fs.ctl);
     APSPIE_LAUNCH_CHECK();
 
-    const int grid = fill_grid(c, kBlock, (int)opt(c, "bfs_blocks_per_cu", 8));
+    // bfs_blocks_per_cu counts 256-thread groups per CU
+    const int grid = fill_grid(
+        c, kBlock, (int)std::max<int64_t>(1, opt(c, "bfs_blocks_per_cu", 8) * 256 / kBlock));
     const DoParams dp{direction, (int32_t)opt(c, "bfs_alpha", 14), (int32_t)opt(c, "bfs_beta", 24),
                       g->nnz, g->m};
     int chunk = (int)std::max<int64_t>(1, opt(c, "bfs_levels_per_sync", 8));

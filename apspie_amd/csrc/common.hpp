@@ -126,6 +126,9 @@ int64_t spmspv_run(Ctx *c, const Graph *g, int semiring, int64_t nx, const int32
                    const float *x_val, const int32_t *mask, int32_t *y_ind, float *y_val,
                    int path);
 Graph *spgemm_run(Ctx *c, const Graph *a, const Graph *b);  // returns new apspie_graph
+// (A * B) o M on M's pattern into cval (device, nnz(M)); returns the entries
+// of M with at least one product
+int64_t spgemm_masked_run(Ctx *c, const Graph *a, const Graph *b, const Graph *mask, float *cval);
 int64_t scan_exclusive_dev(Ctx *c, const int32_t *in, int32_t *out, int64_t n);
 void sort_pairs_dev(Ctx *c, uint32_t *keys, uint32_t *vals, int64_t n, int bits);
 void bfs_batch_run(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc,

@@ -9,7 +9,7 @@ namespace apspie {
 
 static const char *kOptionNames[] = {
     "bfs_tile",          // edges per expansion tile (power of two)
-    "bfs_blocks_per_cu", // persistent blocks per CU for the expansion kernel
+    "bfs_blocks_per_cu", // resident 256-thread groups per CU for the level kernels
     "bfs_levels_per_sync", // levels launched between host checks
     "bfs_direction",     // 0 push only, 1 direction-optimising
     "bfs_alpha",         // push->pull switch: frontier edges * alpha > unexplored edges
@@ -645,6 +645,32 @@ int apspie_spgemm(apspie_ctx *c, const apspie_graph *a, const apspie_graph *b,
         timed(c, [&] { r = spgemm_run(c, a, b); });
         c->graphs.insert(r);
         *out = static_cast<apspie_graph *>(r);
+    });
+}
+
+int apspie_spgemm_masked(apspie_ctx *c, const apspie_graph *a, const apspie_graph *b,
+                         const apspie_graph *mask, float *c_val, int64_t *c_nnz, int mem) {
+    if (!c || !a || !b || !mask || !c_nnz) return APSPIE_ERR_INVALID;
+    return guard(c, [&] {
+        check_graph(c, a);
+        check_graph(c, b);
+        check_graph(c, mask);
+        APSPIE_CHECK(a->n == b->m, APSPIE_ERR_INVALID, "inner dimensions differ");
+        APSPIE_CHECK(mask->m == a->m && mask->n == b->n, APSPIE_ERR_INVALID,
+                     "mask shape differs from the product's");
+        APSPIE_CHECK(mask->nnz == 0 || c_val, APSPIE_ERR_INVALID, "c_val is NULL");
+        set_device(c);
+        float *out = mem == APSPIE_MEM_DEVICE
+                         ? c_val
+                         : (float *)scratch(c, "msk_out",
+                                            sizeof(float) * std::max<int64_t>(mask->nnz, 1));
+        int64_t nnz = 0;
+        timed(c, [&] { nnz = spgemm_masked_run(c, a, b, mask, out); });
+        if (mem == APSPIE_MEM_HOST && mask->nnz > 0) {
+            copy_from_device(c, c_val, out, sizeof(float) * mask->nnz, APSPIE_MEM_HOST);
+            sync(c);
+        }
+        *c_nnz = nnz;
     });
 }
 

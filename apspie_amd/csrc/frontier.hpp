@@ -87,23 +87,31 @@ __device__ __forceinline__ void produce(const int32_t (&cv)[K], const int32_t (&
 // When PRODUCE is false nothing is appended.
 // The tile loop of the expansion, for kernels that add their own prologue.
 // F / E: entries and expanded edges of `cur` (F > 0).
+// LDS of one expansion: the tile's entries and the append scan.  A kernel
+// that also does other work can overlay it with its own LDS in a union.
+template <int BLOCK, int TILE, bool STORE_V_IN>
+struct ExpandLds {
+    int32_t off[TILE + 2];
+    int32_t start[TILE + 2];
+    int32_t v[STORE_V_IN ? TILE + 2 : 1];
+    uint64_t scan[BLOCK / 64 + 1];
+    uint64_t base;
+};
+
 // prefetch loads the row extent of every edge's endpoint together with the
 // column instead of after a successful claim: one dependent round trip less
 // per level for 8 extra bytes per edge, worth it on small (latency-bound)
 // frontiers only.  (A runtime flag: a second instantiation would duplicate
-// the LDS arrays below.)
+// the LDS.)
 template <int BLOCK, int TILE, bool STORE_V_IN, bool STORE_V_OUT, bool PRODUCE, class Op>
-__device__ __forceinline__ void expand_tiles(Op &op, const int32_t *__restrict__ rowptr,
+__device__ __forceinline__ void expand_tiles(ExpandLds<BLOCK, TILE, STORE_V_IN> &L, Op &op,
+                                             const int32_t *__restrict__ rowptr,
                                              const int32_t *__restrict__ col, int32_t nnz,
                                              FrontierBufs cur, int32_t F, int32_t E,
                                              FrontierBufs nxt, uint64_t *cnt_next,
                                              bool prefetch = false) {
     constexpr int K = TILE / BLOCK;
-    __shared__ int32_t s_off[TILE + 2];
-    __shared__ int32_t s_start[TILE + 2];
-    __shared__ int32_t s_v[STORE_V_IN ? TILE + 2 : 1];
-    __shared__ uint64_t s_scan[BLOCK / 64 + 1];
-    __shared__ uint64_t s_base;
+    int32_t *s_off = L.off, *s_start = L.start, *s_v = L.v;
     const int32_t ntiles = (int32_t)((E + TILE - 1) / TILE);
     for (int32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         // entries owning this tile; every degree is >= 1, so at most TILE + 1.
@@ -149,9 +157,20 @@ __device__ __forceinline__ void expand_tiles(Op &op, const int32_t *__restrict__
             }
         }
         if (PRODUCE)
-            produce<BLOCK, K, TILE, STORE_V_OUT>(cv, crs, cd, nxt, cnt_next, s_scan, &s_base);
+            produce<BLOCK, K, TILE, STORE_V_OUT>(cv, crs, cd, nxt, cnt_next, L.scan, &L.base);
         __syncthreads();  // s_off / s_start are overwritten by the next tile
     }
+}
+
+template <int BLOCK, int TILE, bool STORE_V_IN, bool STORE_V_OUT, bool PRODUCE, class Op>
+__device__ __forceinline__ void expand_tiles(Op &op, const int32_t *__restrict__ rowptr,
+                                             const int32_t *__restrict__ col, int32_t nnz,
+                                             FrontierBufs cur, int32_t F, int32_t E,
+                                             FrontierBufs nxt, uint64_t *cnt_next,
+                                             bool prefetch = false) {
+    __shared__ ExpandLds<BLOCK, TILE, STORE_V_IN> L;
+    expand_tiles<BLOCK, TILE, STORE_V_IN, STORE_V_OUT, PRODUCE>(L, op, rowptr, col, nnz, cur, F,
+                                                                 E, nxt, cnt_next, prefetch);
 }
 
 template <int BLOCK, int TILE, bool STORE_V_IN, bool STORE_V_OUT, bool PRODUCE, class Op>

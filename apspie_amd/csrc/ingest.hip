@@ -76,15 +76,31 @@ __global__ void k_keep(const int32_t *row, const int32_t *col, const uint32_t *i
     }
 }
 
+// output entry i = input entry idx[kept[i]] (the entries arrive sorted by
+// output row): column, value and row id
 __global__ void k_emit(const int32_t *row, const int32_t *col, const float *val,
                        const uint32_t *idx, const int32_t *kept, int64_t k, int64_t nnz_in,
-                       int32_t *ocol, float *oval, int32_t *rowcount) {
+                       int32_t *ocol, float *oval, int32_t *orow) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < k;
          i += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t a = idx[kept ? kept[i] : i];
         ocol[i] = col[a];
         if (oval) oval[i] = val ? val[a < nnz_in ? a : a - nnz_in] : 1.0f;
-        atomicAdd(rowcount + row[a], 1);
+        orow[i] = row[a];
+    }
+}
+
+// rowptr[r] = number of entries with row < r, by binary search over the
+// sorted row ids (no per-entry atomics: a hub row would serialise them)
+__global__ void k_rowptr_search(const int32_t *orow, int64_t k, int32_t m, int32_t *rowptr) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r <= m;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        int64_t lo = 0, hi = k;  // first i with orow[i] >= r
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (orow[mid] < r) lo = mid + 1; else hi = mid;
+        }
+        rowptr[r] = (int32_t)lo;
     }
 }
 
@@ -101,10 +117,10 @@ __global__ void k_entry_rows(const int32_t *rowptr, int32_t m, int64_t nnz, int3
     }
 }
 
-void finish_rowptr(Ctx *c, Graph *g, int32_t *rowcount) {
-    fill_i32(c, rowcount + g->m, 0, 1);
-    const int64_t tot = scan_exclusive_i32(c, rowcount, g->rowptr, g->m + 1);
-    APSPIE_CHECK(tot == g->nnz, APSPIE_ERR_DEVICE, "row counts do not add up");
+void finish_rowptr(Ctx *c, Graph *g, const int32_t *orow) {
+    const int grid = fill_grid(c, kBlock, 4);
+    k_rowptr_search<<<grid, kBlock, 0, c->stream>>>(orow, g->nnz, g->m, g->rowptr);
+    APSPIE_LAUNCH_CHECK();
 }
 
 }  // namespace
@@ -152,14 +168,13 @@ void graph_from_coo_dev(Ctx *c, Graph *g, const int32_t *row, const int32_t *col
     g->col = (int32_t *)dev_alloc(c, sizeof(int32_t) * std::max<int64_t>(k, 1));
     g->val = (float *)dev_alloc(c, sizeof(float) * std::max<int64_t>(k, 1));
     g->bytes = (int64_t)(sizeof(int32_t) * (g->m + 1) + 8 * std::max<int64_t>(k, 1));
-    int32_t *rowcount = (int32_t *)scratch(c, "ing_rowcount", sizeof(int32_t) * (g->m + 1));
-    fill_i32(c, rowcount, 0, g->m + 1);
+    int32_t *orow = (int32_t *)scratch(c, "ing_orow", sizeof(int32_t) * std::max<int64_t>(k, 1));
     if (k > 0) {
         k_emit<<<grid, kBlock, 0, c->stream>>>(r2, c2, val, idx, symmetrize ? kept : nullptr, k,
-                                               nnz, g->col, g->val, rowcount);
+                                               nnz, g->col, g->val, orow);
         APSPIE_LAUNCH_CHECK();
     }
-    finish_rowptr(c, g, rowcount);
+    finish_rowptr(c, g, orow);
 }
 
 __global__ void k_neq(const int32_t *a, const int32_t *b, int64_t n, int32_t *flag) {
@@ -215,15 +230,14 @@ void graph_transpose_dev(Ctx *c, const Graph *g, Graph *t) {
     t->col = (int32_t *)dev_alloc(c, sizeof(int32_t) * Ea);
     t->val = g->val ? (float *)dev_alloc(c, sizeof(float) * Ea) : nullptr;
     t->bytes = (int64_t)(sizeof(int32_t) * (t->m + 1) + (g->val ? 8 : 4) * Ea);
-    int32_t *rowcount = (int32_t *)scratch(c, "tr_rowcount", sizeof(int32_t) * (t->m + 1));
-    fill_i32(c, rowcount, 0, t->m + 1);
+    int32_t *orow = (int32_t *)scratch(c, "tr_orow", sizeof(int32_t) * Ea);
     if (E > 0) {
         // new row = old column, new column = old row
         k_emit<<<grid, kBlock, 0, c->stream>>>(g->col, rows, g->val, idx, nullptr, E, E, t->col,
-                                               t->val, rowcount);
+                                               t->val, orow);
         APSPIE_LAUNCH_CHECK();
     }
-    finish_rowptr(c, t, rowcount);
+    finish_rowptr(c, t, orow);
 }
 
 }  // namespace apspie

@@ -7,7 +7,9 @@
 // (test/tc/main.cu:249).  Here the product is never materialised: for every
 // row r of L (the columns of row r below r, a prefix of the sorted row)
 //     per_row[r] = sum_{k in L_r} |L_r cap L_k|
-// = triangles whose largest vertex is r.  Rows are binned by |L_r|:
+// = triangles whose largest vertex is r; each intersection is evaluated
+// from its shorter side (see the pair split below the table helpers).
+// Rows are binned by |L_r|:
 //   * small rows: one wave per row, L_r in a 256-slot LDS hash table;
 //   * medium rows: one block per row, 4096-slot LDS table;
 //   * large rows: one block per row, table in global memory (spill).
@@ -67,7 +69,7 @@ __global__ void k_tc_bin(const int32_t *lowcnt, int32_t m, int32_t *small, int32
         const int64_t r = r0 + threadIdx.x;
         int32_t n = r < m ? lowcnt[r] : 0;
         int bin = n < 2 ? -1 : (n <= kSmallRow ? 0 : (n <= kMediumRow ? 1 : 2));
-        if (r < m && bin < 0 && per_row) per_row[r] = 0;
+        if (r < m && per_row) per_row[r] = 0;  // rows add into each other's counts
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
             const uint64_t mask = __ballot(bin == b);
@@ -105,11 +107,25 @@ __device__ __forceinline__ bool table_has(const int32_t *table, uint32_t mask, i
     }
 }
 
+// Pairs (i, k), k in L_i, are split by size so that an intersection
+// |L_i cap L_k| with a long L_k is counted by probing the shorter list into a
+// table of the longer one (the work drops from sum |L_k| towards
+// sum min(|L_i|, |L_k|)):
+//   |L_k| <= max(|L_i|, kSmallRow): "own" pair of row i -- L_k is scanned
+//     against the table of L_i;
+//   otherwise: "upper" pair of row k (a block-bin row) -- row k holds L_k in
+//     its table and scans L_i for every i in U_k (the part of row k above the
+//     diagonal) with |L_i| < |L_k|; the count goes to row i.
+// Small rows never take upper pairs (a low-id hub has a short L and a huge U).
+// Both enumerations of a row share its table and are load-balanced the same
+// way (exclusive scan of the scanned lengths, one lane per wedge).
+
 // one wave per small row
-__global__ __launch_bounds__(kBlock) void k_tc_small(const int32_t *rowptr, const int32_t *col, int32_t nnz,
-                                                     const int32_t *lowcnt, const int32_t *rows,
-                                                     const uint32_t *cnt, int32_t *per_row,
-                                                     uint64_t *total, uint64_t *wedges) {
+__global__ __launch_bounds__(kBlock) void k_tc_small(const int32_t *rowptr, const int32_t *col,
+                                                     int32_t nnz, const int32_t *lowcnt,
+                                                     const int32_t *rows, const uint32_t *cnt,
+                                                     int32_t *per_row, uint64_t *total,
+                                                     uint64_t *wedges) {
     constexpr int NW = kBlock / 64;
     __shared__ int32_t s_table[NW][kSmallTable];
     __shared__ int32_t s_pre[NW][kSmallRow + 1];
@@ -117,12 +133,13 @@ __global__ __launch_bounds__(kBlock) void k_tc_small(const int32_t *rowptr, cons
     const int w = threadIdx.x / 64, lane = dev::lane_id();
     const int32_t nrows = (int32_t)cnt[0];
     uint64_t my_tri = 0, my_wedges = 0;
-    for (int32_t i = blockIdx.x * NW + w; i < nrows; i += gridDim.x * NW) {
-        const int32_t r = rows[i];
+    for (int32_t ri = blockIdx.x * NW + w; ri < nrows; ri += gridDim.x * NW) {
+        const int32_t r = rows[ri];
         const int32_t n = lowcnt[r];
         const int32_t base = rowptr[r];
         for (int s = lane; s < kSmallTable; s += 64) s_table[w][s] = -1;
         dev::wave_sync();
+        // own pairs: every k in L_r with |L_k| <= kSmallRow
         uint32_t carry = 0;
         for (int c0 = 0; c0 < n; c0 += 64) {
             const int idx = c0 + lane;
@@ -131,7 +148,8 @@ __global__ __launch_bounds__(kBlock) void k_tc_small(const int32_t *rowptr, cons
                 const int32_t k = col[base + idx];
                 table_insert(s_table[w], kSmallTable - 1, k);
                 s_rs[w][idx] = rowptr[k];
-                lc = (uint32_t)lowcnt[k];
+                const int32_t l = lowcnt[k];
+                lc = l <= kSmallRow ? (uint32_t)l : 0u;  // n <= kSmallRow
             }
             const uint32_t inc = dev::wave_inclusive_add(lc);
             if (idx < n) s_pre[w][idx] = (int32_t)(carry + inc - lc);
@@ -148,48 +166,56 @@ __global__ __launch_bounds__(kBlock) void k_tc_small(const int32_t *rowptr, cons
             c += table_has(s_table[w], kSmallTable - 1, col[e]) ? 1u : 0u;
         }
         const uint64_t rc = dev::wave_sum_u64(c);
-        if (lane == 0) per_row[r] = (int32_t)rc;
-        my_tri += rc;
-        my_wedges += (uint64_t)W;
+        if (lane == 0) {
+            if (rc && per_row) atomicAdd(per_row + r, (int32_t)rc);
+            my_tri += rc;
+            my_wedges += (uint64_t)W;
+        }
         dev::wave_sync();
     }
-    if (lane == 0 && (my_tri || my_wedges)) {
-        dev::atomic_add_dev(total, my_tri);
-        dev::atomic_add_dev(wedges, my_wedges);
+    const uint64_t t = dev::wave_sum_u64(my_tri), wd = dev::wave_sum_u64(my_wedges);
+    if (lane == 0 && (t || wd)) {
+        dev::atomic_add_dev(total, t);
+        dev::atomic_add_dev(wedges, wd);
     }
 }
 
 // one block per medium / large row; TABLE > 0: LDS table of that size,
 // TABLE == 0: per-block region of global scratch (spill)
 template <int TABLE>
-__global__ __launch_bounds__(kBlock) void k_tc_block(const int32_t *rowptr, const int32_t *col, int32_t nnz,
-                                                     const int32_t *lowcnt, const int32_t *rows,
-                                                     const uint32_t *cnt, int32_t *per_row,
-                                                     uint64_t *total, uint64_t *wedges,
-                                                     int32_t *g_table, int32_t *g_pre,
-                                                     int32_t *g_rs, int64_t g_cap) {
+__global__ __launch_bounds__(kBlock) void k_tc_block(const int32_t *rowptr, const int32_t *col,
+                                                     int32_t nnz, const int32_t *lowcnt,
+                                                     const int32_t *rows, const uint32_t *cnt,
+                                                     int32_t *per_row, uint64_t *total,
+                                                     uint64_t *wedges, int32_t *g_table,
+                                                     int32_t *g_pre, int32_t *g_rs,
+                                                     int32_t *g_cnt, int64_t g_cap) {
     constexpr int LDS_T = TABLE > 0 ? TABLE : 1;
     constexpr int LDS_R = TABLE > 0 ? TABLE / 2 : 1;
     __shared__ int32_t s_table[LDS_T];
     __shared__ int32_t s_pre[LDS_R + 1];
     __shared__ int32_t s_rs[LDS_R];
+    __shared__ int32_t s_cnt[LDS_R];
     __shared__ uint32_t s_scan[kBlock / 64 + 1];
     __shared__ uint64_t s_red[kBlock / 64];
     int32_t *table = TABLE > 0 ? s_table : g_table + (int64_t)blockIdx.x * g_cap;
     int32_t *pre = TABLE > 0 ? s_pre : g_pre + (int64_t)blockIdx.x * (g_cap / 2 + 1);
     int32_t *rsk = TABLE > 0 ? s_rs : g_rs + (int64_t)blockIdx.x * (g_cap / 2);
+    int32_t *cntu = TABLE > 0 ? s_cnt : g_cnt + (int64_t)blockIdx.x * (g_cap / 2);
+    const int32_t chunk = TABLE > 0 ? LDS_R : (int32_t)(g_cap / 2);
     const int32_t nrows = (int32_t)cnt[0];
     uint64_t my_tri = 0, my_wedges = 0;
-    for (int32_t i = blockIdx.x; i < nrows; i += gridDim.x) {
-        const int32_t r = rows[i];
+    for (int32_t ri = blockIdx.x; ri < nrows; ri += gridDim.x) {
+        const int32_t r = rows[ri];
         const int32_t n = lowcnt[r];
-        const int32_t base = rowptr[r];
+        const int32_t base = rowptr[r], rend = rowptr[r + 1];
         uint32_t tsize = 1;
         while (tsize < 2u * (uint32_t)n) tsize <<= 1;
         if (TABLE > 0) tsize = TABLE;
         const uint32_t mask = tsize - 1;
         for (uint32_t s = threadIdx.x; s < tsize; s += kBlock) table[s] = -1;
         __syncthreads();
+        // own pairs: k in L_r with |L_k| <= |L_r|
         uint32_t carry = 0;
         for (int c0 = 0; c0 < n; c0 += kBlock) {
             const int idx = c0 + threadIdx.x;
@@ -198,7 +224,8 @@ __global__ __launch_bounds__(kBlock) void k_tc_block(const int32_t *rowptr, cons
                 const int32_t k = col[base + idx];
                 table_insert(table, mask, k);
                 rsk[idx] = rowptr[k];
-                lc = (uint32_t)lowcnt[k];
+                const int32_t l = lowcnt[k];
+                lc = l <= n ? (uint32_t)l : 0u;
             }
             uint32_t tot;
             const uint32_t ex = dev::block_exclusive_add32<kBlock>(lc, s_scan, tot);
@@ -207,7 +234,7 @@ __global__ __launch_bounds__(kBlock) void k_tc_block(const int32_t *rowptr, cons
         }
         if (threadIdx.x == 0) pre[n] = (int32_t)carry;
         __syncthreads();
-        const int32_t W = (int32_t)carry;
+        int32_t W = (int32_t)carry;
         uint32_t c = 0;
         for (int32_t x = threadIdx.x; x < W; x += kBlock) {
             const int j = dev::upper_bound_m1(pre, 0, n, x);
@@ -221,11 +248,56 @@ __global__ __launch_bounds__(kBlock) void k_tc_block(const int32_t *rowptr, cons
         if (threadIdx.x == 0) {
             uint64_t rc = 0;
             for (int q = 0; q < kBlock / 64; ++q) rc += s_red[q];
-            per_row[r] = (int32_t)rc;
+            if (rc && per_row) atomicAdd(per_row + r, (int32_t)rc);
             my_tri += rc;
             my_wedges += (uint64_t)W;
         }
         __syncthreads();
+        // upper pairs: i in U_r with |L_i| < |L_r|, in chunks
+        for (int32_t u0 = base + n; u0 < rend; u0 += chunk) {
+            const int32_t cn = min(chunk, rend - u0);
+            carry = 0;
+            for (int c0 = 0; c0 < cn; c0 += kBlock) {
+                const int idx = c0 + threadIdx.x;
+                uint32_t li = 0;
+                if (idx < cn) {
+                    const int32_t i = col[u0 + idx];
+                    rsk[idx] = rowptr[i];
+                    cntu[idx] = 0;
+                    const int32_t l = lowcnt[i];
+                    li = l < n ? (uint32_t)l : 0u;
+                }
+                uint32_t tot;
+                const uint32_t ex = dev::block_exclusive_add32<kBlock>(li, s_scan, tot);
+                if (idx < cn) pre[idx] = (int32_t)(carry + ex);
+                carry += tot;
+            }
+            if (threadIdx.x == 0) pre[cn] = (int32_t)carry;
+            __syncthreads();
+            W = (int32_t)carry;
+            for (int32_t x = threadIdx.x; x < W; x += kBlock) {
+                const int j = dev::upper_bound_m1(pre, 0, cn, x);
+                const int32_t e = rsk[j] + (x - pre[j]);
+                if ((uint32_t)e >= (uint32_t)nnz) continue;  // guard only
+                if (table_has(table, mask, col[e])) atomicAdd(cntu + j, 1);
+            }
+            if (threadIdx.x == 0) my_wedges += (uint64_t)W;
+            __syncthreads();
+            uint64_t cu = 0;
+            for (int idx = threadIdx.x; idx < cn; idx += kBlock) {
+                const int32_t ci = cntu[idx];
+                if (ci) {
+                    if (per_row) atomicAdd(per_row + col[u0 + idx], ci);
+                    cu += (uint64_t)ci;
+                }
+            }
+            const uint64_t wcu = dev::wave_sum_u64(cu);
+            if (dev::lane_id() == 0) s_red[threadIdx.x / 64] = wcu;
+            __syncthreads();
+            if (threadIdx.x == 0)
+                for (int q = 0; q < kBlock / 64; ++q) my_tri += s_red[q];
+            __syncthreads();  // rsk / pre / cntu are refilled by the next chunk
+        }
     }
     if (threadIdx.x == 0 && (my_tri || my_wedges)) {
         dev::atomic_add_dev(total, my_tri);
@@ -270,7 +342,7 @@ void tc_run(Ctx *c, const Graph *g0, uint64_t *total_out, int32_t *per_row) {
     uint32_t *ctl = (uint32_t *)scratch(c, "tc_ctl", 64);
     uint64_t *acc = (uint64_t *)(ctl + 8);  // [0] triangles, [1] wedges
     fill_i32(c, (int32_t *)ctl, 0, 16);
-    int32_t *per = per_row ? per_row : (int32_t *)scratch(c, "tc_rows", sizeof(int32_t) * m);
+    int32_t *per = per_row;  // may be null: total only, no per-row atomics
     int32_t *lists = (int32_t *)scratch(c, "tc_lists", sizeof(int32_t) * 3 * (size_t)m);
     k_tc_bin<<<grid, kBlock, 0, c->stream>>>(g->lowcnt, m, lists, lists + m, lists + 2 * (size_t)m,
                                             ctl, per);
@@ -291,8 +363,8 @@ void tc_run(Ctx *c, const Graph *g0, uint64_t *total_out, int32_t *per_row) {
     if (h[1]) {
         const int blocks = (int)std::min<int64_t>(h[1], 4 * c->cus);
         k_tc_block<kMediumTable><<<blocks, kBlock, 0, c->stream>>>(
-            g->rowptr, g->col, (int32_t)g->nnz, g->lowcnt, lists + m, ctl + 1, per, acc, acc + 1, nullptr, nullptr,
-            nullptr, 0);
+            g->rowptr, g->col, (int32_t)g->nnz, g->lowcnt, lists + m, ctl + 1, per, acc, acc + 1,
+            nullptr, nullptr, nullptr, nullptr, 0);
         APSPIE_LAUNCH_CHECK();
     }
     if (h[2]) {
@@ -302,9 +374,10 @@ void tc_run(Ctx *c, const Graph *g0, uint64_t *total_out, int32_t *per_row) {
         int32_t *gt = (int32_t *)scratch(c, "tc_gtable", sizeof(int32_t) * cap * blocks);
         int32_t *gp = (int32_t *)scratch(c, "tc_gpre", sizeof(int32_t) * (cap / 2 + 1) * blocks);
         int32_t *gr = (int32_t *)scratch(c, "tc_grs", sizeof(int32_t) * (cap / 2) * blocks);
-        k_tc_block<0><<<blocks, kBlock, 0, c->stream>>>(g->rowptr, g->col, (int32_t)g->nnz, g->lowcnt,
-                                                        lists + 2 * (size_t)m, ctl + 2, per, acc,
-                                                        acc + 1, gt, gp, gr, cap);
+        int32_t *gc = (int32_t *)scratch(c, "tc_gcnt", sizeof(int32_t) * (cap / 2) * blocks);
+        k_tc_block<0><<<blocks, kBlock, 0, c->stream>>>(g->rowptr, g->col, (int32_t)g->nnz,
+                                                        g->lowcnt, lists + 2 * (size_t)m, ctl + 2,
+                                                        per, acc, acc + 1, gt, gp, gr, gc, cap);
         APSPIE_LAUNCH_CHECK();
     }
     uint64_t res[2];

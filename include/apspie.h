@@ -191,6 +191,15 @@ int apspie_spmspv(apspie_ctx *ctx, const apspie_graph *g, int semiring, int64_t 
  * sorted rows. */
 int apspie_spgemm(apspie_ctx *ctx, const apspie_graph *a, const apspie_graph *b,
                   apspie_graph **c);
+/* Masked product C = (A * B) o M over (+,x), evaluated on M's pattern only:
+ * c_val[e] = sum_k A[i,k] * B[k,j] for every entry e = (i,j) of M in M's CSR
+ * order (0 where no product lands; a column repeated within a row of M gets
+ * its value at the first occurrence), *c_nnz = entries of M with at least one
+ * product.  Replaces the TC composition's materialised product + mask,
+ * spgemm() + ewiseMultTc() (tc.cuh:62-199).  Unweighted graphs count as all
+ * ones.  c_val (nnz(M) floats) lives in `mem`. */
+int apspie_spgemm_masked(apspie_ctx *ctx, const apspie_graph *a, const apspie_graph *b,
+                         const apspie_graph *mask, float *c_val, int64_t *c_nnz, int mem);
 /* Exclusive prefix sum of n int32 (mgpu::Scan<MgpuScanTypeExc>); *total on host. */
 int apspie_scan_exclusive(apspie_ctx *ctx, const int32_t *in, int32_t *out, int64_t n,
                           int64_t *total, int mem);

@@ -222,6 +222,18 @@ inline void spgemm(d_matrix *C, d_matrix *A, d_matrix *B, int /*partSize*/, int 
     attach(C, c, ctx);
 }
 
+// The TC driver's product-then-mask step (spgemm tc.cuh:62-165 followed by
+// ewiseMultTc tc.cuh:168-199) as one masked product: d_cVal[e] = (A*B)[i,j]
+// for every entry e = (i,j) of M (device, nnz(M) floats); returns the number
+// of entries of M that received at least one product.
+inline int64_t spgemmMasked(float *d_cVal, d_matrix *A, d_matrix *B, d_matrix *M,
+                            Context &ctx) {
+    int64_t nnz = 0;
+    check(apspie_spgemm_masked(ctx.get(), A->g, B->g, M->g, d_cVal, &nnz, APSPIE_MEM_DEVICE),
+          ctx.get(), "spgemmMasked");
+    return nnz;
+}
+
 // mXv (mXv.cuh:249-250), sparse in / sparse out: y = A^T x under (+,x) on
 // device arrays; returns nnz(y) (the reference's in/out `nnz`).
 template <typename T>

@@ -132,6 +132,73 @@ def test_spgemm_vs_oracle(ctx, case):
         G.close()
 
 
+def _masked_want(Am, Bm, Mm):
+    """The oracle's full product, read at the mask's entries."""
+    wrp, wc, wv = O.spgemm(Am.shape[0], Am.shape[1], Bm.shape[1], Am.indptr.astype(np.int64),
+                           Am.indices.astype(np.int32), Am.data.astype(np.float32),
+                           Bm.indptr.astype(np.int64), Bm.indices.astype(np.int32),
+                           Bm.data.astype(np.float32))
+    vals, hit = [], 0
+    for i in range(Mm.shape[0]):
+        row = dict(zip(wc[wrp[i]:wrp[i + 1]].tolist(), wv[wrp[i]:wrp[i + 1]].tolist()))
+        for j in Mm.indices[Mm.indptr[i]:Mm.indptr[i + 1]].tolist():
+            vals.append(row.get(j, 0.0))
+            hit += j in row
+    return np.array(vals, np.float32), hit
+
+
+@pytest.mark.parametrize("case", ["rect", "rmat_tc", "long_mask_row", "empty_rows"])
+def test_spgemm_masked_vs_oracle(ctx, case):
+    """(A*B) o M on M's pattern equals the oracle's full product read at M's
+    entries; on L, L, L the sum is the triangle count."""
+    rng = np.random.default_rng(8)
+    if case == "rect":
+        Am = sp.random(300, 200, 0.05, random_state=1, format="csr")
+        Bm = sp.random(200, 400, 0.05, random_state=2, format="csr")
+        Mm = sp.random(300, 400, 0.1, random_state=3, format="csr")
+    elif case == "rmat_tc":
+        g = T.rmat(10)
+        L = sp.tril(sp.csr_matrix((np.ones(len(g.col)), g.col, g.rowptr), shape=(g.m, g.m)),
+                    k=-1).tocsr()
+        Am = Bm = Mm = L
+    elif case == "long_mask_row":  # a mask row beyond the LDS table (global tier)
+        Am = sp.random(20, 300, 0.3, random_state=4, format="csr")
+        Bm = sp.random(300, 9000, 0.01, random_state=5, format="csr")
+        Mm = sp.lil_matrix((20, 9000))
+        Mm[0, :] = 1
+        Mm[1, ::3] = 1
+        Mm = Mm.tocsr()
+    else:
+        Am = sp.random(100, 100, 0.02, random_state=6, format="csr")
+        Bm = sp.random(100, 100, 0.02, random_state=7, format="csr")
+        Mm = sp.random(100, 100, 0.05, random_state=8, format="csr")
+        Mm = sp.csr_matrix(Mm.multiply(np.arange(100)[:, None] % 3 == 0))
+    for X in (Am, Bm, Mm):
+        X.sort_indices()
+    if case != "rmat_tc":
+        for X in (Am, Bm):
+            X.data = rng.integers(1, 5, X.nnz).astype(np.float64)
+    Gs = [ctx.upload(X.indptr.astype(np.int32), X.indices.astype(np.int32),
+                     X.data.astype(np.float32), n=X.shape[1]) for X in (Am, Bm, Mm)]
+    got, nnz = ctx.spgemm_masked(*Gs)
+    want, hit = _masked_want(Am, Bm, Mm)
+    assert np.array_equal(got, want) and nnz == hit, case
+    if case == "rmat_tc":
+        assert int(got.sum()) == T.GOLD["rmat10"]["triangles"]
+    for G in Gs:
+        G.close()
+
+
+def test_spgemm_masked_rejects_bad_shapes(ctx):
+    A1 = ctx.upload(np.array([0, 1], np.int32), np.array([0], np.int32), n=2)
+    B1 = ctx.upload(np.array([0, 1, 1], np.int32), np.array([1], np.int32), n=3)
+    M_bad = ctx.upload(np.array([0, 1], np.int32), np.array([0], np.int32), n=2)
+    with pytest.raises(A.ApspieError):
+        ctx.spgemm_masked(A1, B1, M_bad)  # mask is 1x2, product is 1x3
+    for G in (A1, B1, M_bad):
+        G.close()
+
+
 @pytest.mark.parametrize("name", T.FIXTURES)
 def test_graph_from_coo_matches_host_build(ctx, name):
     path = T.GOLDEN_DIR / "mtx" / f"{name}.mtx"

@@ -45,6 +45,55 @@ def test_sssp_synthetic_vs_oracle(ctx, key):
     G.close()
 
 
+@pytest.mark.parametrize("delta", [1, 7, 64, 1000, 1 << 20])
+@pytest.mark.parametrize("key", ["rmat10", "rmat14", "lattice64", "test_cc", "directed"])
+def test_sssp_near_far_vs_oracle(ctx, key, delta):
+    """Near-far rounds (sssp_delta > 0) reach the same fixed point as the
+    Bellman-Ford frontier, for thresholds from below the smallest weight to
+    above every distance; also on a directed graph and the fixtures."""
+    if key == "directed":
+        rng = np.random.default_rng(31)
+        n = 3000
+        u = rng.integers(0, n, 20000)
+        v = rng.integers(0, n, 20000)
+        k = np.unique((u * n + v)[u != v])
+        rp, c, _ = O.coo_to_csr(n, (k // n).astype(np.int32), (k % n).astype(np.int32))
+        w = (1 + rng.integers(0, 255, len(c))).astype(np.float32)
+        m, rowptr, col, wt = n, rp, c, w
+    else:
+        g = T.fixture(key) if key == "test_cc" else (
+            T.rmat(int(key[4:])) if key.startswith("rmat") else T.lattice(int(key[7:])))
+        m, rowptr, col, wt = g.m, g.rowptr, g.col, g.w
+    ctx.set_option("sssp_delta", delta)
+    try:
+        G = ctx.upload(rowptr.astype(np.int32), col, wt)
+        deg = np.diff(rowptr)
+        srcs = [int(s) for s in np.nonzero(deg > 0)[0][[0, -1]]] + [int(np.argmax(deg))]
+        for src in srcs:
+            want = to_ref_sentinel(O.sssp(m, rowptr, col, wt, src))
+            got = ctx.sssp(G, src)
+            assert np.array_equal(got, want), (key, delta, src)
+        G.close()
+    finally:
+        ctx.set_option("sssp_delta", 0)
+
+
+def test_sssp_near_far_edge_cases(ctx):
+    ctx.set_option("sssp_delta", 16)
+    try:
+        g = T.path_graph(200)  # one vertex per round, many splits
+        G = ctx.upload(g.rowptr32, g.col, np.full(len(g.col), 3, np.float32))
+        assert ctx.sssp(G, 0).tolist() == [3.0 * i for i in range(200)]
+        G.close()
+        g = T.empty_graph(5)  # isolated source: the first round is already a split
+        G = ctx.upload(g.rowptr32, g.col, np.zeros(0, np.float32))
+        got = ctx.sssp(G, 2)
+        assert got[2] == 0 and np.all(np.delete(got, 2) == A.SSSP_INF)
+        G.close()
+    finally:
+        ctx.set_option("sssp_delta", 0)
+
+
 def test_sssp_edge_cases(ctx):
     g = T.path_graph(200)
     w = np.full(len(g.col), 3, np.float32)
