@@ -275,6 +275,172 @@ __global__ __launch_bounds__(kBlock) void k_batch_bfs_level(
                                                    cnt_next);
 }
 
+// Near-far rounds for the batch (the single-source scheme with one threshold
+// T shared by all sources): a relaxation that lowers dist[v][s] below T sets
+// bit s of next[v]; one that leaves it at or above T sets bit s of pend[v].
+// A round that finds the union frontier empty is a split round: T grows by
+// delta and a sweep (a group of W lanes per vertex, lane w owning word w)
+// moves the pending bits now below T into next, drops the ones that fell
+// below the old T since (already expanded) and keeps the rest.
+// ctl: [12 + (r & 1)] T of round r, [14 + (r & 1)] round r - 1 was a split,
+// pending bits left by the split of round r in the spread slots (r % 3).
+template <int W>
+struct SsspBatchNearOp {
+    const uint32_t *front;
+    uint32_t *next;
+    uint32_t *pend;
+    int32_t *stamp;
+    int32_t *dist;  // [m][32W] float bits
+    const float *wt;
+    const int32_t *lvl;  // device round counter
+    int32_t tbits;
+    __device__ __forceinline__ bool edge(int32_t u, int32_t e, int32_t v) const {
+        const int32_t round = *lvl;
+        constexpr int B = 32 * W;
+        const float we = wt[e];
+        bool any = false;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            uint32_t f = front[(int64_t)u * W + w];
+            uint32_t near = 0, far = 0;
+            while (f) {
+                const int b = __ffs(f) - 1;
+                f &= f - 1;
+                const int s = w * 32 + b;
+                const int32_t nd = dev::fbits(dev::bitsf(dist[(int64_t)u * B + s]) + we);
+                int32_t *pv = dist + (int64_t)v * B + s;
+                if (nd >= *pv) continue;
+                if (nd < dev::atomic_min_dev(pv, nd)) {
+                    if (nd < tbits) near |= 1u << b;
+                    else far |= 1u << b;
+                }
+            }
+            if (near) {
+                __hip_atomic_fetch_or(next + (int64_t)v * W + w, near, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+                any = true;
+            }
+            if (far)
+                __hip_atomic_fetch_or(pend + (int64_t)v * W + w, far, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+        }
+        return any && dev::atomic_exch_dev(stamp + v, round) != round;
+    }
+};
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_batch_sssp_nf(
+    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col, int32_t nnz, int32_t m,
+    SsspBatchNearOp<W> op, float delta, FrontierBufs cur, FrontierBufs nxt, uint64_t *ctl,
+    int32_t r) {
+    constexpr int B = 32 * W;
+    __shared__ BatchLds lds;
+    __shared__ uint64_t s_part[kBlock / 64];
+    const uint64_t cc = ctl[r % 3];
+    const int32_t F = (int32_t)cnt_entries(cc), E = (int32_t)cnt_edges(cc);
+    const int32_t tbits = (int32_t)ctl[12 + (r & 1)];
+    const bool split_last = ctl[14 + (r & 1)] != 0;
+    uint64_t left_before = 0;
+    for (int i = 0; i < kSpread; ++i) left_before += ctl[16 + i * 16 + (r + 2) % 3];
+    const bool split = F == 0;
+    const bool done = split && split_last && left_before == 0;
+    const int32_t tnext = split && !done ? dev::fbits(dev::bitsf(tbits) + delta) : tbits;
+    __syncthreads();  // every thread has read the control words
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl[(r + 2) % 3] = 0;
+        ctl[12 + ((r + 1) & 1)] = (uint64_t)(uint32_t)tnext;
+        ctl[14 + ((r + 1) & 1)] = split ? 1 : 0;
+        for (int i = 0; i < kSpread; ++i) ctl[16 + i * 16 + (r + 1) % 3] = 0;
+        if (F > 0) {
+            ctl[4] += (uint64_t)E;
+            ctl[5] += (uint64_t)F;
+            ctl[6] += 1;
+        }
+    }
+    if (done) return;
+    if (!split) {
+        op.tbits = tbits;
+        expand_tiles<kBlock, kTile, true, true, true>(lds.ex, op, rowptr, col, nnz, cur, F, E,
+                                                       nxt, ctl + (r + 1) % 3);
+        return;
+    }
+    // split: one group of W lanes per vertex
+    constexpr int VPB = kBlock / W;
+    constexpr int KP = 4;
+    PullQueue &q = lds.pq;
+    if (threadIdx.x == 0) q.n = 0;
+    __syncthreads();
+    const int g = threadIdx.x / W, w = threadIdx.x % W;
+    const int lane = dev::lane_id();
+    const uint64_t gmask = W == 64 ? ~0ull : (((1ull << W) - 1) << ((lane / W) * W));
+    const int32_t round = *op.lvl;
+    uint64_t left = 0;
+    for (int64_t pbase = (int64_t)blockIdx.x * VPB * KP; pbase < m;
+         pbase += (int64_t)gridDim.x * VPB * KP) {
+        int32_t cv[KP], crs[KP], cd[KP];
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            const int64_t v = pbase + (int64_t)k * VPB + g;
+            cv[k] = (int32_t)v;
+            crs[k] = 0;
+            cd[k] = 0;
+            uint32_t moved = 0;
+            if (v < m) {
+                const int64_t i = v * W + w;
+                uint32_t pb = op.pend[i];
+                if (pb) {
+                    uint32_t keep = 0, x = pb;
+                    while (x) {
+                        const int b = __ffs(x) - 1;
+                        x &= x - 1;
+                        const int32_t d = op.dist[v * B + w * 32 + b];
+                        if (d < tbits) continue;  // lowered below T since: expanded
+                        if (d < tnext) moved |= 1u << b;
+                        else keep |= 1u << b;
+                    }
+                    op.pend[i] = keep;
+                    left += (uint64_t)__popc(keep);
+                    if (moved) op.next[i] = moved;
+                }
+            }
+            const uint64_t bm = __ballot(moved != 0);
+            if (w == 0 && (bm & gmask)) {
+                op.stamp[cv[k]] = round;
+                const int32_t rs = rowptr[cv[k]];
+                crs[k] = rs;
+                cd[k] = rowptr[cv[k] + 1] - rs;
+            }
+        }
+        uint32_t c = 0;
+#pragma unroll
+        for (int k = 0; k < KP; ++k) c += cd[k] > 0 ? 1u : 0u;
+        uint32_t total;
+        uint32_t pos = dev::block_exclusive_add32<kBlock>(c, q.scan32, total);
+        if (q.n + total > (uint32_t)kQueue) queue_flush(q, nxt, ctl + (r + 1) % 3);
+        pos += q.n;
+#pragma unroll
+        for (int k = 0; k < KP; ++k)
+            if (cd[k] > 0) {
+                q.v[pos] = cv[k];
+                q.rs[pos] = crs[k];
+                q.d[pos] = cd[k];
+                ++pos;
+            }
+        __syncthreads();
+        if (threadIdx.x == 0) q.n += total;
+        __syncthreads();
+    }
+    if (q.n > 0) queue_flush(q, nxt, ctl + (r + 1) % 3);
+    const uint64_t ws = dev::wave_sum_u64(left);
+    if (lane == 0) s_part[threadIdx.x / 64] = ws;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (int qq = 0; qq < kBlock / 64; ++qq) t += s_part[qq];
+        if (t) dev::atomic_add_dev(ctl + 16 + (blockIdx.x % kSpread) * 16 + r % 3, t);
+    }
+}
+
 // visited words at the start: bits of sources beyond nsrc count as visited
 // (the last word of a batch), and with a symmetric CSR a vertex without
 // neighbours counts as visited by every source (it is never reached)
@@ -293,11 +459,13 @@ __global__ void k_batch_vis_init(const int32_t *__restrict__ rowptr, int32_t m, 
 }
 
 // zero the frontier rows of the entries just expanded; advance the level
+// (every round when `every`: near-far rounds number their claims by round,
+// split rounds included)
 template <int W>
 __global__ void k_clear_rows(FrontierBufs cur, const uint64_t *cnt, uint32_t *front,
-                             int32_t *lvl) {
+                             int32_t *lvl, bool every) {
     const int32_t F = (int32_t)cnt_entries(*cnt);
-    if (blockIdx.x == 0 && threadIdx.x == 0 && F > 0) *lvl += 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (F > 0 || every)) *lvl += 1;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)F * W;
          i += (int64_t)gridDim.x * blockDim.x)
         front[(int64_t)cur.v[i / W] * W + (i % W)] = 0;
@@ -362,6 +530,11 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
     APSPIE_HIP(hipMemcpyAsync(ulist, uniq.data(), sizeof(int32_t) * uniq.size(),
                               hipMemcpyHostToDevice, c->stream));
     const int grid = fill_grid(c, kBlock, (int)opt(c, "batch_blocks_per_cu", 4));
+    // near-far rounds for SSSP (threshold step batch_sssp_delta, 0 = off)
+    const float delta = SSSP ? (float)opt(c, "batch_sssp_delta", 256) : 0.0f;
+    const bool near_far = SSSP && delta > 0;
+    uint32_t *pend =
+        near_far ? (uint32_t *)scratch(c, "batch_pend", sizeof(uint32_t) * words) : nullptr;
     // pulling reads a row as in-edges: only for a symmetric CSR
     const bool sym = !SSSP && graph_is_symmetric(c, const_cast<Graph *>(g));
     const int32_t pull_div = sym ? (int32_t)opt(c, "batch_pull_div", 4) : 0;
@@ -374,9 +547,11 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
     fill_i32(c, (int32_t *)fa, 0, words);
     fill_i32(c, (int32_t *)fb, 0, words);
     fill_i32(c, stamp, -1, m);
-    fill_i32(c, (int32_t *)fs.ctl, 0, 32);
-    int32_t *lvl = (int32_t *)(fs.ctl + 12);
+    if (pend) fill_i32(c, (int32_t *)pend, 0, words);
+    fill_i32(c, (int32_t *)fs.ctl, 0, 2 * kCtlWords);
+    int32_t *lvl = (int32_t *)(fs.ctl + 8);  // level / round counter
     fill_i32(c, lvl, 1, 1);
+    if (near_far) fill_i32(c, (int32_t *)(fs.ctl + 12), __builtin_bit_cast(int32_t, delta), 1);
     k_batch_seed<W><<<1, 256, 0, c->stream>>>(srcs, nsrc, vis, fa, depth, m, dist);
     k_build<kBlock, 1, kTile, true><<<1, kBlock, 0, c->stream>>>(
         SeedOp{}, g->rowptr, ulist, (int32_t)uniq.size(), nullptr, fs.f[0], fs.ctl + 0);
@@ -386,7 +561,12 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
     auto level = [&](int l) {
         const int cur = l & 1;
         uint32_t *front = cur ? fb : fa, *next = cur ? fa : fb;
-        if (SSSP)
+        if (near_far)
+            k_batch_sssp_nf<W><<<grid, kBlock, 0, c->stream>>>(
+                g->rowptr, g->col, (int32_t)g->nnz, m,
+                SsspBatchNearOp<W>{front, next, pend, stamp, dist, g->val, lvl, 0}, delta,
+                fs.f[cur], fs.f[cur ^ 1], fs.ctl, l % 6);
+        else if (SSSP)
             k_expand<kBlock, kTile, true, true, true><<<grid, kBlock, 0, c->stream>>>(
                 SsspBatchOp<W>{front, next, stamp, dist, g->val, lvl}, g->rowptr, g->col,
                 (int32_t)g->nnz, fs.f[cur], fs.ctl + (l % 3), fs.f[cur ^ 1],
@@ -397,7 +577,8 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
                 BfsBatchOp<W>{front, vis, next, stamp, depth, m, lvl, nsrc}, next, pull_div,
                 fs.f[cur], fs.ctl + (l % 3), fs.f[cur ^ 1], fs.ctl + ((l + 1) % 3),
                 fs.ctl + ((l + 2) % 3), fs.ctl + 4);
-        k_clear_rows<W><<<grid, kBlock, 0, c->stream>>>(fs.f[cur], fs.ctl + (l % 3), front, lvl);
+        k_clear_rows<W><<<grid, kBlock, 0, c->stream>>>(fs.f[cur], fs.ctl + (l % 3), front, lvl,
+                                                        near_far);
     };
 
     // chunks of K levels captured once and replayed: K is a multiple of 6 so
@@ -429,22 +610,25 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
         }
         kt.end();
         APSPIE_LAUNCH_CHECK();
-        APSPIE_HIP(hipMemcpyAsync(h, fs.ctl + (l % 3), sizeof(uint64_t), hipMemcpyDeviceToHost,
-                                  c->stream));
-        APSPIE_HIP(hipMemcpyAsync(h + 1, fs.ctl + 4, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+        APSPIE_HIP(hipMemcpyAsync(h, fs.ctl, kCtlWords * sizeof(uint64_t), hipMemcpyDeviceToHost,
                                   c->stream));
         sync(c);
         c->stats.host_syncs++;
-        if (cnt_entries(h[0]) == 0) break;
-        if (l > 4 * (int64_t)m + 64) {
+        if (!near_far && cnt_entries(h[l % 3]) == 0) break;
+        if (near_far) {  // the last round was a split that moved and left nothing
+            uint64_t left = 0;
+            for (int i = 0; i < kSpread; ++i) left += h[16 + i * 16 + (l + 2) % 3];
+            if (cnt_entries(h[l % 3]) == 0 && h[14 + (l & 1)] != 0 && left == 0) break;
+        }
+        if (l > 8 * (int64_t)m + 4096) {
             if (exec) (void)hipGraphExecDestroy(exec);
             throw Error{APSPIE_ERR_DEVICE, "batched traversal did not converge"};
         }
     }
     if (exec) APSPIE_HIP(hipGraphExecDestroy(exec));
-    c->stats.edges_traversed += (int64_t)h[1];
-    c->stats.vertices_reached += (int64_t)h[2];
-    c->stats.iterations = std::max(c->stats.iterations, (int32_t)h[3]);
+    c->stats.edges_traversed += (int64_t)h[4];
+    c->stats.vertices_reached += (int64_t)h[5];
+    c->stats.iterations = std::max(c->stats.iterations, (int32_t)h[6]);
     kt.collect();
     if (SSSP) {
         dim3 grd((unsigned)ceil_div(m, 32), (unsigned)ceil_div(nsrc, 32));

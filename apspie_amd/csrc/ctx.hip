@@ -22,6 +22,7 @@ static const char *kOptionNames[] = {
     "batch_blocks_per_cu",
     "batch_graph",       // 1 = replay batched level chunks from a captured hipGraph
     "batch_pull_div",    // batched BFS pulls a level when union-frontier edges * this > nnz (0 = never)
+    "batch_sssp_delta",  // batched SSSP near-far threshold step (0 = Bellman-Ford frontier)
     "profiling",         // 1 = time the dominant kernel with hipEvents
     nullptr};
 
@@ -40,6 +41,7 @@ static int64_t option_default(const std::string &n) {
     if (n == "batch_blocks_per_cu") return 4;
     if (n == "batch_graph") return 1;
     if (n == "batch_pull_div") return 4;
+    if (n == "batch_sssp_delta") return 256;
     if (n == "profiling") return 0;
     return -1;
 }
@@ -259,6 +261,8 @@ int apspie_ctx_set_option(apspie_ctx *c, const char *name, int64_t v) {
         return APSPIE_ERR_INVALID;
     if (n == "bfs_levels_per_sync" && (v < 1 || v > 4096)) return APSPIE_ERR_INVALID;
     if (n == "batch_pull_div" && (v < 0 || v > (1 << 20))) return APSPIE_ERR_INVALID;
+    if ((n == "batch_sssp_delta" || n == "sssp_delta") && (v < 0 || v > (1 << 30)))
+        return APSPIE_ERR_INVALID;
     if (n == "profiling") c->profiling = v != 0;
     c->options[n] = v;
     return APSPIE_OK;

@@ -90,6 +90,33 @@ def test_sssp_batch_vs_oracle(ctx, key, nsrc):
     G.close()
 
 
+@pytest.mark.parametrize("delta", [0, 1, 16, 256, 1 << 20])
+@pytest.mark.parametrize("nsrc", [7, 64, 150])
+@pytest.mark.parametrize("key", ["rmat11", "lattice32", "path"])
+def test_sssp_batch_near_far(ctx, key, nsrc, delta):
+    """Batched near-far rounds (batch_sssp_delta > 0, one threshold for all
+    sources) and the Bellman-Ford frontier (0) give the single-source
+    distances; the path graph needs many split rounds in a row."""
+    if key == "path":
+        g = T.path_graph(300)
+        w = np.full(len(g.col), 7, np.float32)
+    else:
+        g = T.rmat(11) if key == "rmat11" else T.lattice(32)
+        w = g.w
+    ctx.set_option("batch_sssp_delta", delta)
+    try:
+        G = ctx.upload(g.rowptr32, g.col, w)
+        src = T.sources(g, nsrc, seed=29)
+        got = ctx.sssp_batch(G, src)
+        for i, s in enumerate(src):
+            d = O.sssp(g.m, g.rowptr, g.col, w, s)
+            want = np.where(d == -1, A.SSSP_INF, d).astype(np.float32)
+            assert np.array_equal(got[i], want), (key, nsrc, delta, i, s)
+        G.close()
+    finally:
+        ctx.set_option("batch_sssp_delta", 256)
+
+
 @pytest.mark.parametrize("graph_on,lps", [(0, 6), (1, 6), (1, 48)])
 def test_batch_graph_replay_matches(ctx, graph_on, lps):
     g = T.lattice(64)  # ~100 levels: many chunk replays
