@@ -129,6 +129,15 @@ __device__ __forceinline__ uint32_t block_exclusive_add32(uint32_t x, uint32_t *
     return r;
 }
 
+// Slot of `key` in an open-addressing table of mask + 1 (a power of two)
+// slots: Fibonacci hashing, the HIGH bits of key * 2^32 / phi.  (The low bits
+// of the product depend only on the key's low bits, and R-MAT hub ids share
+// long runs of low zero bits: masking the low bits puts them in one probe
+// chain.)
+__device__ __forceinline__ uint32_t fib_slot(uint32_t key, uint32_t mask) {
+    return mask ? (key * 2654435769u) >> __clz(mask) : 0u;
+}
+
 // ---------------------------------------------------------------- atomics
 // Device-scope relaxed atomics: every CU on every XCD sees one order.
 __device__ __forceinline__ uint32_t atomic_or_dev(uint32_t *p, uint32_t v) {

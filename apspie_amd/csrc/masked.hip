@@ -27,11 +27,9 @@ constexpr int kLdsRow = 2048;           // mask rows up to this length: LDS tabl
 constexpr int kLdsTable = 2 * kLdsRow;  // open addressing, load <= 1/2
 constexpr int kChunk = 1024;            // entries of A_i per enumeration chunk
 
-__device__ __forceinline__ uint32_t mhash(uint32_t x) { return x * 2654435769u; }
-
 __device__ __forceinline__ void mtable_insert(int32_t *keys, int32_t *pos, uint32_t mask,
                                               int32_t key, int32_t p) {
-    uint32_t h = mhash((uint32_t)key) & mask;
+    uint32_t h = dev::fib_slot((uint32_t)key, mask);
     for (;;) {
         const int32_t prev = atomicCAS(keys + h, -1, key);
         if (prev == -1) {
@@ -46,7 +44,7 @@ __device__ __forceinline__ void mtable_insert(int32_t *keys, int32_t *pos, uint3
 // position of key in the table, -1 if absent (called after all inserts)
 __device__ __forceinline__ int32_t mtable_find(const int32_t *keys, const int32_t *pos,
                                                uint32_t mask, int32_t key) {
-    uint32_t h = mhash((uint32_t)key) & mask;
+    uint32_t h = dev::fib_slot((uint32_t)key, mask);
     for (;;) {
         const int32_t t = keys[h];
         if (t == key) return pos[h];

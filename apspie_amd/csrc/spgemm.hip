@@ -31,8 +31,6 @@ constexpr int kWaveUb = 256, kWaveTable = 512;
 constexpr int kBlockUb = 4096, kBlockTable = 8192;
 constexpr int32_t kEmpty = 0x7fffffff;  // sorts after every column id
 
-__device__ __forceinline__ uint32_t hash32(uint32_t x) { return x * 2654435769u; }
-
 __global__ void k_row_bound(const int32_t *arp, const int32_t *acol, const int32_t *brp,
                             int32_t m, int64_t *ub) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
@@ -71,7 +69,7 @@ __global__ void k_bin_rows(const int64_t *ub, int32_t m, int32_t *lists, uint32_
 // insert key; returns true if this call created the entry
 __device__ __forceinline__ bool tab_insert(int32_t *keys, uint32_t mask, int32_t key,
                                            uint32_t *slot_out) {
-    uint32_t h = hash32((uint32_t)key) & mask;
+    uint32_t h = dev::fib_slot((uint32_t)key, mask);
     for (;;) {
         const int32_t prev = atomicCAS(keys + h, kEmpty, key);
         if (prev == kEmpty || prev == key) {

@@ -30,10 +30,13 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kSmallRow = 128;     // wave bin: |L_r| <= 128, table 256
 constexpr int kSmallTable = 256;
-constexpr int kMediumRow = 2048;   // block bin: |L_r| <= 2048, table 4096
+constexpr int kMediumRow = 2048;   // block bin: |L_r| <= 2048, LDS table 4096
 constexpr int kMediumTable = 4096;
-
-__device__ __forceinline__ uint32_t hash32(uint32_t x) { return x * 2654435769u; }
+constexpr int kBigRow = 16384;     // 1024-thread block, LDS table 32768 (128 KB)
+constexpr int kBigTable = 32768;
+constexpr int kBigBlock = 1024;    // also for the longest rows (table in global memory)
+constexpr int kBigChunk = 1024;    // entries per enumeration chunk in the big tiers
+constexpr int kRun = 8;            // consecutive wedges per lane (one binary search)
 
 // lowcnt[r] = number of entries of row r below the diagonal (rows sorted)
 __global__ void k_lowcnt(const int32_t *rowptr, const int32_t *col, int32_t m, int32_t *lowcnt) {
@@ -49,29 +52,37 @@ __global__ void k_lowcnt(const int32_t *rowptr, const int32_t *col, int32_t m, i
     }
 }
 
-// rows with sorted columns? flag[0] stays 0 if all are
-__global__ void k_check_sorted(const int32_t *rowptr, const int32_t *col, int32_t m,
-                               int32_t *flag) {
+// rows sorted and duplicate-free?  One thread per entry: entry e must exceed
+// entry e - 1 unless e starts a row (row starts marked in a bitmap first, so
+// a hub row is checked by many threads, not walked by one).
+__global__ void k_row_starts(const int32_t *rowptr, int32_t m, int64_t nnz, uint32_t *bits) {
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < m;
          r += (int64_t)gridDim.x * blockDim.x) {
-        for (int32_t e = rowptr[r] + 1; e < rowptr[r + 1]; ++e)
-            if (col[e - 1] >= col[e]) {
-                flag[0] = 1;
-                break;
-            }
+        const int64_t e = rowptr[r];
+        if (e < nnz) atomicOr(bits + (e >> 5), 1u << (e & 31));
     }
 }
 
-// bins: [0] small list, [1] medium, [2] large; ctl[0..2] counts, ctl[3] max |L_r|
-__global__ void k_tc_bin(const int32_t *lowcnt, int32_t m, int32_t *small, int32_t *medium,
-                         int32_t *large, uint32_t *ctl, int32_t *per_row) {
+__global__ void k_check_sorted(const int32_t *col, int64_t nnz, const uint32_t *bits,
+                               int32_t *flag) {
+    for (int64_t e = 1 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz;
+         e += (int64_t)gridDim.x * blockDim.x)
+        if (!((bits[e >> 5] >> (e & 31)) & 1u) && col[e - 1] >= col[e]) flag[0] = 1;
+}
+
+// bins by |L_r|: [0] small, [1] medium, [2] big, [3] huge (lists[b] has m
+// slots each); ctl[0..3] counts, ctl[4] max |L_r|
+__global__ void k_tc_bin(const int32_t *lowcnt, int32_t m, int32_t *lists, uint32_t *ctl,
+                         int32_t *per_row) {
     for (int64_t r0 = blockIdx.x * (int64_t)blockDim.x; r0 < m; r0 += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = r0 + threadIdx.x;
         int32_t n = r < m ? lowcnt[r] : 0;
-        int bin = n < 2 ? -1 : (n <= kSmallRow ? 0 : (n <= kMediumRow ? 1 : 2));
+        const int bin = n < 2 ? -1
+                              : (n <= kSmallRow ? 0
+                                                : (n <= kMediumRow ? 1 : (n <= kBigRow ? 2 : 3)));
         if (r < m && per_row) per_row[r] = 0;  // rows add into each other's counts
 #pragma unroll
-        for (int b = 0; b < 3; ++b) {
+        for (int b = 0; b < 4; ++b) {
             const uint64_t mask = __ballot(bin == b);
             if (!mask) continue;
             const int leader = __ffsll((unsigned long long)mask) - 1;
@@ -80,16 +91,15 @@ __global__ void k_tc_bin(const int32_t *lowcnt, int32_t m, int32_t *small, int32
             base = __shfl(base, leader, 64);
             if (bin == b) {
                 const uint32_t slot = base + __popcll(mask & dev::lanemask_lt());
-                int32_t *list = b == 0 ? small : (b == 1 ? medium : large);
-                list[slot] = (int32_t)r;
+                lists[(int64_t)b * m + slot] = (int32_t)r;
             }
         }
-        if (bin == 2) atomicMax(ctl + 3, (uint32_t)n);
+        if (bin == 3) atomicMax(ctl + 4, (uint32_t)n);
     }
 }
 
 __device__ __forceinline__ void table_insert(int32_t *table, uint32_t mask, int32_t key) {
-    uint32_t h = hash32((uint32_t)key) & mask;
+    uint32_t h = dev::fib_slot((uint32_t)key, mask);
     for (;;) {
         const int32_t prev = atomicCAS(table + h, -1, key);
         if (prev == -1 || prev == key) return;
@@ -98,7 +108,7 @@ __device__ __forceinline__ void table_insert(int32_t *table, uint32_t mask, int3
 }
 
 __device__ __forceinline__ bool table_has(const int32_t *table, uint32_t mask, int32_t key) {
-    uint32_t h = hash32((uint32_t)key) & mask;
+    uint32_t h = dev::fib_slot((uint32_t)key, mask);
     for (;;) {
         const int32_t t = table[h];
         if (t == key) return true;
@@ -180,84 +190,104 @@ __global__ __launch_bounds__(kBlock) void k_tc_small(const int32_t *rowptr, cons
     }
 }
 
-// one block per medium / large row; TABLE > 0: LDS table of that size,
-// TABLE == 0: per-block region of global scratch (spill)
-template <int TABLE>
-__global__ __launch_bounds__(kBlock) void k_tc_block(const int32_t *rowptr, const int32_t *col,
-                                                     int32_t nnz, const int32_t *lowcnt,
-                                                     const int32_t *rows, const uint32_t *cnt,
-                                                     int32_t *per_row, uint64_t *total,
-                                                     uint64_t *wedges, int32_t *g_table,
-                                                     int32_t *g_pre, int32_t *g_rs,
-                                                     int32_t *g_cnt, int64_t g_cap) {
-    constexpr int LDS_T = TABLE > 0 ? TABLE : 1;
-    constexpr int LDS_R = TABLE > 0 ? TABLE / 2 : 1;
-    __shared__ int32_t s_table[LDS_T];
-    __shared__ int32_t s_pre[LDS_R + 1];
-    __shared__ int32_t s_rs[LDS_R];
-    __shared__ int32_t s_cnt[LDS_R];
-    __shared__ uint32_t s_scan[kBlock / 64 + 1];
-    __shared__ uint64_t s_red[kBlock / 64];
-    int32_t *table = TABLE > 0 ? s_table : g_table + (int64_t)blockIdx.x * g_cap;
-    int32_t *pre = TABLE > 0 ? s_pre : g_pre + (int64_t)blockIdx.x * (g_cap / 2 + 1);
-    int32_t *rsk = TABLE > 0 ? s_rs : g_rs + (int64_t)blockIdx.x * (g_cap / 2);
-    int32_t *cntu = TABLE > 0 ? s_cnt : g_cnt + (int64_t)blockIdx.x * (g_cap / 2);
-    const int32_t chunk = TABLE > 0 ? LDS_R : (int32_t)(g_cap / 2);
-    const int32_t nrows = (int32_t)cnt[0];
+// The wedges of one chunk of entries j in [0, cn): entry j owns the wedges
+// [pre[j], pre[j + 1]) starting at CSR position rsk[j]; pre[cn] = W.  A lane
+// takes kRun consecutive wedges, finds the first one's entry by binary search
+// and walks on from there; f(j, e) is called for every wedge.
+template <int BLOCK, class F>
+__device__ __forceinline__ void for_wedges(const int32_t *pre, const int32_t *rsk, int32_t cn,
+                                           int32_t W, int32_t nnz, F &&f) {
+    for (int64_t x0 = (int64_t)threadIdx.x * kRun; x0 < W; x0 += (int64_t)BLOCK * kRun) {
+        int j = dev::upper_bound_m1(pre, 0, cn, (int32_t)x0);
+        const int32_t xe = (int32_t)min<int64_t>(x0 + kRun, W);
+        for (int32_t x = (int32_t)x0; x < xe; ++x) {
+            while (pre[j + 1] <= x) ++j;
+            const int32_t e = rsk[j] + (x - pre[j]);
+            if ((uint32_t)e < (uint32_t)nnz) f(j, e);  // guard only
+        }
+    }
+}
+
+// one block per row of the block bins: the rows of list a (cnt_a[0] of them,
+// taken first) then of list b.  A row with |L_r| <= TABLE / 2 keeps its table
+// in LDS; a longer one in the block's region of global memory (g_cap slots).
+// The scanned lengths are staged in LDS, CHUNK entries at a time, for both
+// pair kinds.
+template <int TABLE, int BLOCK, int CHUNK>
+__global__ __launch_bounds__(BLOCK) void k_tc_block(const int32_t *rowptr, const int32_t *col,
+                                                    int32_t nnz, const int32_t *lowcnt,
+                                                    const int32_t *rows_a, const uint32_t *cnt_a,
+                                                    const int32_t *rows_b, const uint32_t *cnt_b,
+                                                    int32_t *per_row, uint64_t *total,
+                                                    uint64_t *wedges, int32_t *g_table,
+                                                    int64_t g_cap) {
+    __shared__ int32_t s_table[TABLE];
+    __shared__ int32_t pre[CHUNK + 1];
+    __shared__ int32_t rsk[CHUNK];
+    __shared__ int32_t cntu[CHUNK];
+    __shared__ uint32_t s_scan[BLOCK / 64 + 1];
+    __shared__ uint64_t s_red[BLOCK / 64];
+    const int32_t na = (int32_t)cnt_a[0], nrows = na + (cnt_b ? (int32_t)cnt_b[0] : 0);
     uint64_t my_tri = 0, my_wedges = 0;
     for (int32_t ri = blockIdx.x; ri < nrows; ri += gridDim.x) {
-        const int32_t r = rows[ri];
+        const int32_t r = ri < na ? rows_a[ri] : rows_b[ri - na];
         const int32_t n = lowcnt[r];
         const int32_t base = rowptr[r], rend = rowptr[r + 1];
-        uint32_t tsize = 1;
-        while (tsize < 2u * (uint32_t)n) tsize <<= 1;
-        if (TABLE > 0) tsize = TABLE;
-        const uint32_t mask = tsize - 1;
-        for (uint32_t s = threadIdx.x; s < tsize; s += kBlock) table[s] = -1;
-        __syncthreads();
-        // own pairs: k in L_r with |L_k| <= |L_r|
-        uint32_t carry = 0;
-        for (int c0 = 0; c0 < n; c0 += kBlock) {
-            const int idx = c0 + threadIdx.x;
-            uint32_t lc = 0;
-            if (idx < n) {
-                const int32_t k = col[base + idx];
-                table_insert(table, mask, k);
-                rsk[idx] = rowptr[k];
-                const int32_t l = lowcnt[k];
-                lc = l <= n ? (uint32_t)l : 0u;
-            }
-            uint32_t tot;
-            const uint32_t ex = dev::block_exclusive_add32<kBlock>(lc, s_scan, tot);
-            if (idx < n) pre[idx] = (int32_t)(carry + ex);
-            carry += tot;
+        const bool lds = 2 * n <= TABLE;
+        int32_t *table = lds ? s_table : g_table + (int64_t)blockIdx.x * g_cap;
+        uint32_t tsize = TABLE;
+        if (!lds) {
+            tsize = 1;
+            while (tsize < 2u * (uint32_t)n) tsize <<= 1;
         }
-        if (threadIdx.x == 0) pre[n] = (int32_t)carry;
+        const uint32_t mask = tsize - 1;
+        for (uint32_t s = threadIdx.x; s < tsize; s += BLOCK) table[s] = -1;
         __syncthreads();
-        int32_t W = (int32_t)carry;
+        for (int32_t idx = threadIdx.x; idx < n; idx += BLOCK)
+            table_insert(table, mask, col[base + idx]);
+        __syncthreads();
+        // own pairs: k in L_r with |L_k| <= |L_r|, CHUNK entries of L_r at a time
         uint32_t c = 0;
-        for (int32_t x = threadIdx.x; x < W; x += kBlock) {
-            const int j = dev::upper_bound_m1(pre, 0, n, x);
-            const int32_t e = rsk[j] + (x - pre[j]);
-            if ((uint32_t)e >= (uint32_t)nnz) continue;  // guard only
-            c += table_has(table, mask, col[e]) ? 1u : 0u;
+        for (int32_t k0 = 0; k0 < n; k0 += CHUNK) {
+            const int32_t cn = min(CHUNK, n - k0);
+            uint32_t carry = 0;
+            for (int c0 = 0; c0 < cn; c0 += BLOCK) {
+                const int idx = c0 + threadIdx.x;
+                uint32_t lc = 0;
+                if (idx < cn) {
+                    const int32_t k = col[base + k0 + idx];
+                    rsk[idx] = rowptr[k];
+                    const int32_t l = lowcnt[k];
+                    lc = l <= n ? (uint32_t)l : 0u;
+                }
+                uint32_t tot;
+                const uint32_t ex = dev::block_exclusive_add32<BLOCK>(lc, s_scan, tot);
+                if (idx < cn) pre[idx] = (int32_t)(carry + ex);
+                carry += tot;
+            }
+            if (threadIdx.x == 0) pre[cn] = (int32_t)carry;
+            __syncthreads();
+            for_wedges<BLOCK>(pre, rsk, cn, (int32_t)carry, nnz, [&](int, int32_t e) {
+                c += table_has(table, mask, col[e]) ? 1u : 0u;
+            });
+            if (threadIdx.x == 0) my_wedges += carry;
+            __syncthreads();  // pre / rsk are refilled by the next chunk
         }
         const uint64_t wc = dev::wave_sum_u64(c);
         if (dev::lane_id() == 0) s_red[threadIdx.x / 64] = wc;
         __syncthreads();
         if (threadIdx.x == 0) {
             uint64_t rc = 0;
-            for (int q = 0; q < kBlock / 64; ++q) rc += s_red[q];
+            for (int q = 0; q < BLOCK / 64; ++q) rc += s_red[q];
             if (rc && per_row) atomicAdd(per_row + r, (int32_t)rc);
             my_tri += rc;
-            my_wedges += (uint64_t)W;
         }
         __syncthreads();
-        // upper pairs: i in U_r with |L_i| < |L_r|, in chunks
-        for (int32_t u0 = base + n; u0 < rend; u0 += chunk) {
-            const int32_t cn = min(chunk, rend - u0);
-            carry = 0;
-            for (int c0 = 0; c0 < cn; c0 += kBlock) {
+        // upper pairs: i in U_r with |L_i| < |L_r|, CHUNK entries of U_r at a time
+        for (int32_t u0 = base + n; u0 < rend; u0 += CHUNK) {
+            const int32_t cn = min(CHUNK, rend - u0);
+            uint32_t carry = 0;
+            for (int c0 = 0; c0 < cn; c0 += BLOCK) {
                 const int idx = c0 + threadIdx.x;
                 uint32_t li = 0;
                 if (idx < cn) {
@@ -268,34 +298,33 @@ __global__ __launch_bounds__(kBlock) void k_tc_block(const int32_t *rowptr, cons
                     li = l < n ? (uint32_t)l : 0u;
                 }
                 uint32_t tot;
-                const uint32_t ex = dev::block_exclusive_add32<kBlock>(li, s_scan, tot);
+                const uint32_t ex = dev::block_exclusive_add32<BLOCK>(li, s_scan, tot);
                 if (idx < cn) pre[idx] = (int32_t)(carry + ex);
                 carry += tot;
             }
             if (threadIdx.x == 0) pre[cn] = (int32_t)carry;
             __syncthreads();
-            W = (int32_t)carry;
-            for (int32_t x = threadIdx.x; x < W; x += kBlock) {
-                const int j = dev::upper_bound_m1(pre, 0, cn, x);
-                const int32_t e = rsk[j] + (x - pre[j]);
-                if ((uint32_t)e >= (uint32_t)nnz) continue;  // guard only
-                if (table_has(table, mask, col[e])) atomicAdd(cntu + j, 1);
-            }
-            if (threadIdx.x == 0) my_wedges += (uint64_t)W;
-            __syncthreads();
-            uint64_t cu = 0;
-            for (int idx = threadIdx.x; idx < cn; idx += kBlock) {
-                const int32_t ci = cntu[idx];
-                if (ci) {
-                    if (per_row) atomicAdd(per_row + col[u0 + idx], ci);
-                    cu += (uint64_t)ci;
+            if (carry > 0) {  // uniform: most chunks of a hub's upper part scan nothing
+                for_wedges<BLOCK>(pre, rsk, cn, (int32_t)carry, nnz, [&](int j, int32_t e) {
+                    if (table_has(table, mask, col[e])) atomicAdd(cntu + j, 1);
+                });
+                __syncthreads();
+                uint64_t cu = 0;
+                for (int idx = threadIdx.x; idx < cn; idx += BLOCK) {
+                    const int32_t ci = cntu[idx];
+                    if (ci) {
+                        if (per_row) atomicAdd(per_row + col[u0 + idx], ci);
+                        cu += (uint64_t)ci;
+                    }
+                }
+                const uint64_t wcu = dev::wave_sum_u64(cu);
+                if (dev::lane_id() == 0) s_red[threadIdx.x / 64] = wcu;
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    for (int q = 0; q < BLOCK / 64; ++q) my_tri += s_red[q];
+                    my_wedges += carry;
                 }
             }
-            const uint64_t wcu = dev::wave_sum_u64(cu);
-            if (dev::lane_id() == 0) s_red[threadIdx.x / 64] = wcu;
-            __syncthreads();
-            if (threadIdx.x == 0)
-                for (int q = 0; q < kBlock / 64; ++q) my_tri += s_red[q];
             __syncthreads();  // rsk / pre / cntu are refilled by the next chunk
         }
     }
@@ -317,7 +346,10 @@ void graph_prepare_tc(Ctx *c, Graph *g) {
     uint32_t *ctl = (uint32_t *)scratch(c, "tc_prep", 64);
     fill_i32(c, (int32_t *)ctl, 0, 16);
     {
-        k_check_sorted<<<grid, kBlock, 0, c->stream>>>(g->rowptr, g->col, m, (int32_t *)ctl + 4);
+        uint32_t *bits = (uint32_t *)scratch(c, "tc_rowstarts", sizeof(uint32_t) * (g->nnz / 32 + 1));
+        fill_i32(c, (int32_t *)bits, 0, g->nnz / 32 + 1);
+        k_row_starts<<<grid, kBlock, 0, c->stream>>>(g->rowptr, m, g->nnz, bits);
+        k_check_sorted<<<grid, kBlock, 0, c->stream>>>(g->col, g->nnz, bits, (int32_t *)ctl + 4);
         uint32_t flag = 0;
         APSPIE_HIP(hipMemcpyAsync(&flag, ctl + 4, 4, hipMemcpyDeviceToHost, c->stream));
         sync(c);
@@ -343,50 +375,47 @@ void tc_run(Ctx *c, const Graph *g0, uint64_t *total_out, int32_t *per_row) {
     uint64_t *acc = (uint64_t *)(ctl + 8);  // [0] triangles, [1] wedges
     fill_i32(c, (int32_t *)ctl, 0, 16);
     int32_t *per = per_row;  // may be null: total only, no per-row atomics
-    int32_t *lists = (int32_t *)scratch(c, "tc_lists", sizeof(int32_t) * 3 * (size_t)m);
-    k_tc_bin<<<grid, kBlock, 0, c->stream>>>(g->lowcnt, m, lists, lists + m, lists + 2 * (size_t)m,
-                                            ctl, per);
+    int32_t *lists = (int32_t *)scratch(c, "tc_lists", sizeof(int32_t) * 4 * (size_t)m);
+    k_tc_bin<<<grid, kBlock, 0, c->stream>>>(g->lowcnt, m, lists, ctl, per);
     APSPIE_LAUNCH_CHECK();
-    uint32_t h[4];
+    uint32_t h[5];
     APSPIE_HIP(hipMemcpyAsync(h, ctl, sizeof h, hipMemcpyDeviceToHost, c->stream));
     sync(c);
     c->stats.host_syncs++;
     KernelTimer kt(c);
-    if (h[0]) {
-        const int blocks = (int)std::min<int64_t>(ceil_div(h[0], kBlock / 64), 8 * c->cus);
-        kt.begin();
-        k_tc_small<<<blocks, kBlock, 0, c->stream>>>(g->rowptr, g->col, (int32_t)g->nnz, g->lowcnt, lists, ctl + 0,
-                                                     per, acc, acc + 1);
-        kt.end();
+    kt.begin();
+    if (h[2] + h[3]) {  // longest rows first: they set the tail
+        int64_t cap = 1;
+        while (cap < 2 * (int64_t)h[4]) cap <<= 1;
+        const int blocks = (int)std::min<int64_t>(h[2] + h[3], c->cus);
+        int32_t *gt = h[3] ? (int32_t *)scratch(c, "tc_gtable", sizeof(int32_t) * cap * blocks)
+                           : nullptr;
+        k_tc_block<kBigTable, kBigBlock, kBigChunk><<<blocks, kBigBlock, 0, c->stream>>>(
+            g->rowptr, g->col, (int32_t)g->nnz, g->lowcnt, lists + 3 * (size_t)m, ctl + 3,
+            lists + 2 * (size_t)m, ctl + 2, per, acc, acc + 1, gt, cap);
         APSPIE_LAUNCH_CHECK();
     }
     if (h[1]) {
         const int blocks = (int)std::min<int64_t>(h[1], 4 * c->cus);
-        k_tc_block<kMediumTable><<<blocks, kBlock, 0, c->stream>>>(
-            g->rowptr, g->col, (int32_t)g->nnz, g->lowcnt, lists + m, ctl + 1, per, acc, acc + 1,
-            nullptr, nullptr, nullptr, nullptr, 0);
+        k_tc_block<kMediumTable, kBlock, kMediumRow><<<blocks, kBlock, 0, c->stream>>>(
+            g->rowptr, g->col, (int32_t)g->nnz, g->lowcnt, lists + m, ctl + 1, nullptr, nullptr,
+            per, acc, acc + 1, nullptr, 0);
         APSPIE_LAUNCH_CHECK();
     }
-    if (h[2]) {
-        int64_t cap = 1;
-        while (cap < 2 * (int64_t)h[3]) cap <<= 1;
-        const int blocks = (int)std::min<int64_t>(h[2], 2 * c->cus);
-        int32_t *gt = (int32_t *)scratch(c, "tc_gtable", sizeof(int32_t) * cap * blocks);
-        int32_t *gp = (int32_t *)scratch(c, "tc_gpre", sizeof(int32_t) * (cap / 2 + 1) * blocks);
-        int32_t *gr = (int32_t *)scratch(c, "tc_grs", sizeof(int32_t) * (cap / 2) * blocks);
-        int32_t *gc = (int32_t *)scratch(c, "tc_gcnt", sizeof(int32_t) * (cap / 2) * blocks);
-        k_tc_block<0><<<blocks, kBlock, 0, c->stream>>>(g->rowptr, g->col, (int32_t)g->nnz,
-                                                        g->lowcnt, lists + 2 * (size_t)m, ctl + 2,
-                                                        per, acc, acc + 1, gt, gp, gr, gc, cap);
+    if (h[0]) {
+        const int blocks = (int)std::min<int64_t>(ceil_div(h[0], kBlock / 64), 8 * c->cus);
+        k_tc_small<<<blocks, kBlock, 0, c->stream>>>(g->rowptr, g->col, (int32_t)g->nnz,
+                                                     g->lowcnt, lists, ctl + 0, per, acc, acc + 1);
         APSPIE_LAUNCH_CHECK();
     }
+    kt.end(1);
     uint64_t res[2];
     APSPIE_HIP(hipMemcpyAsync(res, acc, sizeof res, hipMemcpyDeviceToHost, c->stream));
     sync(c);
     c->stats.host_syncs++;
     *total_out = res[0];
     c->stats.edges_traversed = (int64_t)res[1];  // wedges probed
-    c->stats.vertices_reached = (int64_t)h[0] + h[1] + h[2];
+    c->stats.vertices_reached = (int64_t)h[0] + h[1] + h[2] + h[3];
     c->stats.iterations = 1;
     kt.collect();
 }

@@ -199,6 +199,24 @@ def test_tc_all_bins():
             assert total == wt and np.array_equal(rows, wr), n
 
 
+def test_tc_longest_rows_in_global_tables(ctx):
+    """Rows with more than 16384 entries below the diagonal keep their table in
+    global memory: a wheel (hub = the highest id, rim path) has one triangle
+    per rim edge, all counted at the hub; a second hub in the middle adds
+    upper pairs."""
+    n = 20000
+    rim = np.arange(n - 2)
+    u = np.concatenate([rim[:-1], rim, np.full(9000, 9000)])
+    v = np.concatenate([rim[1:], np.full(n - 2, n - 1), np.arange(9001, 18001)])
+    rp, c = O.edges_to_csr(n, u.astype(np.int32), v.astype(np.int32))
+    G = ctx.upload(rp.astype(np.int32), c)
+    total, rows = ctx.tc(G)
+    wt, wr = O.tc(n, rp, c)
+    assert total == wt and np.array_equal(rows, wr)
+    assert rows[n - 1] >= n - 3  # the rim triangles at the hub
+    G.close()
+
+
 def test_tc_rejects_unsorted_rows(ctx):
     rp = np.array([0, 2, 3, 4], np.int32)
     col = np.array([2, 1, 0, 0], np.int32)  # row 0 unsorted
