@@ -12,11 +12,16 @@
 // fit the worker's share of the budget is split in halves; a single query
 // that does not fit fails with APSPIE_ERR_BUDGET.  Results land in host
 // memory; latency is measured from submit to completion on the host clock.
+// Result buffers are pinned host memory from a pool owned by the server: one
+// direct D2H copy per query (a batch writes its rows to device memory and each
+// row goes straight to its query's buffer), and the buffer returns to the
+// pool when the caller collects the result.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -29,13 +34,51 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// Pinned result buffers, reused by size (a buffer serves any request up to
+// its capacity and at most twice smaller, so big buffers are not tied up by
+// small results).
+class PinnedPool {
+  public:
+    ~PinnedPool() {
+        for (auto &kv : cap_) (void)hipHostFree(kv.first);
+    }
+    void *get(size_t bytes) {
+        bytes = std::max<size_t>(bytes, 256);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            auto it = free_.lower_bound(bytes);
+            if (it != free_.end() && it->first <= 2 * bytes) {
+                void *p = it->second;
+                free_.erase(it);
+                return p;
+            }
+        }
+        void *p = nullptr;
+        if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+        std::lock_guard<std::mutex> lk(mu_);
+        cap_[p] = bytes;
+        return p;
+    }
+    void put(void *p) {
+        if (!p) return;
+        std::lock_guard<std::mutex> lk(mu_);
+        free_.emplace(cap_[p], p);
+    }
+
+  private:
+    std::mutex mu_;
+    std::multimap<size_t, void *> free_;
+    std::unordered_map<void *, size_t> cap_;
+};
+
 struct Request {
     int64_t ticket = 0;
     int32_t graph = 0;
     int kind = 0;
     int32_t source = 0;
     Clock::time_point t_submit, t_done;
-    std::vector<uint8_t> result;
+    void *buf = nullptr;    // pinned result (BFS depths / SSSP distances)
+    uint64_t triangles = 0;  // TC result
     int status = APSPIE_OK;
     bool done = false;
 };
@@ -60,6 +103,7 @@ struct apspie_server {
     std::unordered_map<int64_t, std::shared_ptr<Request>> live;
     std::vector<std::thread> workers;
     std::vector<apspie_ctx *> ctxs;
+    PinnedPool pinned;
     int64_t next_ticket = 1;
     int busy = 0;
     bool stop = false;
@@ -71,46 +115,62 @@ int64_t result_bytes(const apspie_server *s, int kind, int32_t graph) {
     return kind == APSPIE_Q_TC ? (int64_t)sizeof(uint64_t) : 4 * (int64_t)s->graphs[graph].m;
 }
 
-// run one coalesced batch; on a budget failure split it
-void execute(apspie_server *s, apspie_ctx *ctx, std::vector<std::shared_ptr<Request>> &batch) {
+int run_batch(apspie_server *s, apspie_ctx *ctx, std::vector<std::shared_ptr<Request>> &batch) {
     const Resident &R = s->graphs[batch[0]->graph];
     const int kind = batch[0]->kind;
     const size_t n = batch.size();
-    int st = APSPIE_OK;
+    const size_t row = 4 * (size_t)R.m;
     if (kind == APSPIE_Q_TC) {
         uint64_t total = 0;
-        st = apspie_tc(ctx, R.g, &total, nullptr, APSPIE_MEM_HOST);
-        for (auto &r : batch) {
-            r->result.resize(sizeof total);
-            std::memcpy(r->result.data(), &total, sizeof total);
-        }
-    } else if (n == 1) {
+        const int st = apspie_tc(ctx, R.g, &total, nullptr, APSPIE_MEM_HOST);
+        for (auto &r : batch) r->triangles = total;
+        return st;
+    }
+    if (n == 1) {
         auto &r = batch[0];
-        r->result.resize(4 * (size_t)R.m);
+        if (!r->buf) r->buf = s->pinned.get(row);
+        if (!r->buf) return APSPIE_ERR_OOM;
+        return kind == APSPIE_Q_BFS
+                   ? apspie_bfs(ctx, R.g, r->source, (int32_t *)r->buf, APSPIE_MEM_HOST)
+                   : apspie_sssp(ctx, R.g, r->source, (float *)r->buf, APSPIE_MEM_HOST);
+    }
+    std::vector<int32_t> src(n);
+    for (size_t i = 0; i < n; ++i) src[i] = batch[i]->source;
+    void *dout = nullptr;
+    int st = apspie_dev_alloc(ctx, (int64_t)(row * n), &dout);
+    if (st == APSPIE_OK)
         st = kind == APSPIE_Q_BFS
-                 ? apspie_bfs(ctx, R.g, r->source, (int32_t *)r->result.data(), APSPIE_MEM_HOST)
-                 : apspie_sssp(ctx, R.g, r->source, (float *)r->result.data(), APSPIE_MEM_HOST);
-    } else {
-        std::vector<int32_t> src(n);
-        for (size_t i = 0; i < n; ++i) src[i] = batch[i]->source;
-        std::vector<uint8_t> out(4 * (size_t)R.m * n);
-        st = kind == APSPIE_Q_BFS
-                 ? apspie_bfs_batch(ctx, R.g, src.data(), (int32_t)n, (int32_t *)out.data(),
-                                    APSPIE_MEM_HOST)
-                 : apspie_sssp_batch(ctx, R.g, src.data(), (int32_t)n, (float *)out.data(),
-                                     APSPIE_MEM_HOST);
-        if (st == APSPIE_ERR_BUDGET || st == APSPIE_ERR_OOM) {
-            apspie_ctx_trim(ctx);
+                 ? apspie_bfs_batch(ctx, R.g, src.data(), (int32_t)n, (int32_t *)dout,
+                                    APSPIE_MEM_DEVICE)
+                 : apspie_sssp_batch(ctx, R.g, src.data(), (int32_t)n, (float *)dout,
+                                     APSPIE_MEM_DEVICE);
+    // each row straight to its query's pinned buffer
+    for (size_t i = 0; i < n && st == APSPIE_OK; ++i) {
+        if (!batch[i]->buf) batch[i]->buf = s->pinned.get(row);
+        st = !batch[i]->buf ? APSPIE_ERR_OOM
+                            : apspie_memcpy(ctx, batch[i]->buf, (uint8_t *)dout + row * i,
+                                            (int64_t)row, 1);
+    }
+    if (dout) apspie_dev_free(ctx, dout, (int64_t)(row * n));
+    return st;
+}
+
+// run one coalesced batch.  Out of budget or memory: the worker's context
+// drops its cached scratch (buffers sized for earlier, larger batches) and
+// tries again; a batch that still does not fit is split in halves.
+void execute(apspie_server *s, apspie_ctx *ctx, std::vector<std::shared_ptr<Request>> &batch) {
+    int st = run_batch(s, ctx, batch);
+    if (st == APSPIE_ERR_BUDGET || st == APSPIE_ERR_OOM) {
+        apspie_ctx_trim(ctx);
+        st = run_batch(s, ctx, batch);
+        if ((st == APSPIE_ERR_BUDGET || st == APSPIE_ERR_OOM) && batch.size() > 1) {
+            const size_t n = batch.size();
             std::vector<std::shared_ptr<Request>> lo(batch.begin(), batch.begin() + n / 2),
                 hi(batch.begin() + n / 2, batch.end());
             execute(s, ctx, lo);
             execute(s, ctx, hi);
             return;
         }
-        if (st == APSPIE_OK)
-            for (size_t i = 0; i < n; ++i)
-                batch[i]->result.assign(out.begin() + 4 * (size_t)R.m * i,
-                                        out.begin() + 4 * (size_t)R.m * (i + 1));
     }
     const auto now = Clock::now();
     std::lock_guard<std::mutex> lk(s->mu);
@@ -266,13 +326,17 @@ int apspie_server_wait(apspie_server *s, int64_t ticket, void *out, int64_t out_
     s->live.erase(ticket);
     if (latency_ms)
         *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
-    if (r->status != APSPIE_OK) return r->status;
+    lk.unlock();
+    int st = r->status;
     const int64_t need = result_bytes(s, r->kind, r->graph);
-    if (out) {
-        if (out_bytes < need) return APSPIE_ERR_INVALID;
-        std::memcpy(out, r->result.data(), (size_t)need);
+    if (st == APSPIE_OK && out) {
+        if (out_bytes < need) st = APSPIE_ERR_INVALID;
+        else if (r->kind == APSPIE_Q_TC) std::memcpy(out, &r->triangles, sizeof r->triangles);
+        else std::memcpy(out, r->buf, (size_t)need);
     }
-    return APSPIE_OK;
+    s->pinned.put(r->buf);
+    r->buf = nullptr;
+    return st;
 }
 
 int apspie_server_drain(apspie_server *s) {

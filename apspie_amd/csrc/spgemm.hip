@@ -1,21 +1,26 @@
-// General sparse product C = A * B over (+, x) by row-wise hash accumulation.
+// General sparse product C = A * B over (+, x) by row-wise accumulation.
 //
 // Reference: the hash SpGEMM spgemm() (spgemm.cuh:131-342) with its insert
 // kernels (spgemmKernel.cuh:179-343): open-addressing tables keyed by output
 // coordinates, atomic value accumulation, a sort to order the output.  The
 // reference tiles C in 2D over global-memory tables and drops products after
-// 25 probes; here each output row gets its own table sized from the row's
-// symbolic bound, so nothing is ever dropped:
-//   0. symbolic bound ub[i] = sum_{k in A_i} |B_k| (products of row i)
-//   1. rows binned by ub: wave + 512-slot LDS table (ub <= 256), block +
-//      8192-slot LDS table (ub <= 4096), block + global-memory table (spill)
-//   2. symbolic pass: distinct columns per row (key-only inserts)
+// 25 probes; here each output row gets its own accumulator sized from the
+// row, so nothing is ever dropped:
+//   0. u[i] = min(sum_{k in A_i} |B_k|, n): the products of row i, capped by
+//      the number of columns (no row has more distinct outputs)
+//   1. rows binned by u: wave + 512-slot LDS table (u <= 256); block +
+//      8192-slot LDS table (u <= 4096); longer rows with n <= 32768 a dense
+//      LDS accumulator (value per column + a column bitmap, no hashing and no
+//      sort); longer rows of wider matrices a table in global memory
+//   2. symbolic pass: distinct columns per row
 //   3. exclusive scan -> C row offsets
-//   4. numeric pass: key insert + atomic value add, bitonic sort of the
-//      table in place (empty slots sort last), the first nnz(row) entries
-//      are the sorted row.
-// Values are exact for integer-valued inputs; float (+) order otherwise
-// follows the atomic order.
+//   4. numeric pass: accumulate, then write the row in column order (the
+//      dense accumulator is in order already; a table's used slots are
+//      compacted and only those are sorted)
+// The block tiers walk a row's products load-balanced: an exclusive scan of
+// |B_k| over a chunk of A_i's entries, then each lane takes 8 consecutive
+// products (one binary search).  Values are exact for integer-valued inputs;
+// float (+) order otherwise follows the atomic order.
 #include <algorithm>
 
 #include "common.hpp"
@@ -27,12 +32,17 @@ namespace apspie {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kBig = 1024;          // threads of the dense and global tiers
 constexpr int kWaveUb = 256, kWaveTable = 512;
 constexpr int kBlockUb = 4096, kBlockTable = 8192;
+constexpr int kDenseN = 32768;      // dense accumulator: value + bit per column
+constexpr int kChunk = 1024;        // entries of A_i staged per enumeration chunk
+constexpr int kRun = 8;             // consecutive products per lane
 constexpr int32_t kEmpty = 0x7fffffff;  // sorts after every column id
 
+// u[i] = min(products of row i, n)
 __global__ void k_row_bound(const int32_t *arp, const int32_t *acol, const int32_t *brp,
-                            int32_t m, int64_t *ub) {
+                            int32_t m, int32_t n, int64_t *ub) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
          i += (int64_t)gridDim.x * blockDim.x) {
         int64_t s = 0;
@@ -40,20 +50,24 @@ __global__ void k_row_bound(const int32_t *arp, const int32_t *acol, const int32
             const int32_t k = acol[e];
             s += brp[k + 1] - brp[k];
         }
-        ub[i] = s;
+        ub[i] = s == 0 ? 0 : std::min<int64_t>(s, n);
     }
 }
 
-// bins: 0 wave, 1 block-LDS, 2 block-global; ctl[0..2] counts; ctl[3] max ub (global bin)
-__global__ void k_bin_rows(const int64_t *ub, int32_t m, int32_t *lists, uint32_t *ctl,
-                           int32_t *row_nnz) {
+// bins: 0 wave, 1 block-LDS, 2 dense, 3 block-global; ctl[0..3] counts;
+// ctl[4] max u of the global bin
+__global__ void k_bin_rows(const int64_t *ub, int32_t m, bool dense_ok, int32_t *lists,
+                           uint32_t *ctl, int32_t *row_nnz) {
     for (int64_t r0 = blockIdx.x * (int64_t)blockDim.x; r0 < m; r0 += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = r0 + threadIdx.x;
         const int64_t u = r < m ? ub[r] : 0;
-        const int bin = u == 0 ? -1 : (u <= kWaveUb ? 0 : (u <= kBlockUb ? 1 : 2));
+        const int bin = u == 0 ? -1
+                        : u <= kWaveUb ? 0
+                        : u <= kBlockUb ? 1
+                        : dense_ok ? 2 : 3;
         if (r < m && bin < 0) row_nnz[r] = 0;
 #pragma unroll
-        for (int b = 0; b < 3; ++b) {
+        for (int b = 0; b < 4; ++b) {
             const uint64_t mask = __ballot(bin == b);
             if (!mask) continue;
             const int leader = __ffsll((unsigned long long)mask) - 1;
@@ -62,7 +76,7 @@ __global__ void k_bin_rows(const int64_t *ub, int32_t m, int32_t *lists, uint32_
             base = __shfl(base, leader, 64);
             if (bin == b) lists[(int64_t)b * m + base + __popcll(mask & dev::lanemask_lt())] = (int32_t)r;
         }
-        if (bin == 2) atomicMax(ctl + 3, (uint32_t)min<int64_t>(u, 0x7fffffff));
+        if (bin == 3) atomicMax(ctl + 4, (uint32_t)min<int64_t>(u, 0x7fffffff));
     }
 }
 
@@ -108,31 +122,27 @@ __device__ __forceinline__ void bitonic(int32_t *keys, float *vals, uint32_t siz
     }
 }
 
-// The row worker, shared by the three tiers.  NUMERIC=false counts distinct
-// columns into row_nnz; NUMERIC=true writes the sorted row into C.
-template <bool NUMERIC, bool WAVE>
-__device__ __forceinline__ void row_product(int32_t r, const int32_t *arp, const int32_t *acol,
-                                            const float *aval, const int32_t *brp,
-                                            const int32_t *bcol, const float *bval,
-                                            int32_t *keys, float *vals, uint32_t tsize,
-                                            int tid, int nthreads, int32_t *row_nnz,
-                                            const int32_t *crp, int32_t *ccol, float *cval,
-                                            uint32_t *s_count) {
-    auto barrier = [] {
-        if (WAVE) dev::wave_sync(); else __syncthreads();
-    };
-    const uint32_t mask = tsize - 1;
-    for (uint32_t s = tid; s < tsize; s += nthreads) {
+// Small rows: one wave per row, products walked entry by entry of A_i (the
+// lanes stride over B_k), 512-slot LDS table sorted whole.
+template <bool NUMERIC>
+__device__ __forceinline__ void wave_row(int32_t r, const int32_t *arp, const int32_t *acol,
+                                         const float *aval, const int32_t *brp,
+                                         const int32_t *bcol, const float *bval, int32_t *keys,
+                                         float *vals, int32_t *row_nnz, const int32_t *crp,
+                                         int32_t *ccol, float *cval, uint32_t *s_count) {
+    const int tid = dev::lane_id();
+    constexpr uint32_t tsize = kWaveTable, mask = kWaveTable - 1;
+    for (uint32_t s = tid; s < tsize; s += 64) {
         keys[s] = kEmpty;
         if (NUMERIC) vals[s] = 0.0f;
     }
     if (tid == 0) *s_count = 0;
-    barrier();
+    dev::wave_sync();
     uint32_t created = 0;
     for (int32_t e = arp[r]; e < arp[r + 1]; ++e) {
         const int32_t k = acol[e];
         const float av = aval ? aval[e] : 1.0f;
-        for (int32_t f = brp[k] + tid; f < brp[k + 1]; f += nthreads) {
+        for (int32_t f = brp[k] + tid; f < brp[k + 1]; f += 64) {
             uint32_t slot;
             created += tab_insert(keys, mask, bcol[f], &slot) ? 1u : 0u;
             if (NUMERIC) atomicAdd(vals + slot, av * (bval ? bval[f] : 1.0f));
@@ -140,19 +150,19 @@ __device__ __forceinline__ void row_product(int32_t r, const int32_t *arp, const
     }
     if (!NUMERIC) {
         if (created) atomicAdd(s_count, created);
-        barrier();
+        dev::wave_sync();
         if (tid == 0) row_nnz[r] = (int32_t)*s_count;
-        barrier();
+        dev::wave_sync();
         return;
     }
-    barrier();
-    bitonic(keys, vals, tsize, tid, nthreads, barrier);
+    dev::wave_sync();
+    bitonic(keys, vals, tsize, tid, 64, [] { dev::wave_sync(); });
     const int32_t c0 = crp[r], nz = crp[r + 1] - c0;
-    for (int32_t s = tid; s < nz; s += nthreads) {
+    for (int32_t s = tid; s < nz; s += 64) {
         ccol[c0 + s] = keys[s];
         cval[c0 + s] = vals[s];
     }
-    barrier();
+    dev::wave_sync();
 }
 
 template <bool NUMERIC>
@@ -169,36 +179,203 @@ __global__ __launch_bounds__(kBlock) void k_rows_wave(const int32_t *rows, const
     const int w = threadIdx.x / 64;
     const int32_t n = (int32_t)cnt[0];
     for (int32_t i = blockIdx.x * NW + w; i < n; i += gridDim.x * NW)
-        row_product<NUMERIC, true>(rows[i], arp, acol, aval, brp, bcol, bval, s_keys[w],
-                                   NUMERIC ? s_vals[w] : nullptr, kWaveTable, dev::lane_id(), 64,
-                                   row_nnz, crp, ccol, cval, &s_count[w]);
+        wave_row<NUMERIC>(rows[i], arp, acol, aval, brp, bcol, bval, s_keys[w],
+                          NUMERIC ? s_vals[w] : nullptr, row_nnz, crp, ccol, cval, &s_count[w]);
 }
 
-template <bool NUMERIC, bool GLOBAL>
-__global__ __launch_bounds__(kBlock) void k_rows_block(const int32_t *rows, const uint32_t *cnt,
-                                                       const int32_t *arp, const int32_t *acol,
-                                                       const float *aval, const int32_t *brp,
-                                                       const int32_t *bcol, const float *bval,
-                                                       int32_t *row_nnz, const int32_t *crp,
-                                                       int32_t *ccol, float *cval,
-                                                       const int64_t *ub, int32_t *g_keys,
-                                                       float *g_vals, int64_t g_cap) {
-    __shared__ int32_t s_keys[GLOBAL ? 1 : kBlockTable];
-    __shared__ float s_vals[(GLOBAL || !NUMERIC) ? 1 : kBlockTable];
+// The products of row r, load-balanced over the block: f(col, value) per
+// product.  pre / rs / av are LDS staging of kChunk entries of A_r.
+template <int BLOCK, class F>
+__device__ __forceinline__ void for_products(int32_t r, const int32_t *arp, const int32_t *acol,
+                                             const float *aval, const int32_t *brp,
+                                             const int32_t *bcol, const float *bval,
+                                             int32_t *pre, int32_t *rs, float *av,
+                                             uint32_t *s_scan, F &&f) {
+    const int32_t ab = arp[r], an = arp[r + 1] - ab;
+    for (int32_t c0 = 0; c0 < an; c0 += kChunk) {
+        const int32_t cn = min(kChunk, an - c0);
+        uint32_t carry = 0;
+        for (int32_t x0 = 0; x0 < cn; x0 += BLOCK) {
+            const int32_t x = x0 + threadIdx.x;
+            uint32_t len = 0;
+            if (x < cn) {
+                const int32_t k = acol[ab + c0 + x];
+                const int32_t b0 = brp[k];
+                len = (uint32_t)(brp[k + 1] - b0);
+                rs[x] = b0;
+                av[x] = aval ? aval[ab + c0 + x] : 1.0f;
+            }
+            uint32_t tot;
+            const uint32_t ex = dev::block_exclusive_add32<BLOCK>(len, s_scan, tot);
+            if (x < cn) pre[x] = (int32_t)(carry + ex);
+            carry += tot;
+        }
+        if (threadIdx.x == 0) pre[cn] = (int32_t)carry;
+        __syncthreads();
+        const int64_t W = carry;
+        for (int64_t x0 = (int64_t)threadIdx.x * kRun; x0 < W; x0 += (int64_t)BLOCK * kRun) {
+            int j = dev::upper_bound_m1(pre, 0, cn, (int32_t)x0);
+            const int64_t xe = min<int64_t>(x0 + kRun, W);
+            for (int64_t x = x0; x < xe; ++x) {
+                while (pre[j + 1] <= x) ++j;
+                const int64_t e = (int64_t)rs[j] + (x - pre[j]);
+                f(bcol[e], av[j] * (bval ? bval[e] : 1.0f));
+            }
+        }
+        __syncthreads();  // pre / rs / av are refilled by the next chunk
+    }
+}
+
+// Moves the used slots of a table to its front, in slot order, 256 slots at
+// a time (a slot's destination never lies beyond the slots already read), and
+// returns their number; slots [count, next power of two) are set empty.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t compact_table(int32_t *keys, float *vals, uint32_t tsize,
+                                                  uint32_t *s_scan, uint32_t *s_n) {
+    if (threadIdx.x == 0) *s_n = 0;
+    __syncthreads();
+    for (uint32_t s0 = 0; s0 < tsize; s0 += BLOCK) {
+        const uint32_t s = s0 + threadIdx.x;
+        int32_t k = kEmpty;
+        float v = 0.0f;
+        if (s < tsize) {
+            k = keys[s];
+            if (vals) v = vals[s];
+        }
+        const uint32_t used = k != kEmpty ? 1u : 0u;
+        uint32_t tot;
+        const uint32_t ex = dev::block_exclusive_add32<BLOCK>(used, s_scan, tot);  // syncs
+        const uint32_t base = *s_n;
+        if (used) {
+            keys[base + ex] = k;
+            if (vals) vals[base + ex] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) *s_n = base + tot;
+        __syncthreads();
+    }
+    const uint32_t n = *s_n;
+    uint32_t p = 1;
+    while (p < n) p <<= 1;
+    for (uint32_t s = n + threadIdx.x; s < p; s += BLOCK) keys[s] = kEmpty;
+    __syncthreads();
+    return n;
+}
+
+// Longer rows: one block per row with an open-addressing table in LDS
+// (TABLE > 0, 2u <= TABLE) or in the block's region of global memory.
+template <bool NUMERIC, int BLOCK, int TABLE>
+__global__ __launch_bounds__(BLOCK) void k_rows_table(const int32_t *rows, const uint32_t *cnt,
+                                                      const int32_t *arp, const int32_t *acol,
+                                                      const float *aval, const int32_t *brp,
+                                                      const int32_t *bcol, const float *bval,
+                                                      int32_t *row_nnz, const int32_t *crp,
+                                                      int32_t *ccol, float *cval,
+                                                      const int64_t *ub, int32_t *g_keys,
+                                                      float *g_vals, int64_t g_cap) {
+    constexpr bool LDS = TABLE > 0;
+    __shared__ int32_t s_keys[LDS ? TABLE : 1];
+    __shared__ float s_vals[(LDS && NUMERIC) ? TABLE : 1];
+    __shared__ int32_t pre[kChunk + 1], rs[kChunk];
+    __shared__ float av[kChunk];
+    __shared__ uint32_t s_scan[BLOCK / 64 + 1];
     __shared__ uint32_t s_count;
-    const int32_t n = (int32_t)cnt[0];
-    int32_t *keys = GLOBAL ? g_keys + (int64_t)blockIdx.x * g_cap : s_keys;
-    float *vals = GLOBAL ? (NUMERIC ? g_vals + (int64_t)blockIdx.x * g_cap : nullptr)
-                         : (NUMERIC ? s_vals : nullptr);
-    for (int32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    int32_t *keys = LDS ? s_keys : g_keys + (int64_t)blockIdx.x * g_cap;
+    float *vals = NUMERIC ? (LDS ? s_vals : g_vals + (int64_t)blockIdx.x * g_cap) : nullptr;
+    const int32_t nrows = (int32_t)cnt[0];
+    for (int32_t i = blockIdx.x; i < nrows; i += gridDim.x) {
         const int32_t r = rows[i];
-        uint32_t tsize = kBlockTable;
-        if (GLOBAL) {
+        uint32_t tsize = TABLE;
+        if (!LDS) {
             tsize = 1;
             while (tsize < 2 * (uint64_t)ub[r]) tsize <<= 1;
         }
-        row_product<NUMERIC, false>(r, arp, acol, aval, brp, bcol, bval, keys, vals, tsize,
-                                    threadIdx.x, kBlock, row_nnz, crp, ccol, cval, &s_count);
+        const uint32_t mask = tsize - 1;
+        for (uint32_t s = threadIdx.x; s < tsize; s += BLOCK) {
+            keys[s] = kEmpty;
+            if (NUMERIC) vals[s] = 0.0f;
+        }
+        if (threadIdx.x == 0) s_count = 0;
+        __syncthreads();
+        uint32_t created = 0;
+        for_products<BLOCK>(r, arp, acol, aval, brp, bcol, bval, pre, rs, av, s_scan,
+                            [&](int32_t col, float v) {
+                                uint32_t slot;
+                                created += tab_insert(keys, mask, col, &slot) ? 1u : 0u;
+                                if (NUMERIC) atomicAdd(vals + slot, v);
+                            });
+        if (!NUMERIC) {
+            if (created) atomicAdd(&s_count, created);
+            __syncthreads();
+            if (threadIdx.x == 0) row_nnz[r] = (int32_t)s_count;
+            __syncthreads();
+            continue;
+        }
+        const uint32_t nz = compact_table<BLOCK>(keys, vals, tsize, s_scan, &s_count);
+        uint32_t p = 1;
+        while (p < nz) p <<= 1;
+        bitonic(keys, vals, p, threadIdx.x, BLOCK, [] { __syncthreads(); });
+        const int32_t c0 = crp[r];
+        for (uint32_t s = threadIdx.x; s < nz; s += BLOCK) {
+            ccol[c0 + s] = keys[s];
+            cval[c0 + s] = vals[s];
+        }
+        __syncthreads();
+    }
+}
+
+// Rows of matrices with at most kDenseN columns: a dense LDS accumulator, one
+// value and one bit per column; the row comes out in column order.
+template <bool NUMERIC>
+__global__ __launch_bounds__(kBig) void k_rows_dense(const int32_t *rows, const uint32_t *cnt,
+                                                     int32_t ncols, const int32_t *arp,
+                                                     const int32_t *acol, const float *aval,
+                                                     const int32_t *brp, const int32_t *bcol,
+                                                     const float *bval, int32_t *row_nnz,
+                                                     const int32_t *crp, int32_t *ccol,
+                                                     float *cval) {
+    constexpr int NWORD = kDenseN / 32;
+    __shared__ float s_val[NUMERIC ? kDenseN : 1];
+    __shared__ uint32_t s_bits[NWORD];
+    __shared__ int32_t pre[kChunk + 1], rs[kChunk];
+    __shared__ float av[kChunk];
+    __shared__ uint32_t s_scan[kBig / 64 + 1];
+    const int32_t nw = (ncols + 31) / 32;
+    const int32_t nrows = (int32_t)cnt[0];
+    for (int32_t i = blockIdx.x; i < nrows; i += gridDim.x) {
+        const int32_t r = rows[i];
+        for (int32_t q = threadIdx.x; q < nw; q += kBig) s_bits[q] = 0;
+        if (NUMERIC)
+            for (int32_t q = threadIdx.x; q < ncols; q += kBig) s_val[q] = 0.0f;
+        __syncthreads();
+        for_products<kBig>(r, arp, acol, aval, brp, bcol, bval, pre, rs, av, s_scan,
+                           [&](int32_t col, float v) {
+                               atomicOr(s_bits + (col >> 5), 1u << (col & 31));
+                               if (NUMERIC) atomicAdd(s_val + col, v);
+                           });
+        // prefix of the words' popcounts -> each word's first output slot
+        uint32_t carry = 0;
+        const int32_t c0 = NUMERIC ? crp[r] : 0;
+        for (int32_t q0 = 0; q0 < nw; q0 += kBig) {
+            const int32_t q = q0 + threadIdx.x;
+            const uint32_t bits = q < nw ? s_bits[q] : 0u;
+            uint32_t tot;
+            const uint32_t ex = dev::block_exclusive_add32<kBig>((uint32_t)__popc(bits), s_scan, tot);
+            if (NUMERIC && bits) {
+                uint32_t b = bits, o = carry + ex;
+                while (b) {
+                    const int t = __ffs(b) - 1;
+                    b &= b - 1;
+                    const int32_t col = q * 32 + t;
+                    ccol[c0 + o] = col;
+                    cval[c0 + o] = s_val[col];
+                    ++o;
+                }
+            }
+            carry += tot;
+        }
+        if (!NUMERIC && threadIdx.x == 0) row_nnz[r] = (int32_t)carry;
+        __syncthreads();
     }
 }
 
@@ -217,42 +394,47 @@ Graph *spgemm_run(Ctx *c, const Graph *a, const Graph *b) {
         C->rowptr = (int32_t *)dev_alloc(c, sizeof(int32_t) * (m + 1));
         int64_t *ub = (int64_t *)scratch(c, "sg_ub", sizeof(int64_t) * std::max(m, 1));
         int32_t *row_nnz = (int32_t *)scratch(c, "sg_rownnz", sizeof(int32_t) * (m + 1));
-        int32_t *lists = (int32_t *)scratch(c, "sg_lists", sizeof(int32_t) * 3 * (size_t)std::max(m, 1));
+        int32_t *lists = (int32_t *)scratch(c, "sg_lists", sizeof(int32_t) * 4 * (size_t)std::max(m, 1));
         uint32_t *ctl = (uint32_t *)scratch(c, "sg_ctl", 64);
         fill_i32(c, (int32_t *)ctl, 0, 16);
+        const bool dense_ok = b->n <= kDenseN;
         if (m > 0) {
-            k_row_bound<<<grid, kBlock, 0, c->stream>>>(a->rowptr, a->col, b->rowptr, m, ub);
-            k_bin_rows<<<grid, kBlock, 0, c->stream>>>(ub, m, lists, ctl, row_nnz);
+            k_row_bound<<<grid, kBlock, 0, c->stream>>>(a->rowptr, a->col, b->rowptr, m, b->n, ub);
+            k_bin_rows<<<grid, kBlock, 0, c->stream>>>(ub, m, dense_ok, lists, ctl, row_nnz);
             APSPIE_LAUNCH_CHECK();
         }
-        uint32_t h[4] = {0, 0, 0, 0};
+        uint32_t h[5] = {0, 0, 0, 0, 0};
         APSPIE_HIP(hipMemcpyAsync(h, ctl, sizeof h, hipMemcpyDeviceToHost, c->stream));
         sync(c);
         fill_i32(c, row_nnz + m, 0, 1);
         int64_t gcap = 1;
-        while (gcap < 2 * (int64_t)h[3]) gcap <<= 1;
-        // spill tables: at most ~2 GiB of keys + values in flight
+        while (gcap < 2 * (int64_t)h[4]) gcap <<= 1;
+        // global tables: at most ~2 GiB of keys + values in flight
         const int64_t by_mem = std::max<int64_t>(1, ((int64_t)1 << 31) / (8 * gcap));
         const int gblocks = (int)std::min<int64_t>(
-            std::min<int64_t>(std::max<uint32_t>(h[2], 1), 2 * c->cus), by_mem);
-        int32_t *gkeys = h[2] ? (int32_t *)scratch(c, "sg_gkeys", sizeof(int32_t) * gcap * gblocks) : nullptr;
-        float *gvals = h[2] ? (float *)scratch(c, "sg_gvals", sizeof(float) * gcap * gblocks) : nullptr;
+            std::min<int64_t>(std::max<uint32_t>(h[3], 1), c->cus), by_mem);
+        int32_t *gkeys = h[3] ? (int32_t *)scratch(c, "sg_gkeys", sizeof(int32_t) * gcap * gblocks) : nullptr;
+        float *gvals = h[3] ? (float *)scratch(c, "sg_gvals", sizeof(float) * gcap * gblocks) : nullptr;
         auto pass = [&](auto numeric_tag, int32_t *crp, int32_t *ccol, float *cval) {
             constexpr bool NUM = decltype(numeric_tag)::value;
+            if (h[3])  // longest rows first
+                k_rows_table<NUM, kBig, 0><<<gblocks, kBig, 0, c->stream>>>(
+                    lists + 3 * (size_t)m, ctl + 3, a->rowptr, a->col, a->val, b->rowptr, b->col,
+                    b->val, row_nnz, crp, ccol, cval, ub, gkeys, gvals, gcap);
+            if (h[2])
+                k_rows_dense<NUM><<<(int)std::min<int64_t>(h[2], c->cus), kBig, 0, c->stream>>>(
+                    lists + 2 * (size_t)m, ctl + 2, b->n, a->rowptr, a->col, a->val, b->rowptr,
+                    b->col, b->val, row_nnz, crp, ccol, cval);
+            if (h[1])
+                k_rows_table<NUM, kBlock, kBlockTable>
+                    <<<(int)std::min<int64_t>(h[1], 4 * c->cus), kBlock, 0, c->stream>>>(
+                        lists + m, ctl + 1, a->rowptr, a->col, a->val, b->rowptr, b->col, b->val,
+                        row_nnz, crp, ccol, cval, ub, nullptr, nullptr, 0);
             if (h[0])
                 k_rows_wave<NUM><<<(int)std::min<int64_t>(ceil_div(h[0], kBlock / 64), 8 * c->cus),
                                    kBlock, 0, c->stream>>>(lists, ctl + 0, a->rowptr, a->col,
                                                            a->val, b->rowptr, b->col, b->val,
                                                            row_nnz, crp, ccol, cval);
-            if (h[1])
-                k_rows_block<NUM, false><<<(int)std::min<int64_t>(h[1], 4 * c->cus), kBlock, 0,
-                                           c->stream>>>(lists + m, ctl + 1, a->rowptr, a->col,
-                                                        a->val, b->rowptr, b->col, b->val, row_nnz,
-                                                        crp, ccol, cval, ub, nullptr, nullptr, 0);
-            if (h[2])
-                k_rows_block<NUM, true><<<gblocks, kBlock, 0, c->stream>>>(
-                    lists + 2 * (size_t)m, ctl + 2, a->rowptr, a->col, a->val, b->rowptr, b->col,
-                    b->val, row_nnz, crp, ccol, cval, ub, gkeys, gvals, gcap);
             APSPIE_LAUNCH_CHECK();
         };
         pass(std::false_type{}, nullptr, nullptr, nullptr);

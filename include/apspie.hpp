@@ -10,7 +10,7 @@
 //     argument (a CPU-computed level count, test/bfs/main.cu:129-130) is
 //     accepted and ignored -- with the CPU depth the outputs are the same;
 //   * sssp() uses the real weights in every round (the reference switches to
-//     unit weights after round 1, sssp.cuh:83,91);
+//     unit weights after its first relaxation round, sssp.cuh:83,91);
 //   * spmspvMis() takes the priorities as given (any positive float order, or
 //     uint32) with ties broken to the lower id; `delta` (the reference's
 //     candidate threshold schedule) changes no result and is ignored;
@@ -25,6 +25,8 @@
 #include <cstdlib>
 #include <stdexcept>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "apspie.h"
 
@@ -51,37 +53,67 @@ inline void check(int st, apspie_ctx *ctx, const char *what) {
 #endif
 }
 
-// Replaces mgpu::CudaContext (one device, one stream).
+// Replaces mgpu::CudaContext (one device, one stream).  It also keeps the
+// graph views of the caller's device arrays between calls, so what is derived
+// once per graph (the symmetry check that enables pulling, TC's row counts)
+// is not redone on every call with the same arrays -- the reference's drivers
+// call bfs() / sssp() repeatedly on one CSR.  After changing the contents of
+// arrays already passed in, call forget() (or use new arrays).
 class Context {
   public:
     explicit Context(int device = 0) { check(apspie_ctx_create(device, &h_), nullptr, "context"); }
-    ~Context() { if (h_) apspie_ctx_destroy(h_); }
+    ~Context() {
+        forget();
+        if (h_) apspie_ctx_destroy(h_);
+    }
     Context(const Context &) = delete;
     Context &operator=(const Context &) = delete;
     apspie_ctx *get() const { return h_; }
 
+    const apspie_graph *view(int m, int n, int64_t nnz, const int *d_rowptr, const int *d_col,
+                             const float *d_val) {
+        const Key k{m, n, nnz, d_rowptr, d_col, d_val};
+        for (auto &e : views_)
+            if (e.first == k) return e.second;
+        apspie_csr c{m, n, nnz, d_rowptr, d_col, d_val};
+        apspie_graph *g = nullptr;
+        check(apspie_graph_wrap(h_, &c, &g), h_, "graph_wrap");
+        views_.emplace_back(k, g);
+        return g;
+    }
+    void forget() {
+        for (auto &e : views_) apspie_graph_destroy(e.second);
+        views_.clear();
+    }
+
   private:
+    struct Key {
+        int m, n;
+        int64_t nnz;
+        const void *rowptr, *col, *val;
+        bool operator==(const Key &o) const {
+            return m == o.m && n == o.n && nnz == o.nnz && rowptr == o.rowptr && col == o.col &&
+                   val == o.val;
+        }
+    };
     apspie_ctx *h_ = nullptr;
+    std::vector<std::pair<Key, apspie_graph *>> views_;
 };
 
 // mgpu::CreateCudaDevice(0) (test/bfs/main.cu:27)
 inline Context *CreateDevice(int device) { return new Context(device); }
 
-// A non-owning graph view over caller-owned device CSR arrays.
+// A non-owning graph view over caller-owned device CSR arrays (kept by the
+// context for later calls with the same arrays).
 class GraphView {
   public:
     GraphView(Context &ctx, int m, int n, int64_t nnz, const int *d_rowptr, const int *d_col,
-              const float *d_val) {
-        apspie_csr c{m, n, nnz, d_rowptr, d_col, d_val};
-        check(apspie_graph_wrap(ctx.get(), &c, &g_), ctx.get(), "graph_wrap");
-    }
-    ~GraphView() { if (g_) apspie_graph_destroy(g_); }
-    GraphView(const GraphView &) = delete;
-    GraphView &operator=(const GraphView &) = delete;
+              const float *d_val)
+        : g_(ctx.view(m, n, nnz, d_rowptr, d_col, d_val)) {}
     const apspie_graph *get() const { return g_; }
 
   private:
-    apspie_graph *g_ = nullptr;
+    const apspie_graph *g_;
 };
 
 // d_matrix (matrix.hpp:6-37) in its CSR use: device arrays owned by a

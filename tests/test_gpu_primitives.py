@@ -94,10 +94,24 @@ def _scipy_csr(g_or_mat):
     return g_or_mat
 
 
-@pytest.mark.parametrize("case", ["rect", "square_int", "rmat_sq", "hub"])
+@pytest.mark.parametrize("case", ["rect", "square_int", "rmat_sq", "hub", "rmat12_sq",
+                                  "wide_global"])
 def test_spgemm_vs_oracle(ctx, case):
+    """All accumulator tiers: wave and block LDS tables, the dense accumulator
+    (a row of 7000 columns in 'hub', R-MAT-12 squared) and the global table
+    (a row of 50000 possible columns in 'wide_global')."""
     rng = np.random.default_rng(3)
-    if case == "rect":
+    if case == "rmat12_sq":
+        g = T.rmat(12)
+        Am = sp.csr_matrix((np.ones(len(g.col)), g.col, g.rowptr), shape=(g.m, g.m))
+        Bm = Am
+    elif case == "wide_global":
+        Am = sp.lil_matrix((20, 300))
+        Am[0, :] = 1
+        Am[1, :10] = 1
+        Am = Am.tocsr()
+        Bm = sp.random(300, 50000, 0.01, random_state=6, format="csr")
+    elif case == "rect":
         Am = sp.random(300, 200, 0.05, random_state=1, format="csr")
         Bm = sp.random(200, 400, 0.05, random_state=2, format="csr")
     elif case == "square_int":
