@@ -61,6 +61,14 @@ struct Graph {
     // derived, built lazily and cached with the graph
     int32_t *lowcnt = nullptr;  // per row: number of cols < row (TC)
     int symmetric = -1;         // 1 / 0 once known (pull traversal needs A == A^T)
+    int sym_weights = -1;       // 1 / 0 once known: values equal to the transpose's too
+    // rows longer than the SSSP pull's lane limit, as a frontier over those
+    // vertices (entry vertex, row start, edge offset, tile owners; one block
+    // of device memory owned by the graph) and its (entries << 32 | edges)
+    void *heavy_mem = nullptr;
+    int32_t *heavy_v = nullptr, *heavy_start = nullptr, *heavy_off = nullptr,
+            *heavy_tfirst = nullptr;
+    uint64_t heavy_cnt = 0;
     std::mutex derived_mu;      // guards lazily built derived data
 };
 
@@ -143,6 +151,9 @@ void fill_i32(Ctx *c, int32_t *p, int32_t v, int64_t n);
 void graph_prepare_tc(Ctx *c, Graph *g);
 // Whether the CSR equals its transpose (cached with the graph).
 bool graph_is_symmetric(Ctx *c, Graph *g);
+// symmetric structure and values (pulling a weighted relaxation reads w(v, u)
+// for w(u, v))
+bool graph_is_symmetric_weighted(Ctx *c, Graph *g);
 
 }  // namespace apspie
 

@@ -15,6 +15,8 @@ static const char *kOptionNames[] = {
     "bfs_alpha",         // push->pull switch: frontier edges * alpha > unexplored edges
     "bfs_beta",          // pull->push switch: frontier vertices * beta < m
     "sssp_delta",        // near/far split width (0 = plain Bellman-Ford frontier)
+    "sssp_pull_div",     // pull an SSSP round when frontier edges * this > nnz (0 = never;
+                         // symmetric weights only)
     "spmspv_path",       // 0 auto, 1 sort, 2 atomic
     "spmspv_sort_max",   // auto: use the sort path up to this many expanded edges
     "tc_small_row",      // rows with |L_r| <= this use a wave + LDS hash
@@ -34,6 +36,7 @@ static int64_t option_default(const std::string &n) {
     if (n == "bfs_alpha") return 14;
     if (n == "bfs_beta") return 24;
     if (n == "sssp_delta") return 0;
+    if (n == "sssp_pull_div") return 0;
     if (n == "spmspv_path") return 0;
     if (n == "spmspv_sort_max") return 1 << 16;
     if (n == "tc_small_row") return 256;
@@ -263,6 +266,7 @@ int apspie_ctx_set_option(apspie_ctx *c, const char *name, int64_t v) {
     if (n == "batch_pull_div" && (v < 0 || v > (1 << 20))) return APSPIE_ERR_INVALID;
     if ((n == "batch_sssp_delta" || n == "sssp_delta") && (v < 0 || v > (1 << 30)))
         return APSPIE_ERR_INVALID;
+    if (n == "sssp_pull_div" && (v < 0 || v > (1 << 20))) return APSPIE_ERR_INVALID;
     if (n == "profiling") c->profiling = v != 0;
     c->options[n] = v;
     return APSPIE_OK;
@@ -452,6 +456,7 @@ int apspie_graph_destroy(apspie_graph *g) {
         }
     }
     if (g->lowcnt) (void)hipFree(g->lowcnt);
+    if (g->heavy_mem) (void)hipFree(g->heavy_mem);
     delete g;
     return APSPIE_OK;
 }

@@ -183,29 +183,47 @@ __global__ void k_neq(const int32_t *a, const int32_t *b, int64_t n, int32_t *fl
         if (a[i] != b[i]) flag[0] = 1;
 }
 
-bool graph_is_symmetric(Ctx *c, Graph *g) {
-    std::lock_guard<std::mutex> lock(g->derived_mu);
-    if (g->symmetric >= 0) return g->symmetric == 1;
+// Structure (and, for a weighted graph, values) compared with the transpose,
+// once per graph.  The transpose's rows come out sorted, so with equal
+// structure the values line up position by position.
+static void check_symmetry(Ctx *c, Graph *g) {
+    if (g->symmetric >= 0) return;
     if (g->m != g->n) {
         g->symmetric = 0;
-        return false;
+        g->sym_weights = 0;
+        return;
     }
     Graph t;
     graph_transpose_dev(c, g, &t);
     int32_t *flag = (int32_t *)scratch(c, "sym_flag", 256);
-    fill_i32(c, flag, 0, 1);
+    fill_i32(c, flag, 0, 2);
     const int grid = fill_grid(c, kBlock, 4);
     k_neq<<<grid, kBlock, 0, c->stream>>>(g->rowptr, t.rowptr, g->m + 1, flag);
     k_neq<<<grid, kBlock, 0, c->stream>>>(g->col, t.col, g->nnz, flag);
+    if (g->val && t.val)
+        k_neq<<<grid, kBlock, 0, c->stream>>>((const int32_t *)g->val, (const int32_t *)t.val,
+                                               g->nnz, flag + 1);
     APSPIE_LAUNCH_CHECK();
-    int32_t h = 1;
-    APSPIE_HIP(hipMemcpyAsync(&h, flag, 4, hipMemcpyDeviceToHost, c->stream));
+    int32_t h[2] = {1, 1};
+    APSPIE_HIP(hipMemcpyAsync(h, flag, 8, hipMemcpyDeviceToHost, c->stream));
     sync(c);
     dev_free(c, t.rowptr, sizeof(int32_t) * (t.m + 1));
     dev_free(c, t.col, sizeof(int32_t) * std::max<int64_t>(t.nnz, 1));
     if (t.val) dev_free(c, t.val, sizeof(float) * std::max<int64_t>(t.nnz, 1));
-    g->symmetric = h == 0 ? 1 : 0;
+    g->symmetric = h[0] == 0 ? 1 : 0;
+    g->sym_weights = (g->symmetric == 1 && g->val && h[1] == 0) ? 1 : 0;
+}
+
+bool graph_is_symmetric(Ctx *c, Graph *g) {
+    std::lock_guard<std::mutex> lock(g->derived_mu);
+    check_symmetry(c, g);
     return g->symmetric == 1;
+}
+
+bool graph_is_symmetric_weighted(Ctx *c, Graph *g) {
+    std::lock_guard<std::mutex> lock(g->derived_mu);
+    check_symmetry(c, g);
+    return g->sym_weights == 1;
 }
 
 void graph_transpose_dev(Ctx *c, const Graph *g, Graph *t) {

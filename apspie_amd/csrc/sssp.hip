@@ -11,8 +11,9 @@
 // down in the previous round; each expanded edge relaxes its head with an
 // atomic min on the distance bits (non-negative floats order like their bit
 // patterns), and the thread whose relaxation lowered a vertex first in this
-// round appends it to the next frontier (per-vertex round stamp).  With
-// integer-valued weights every sum is exact, so the fixed point does not
+// round appends it to the next frontier (the atomic OR that sets its bit in
+// the next frontier's bitmap; near-far rounds use a per-vertex round stamp).
+// With integer-valued weights every sum is exact, so the fixed point does not
 // depend on the relaxation order.
 #include "common.hpp"
 #include "device.hpp"
@@ -23,23 +24,7 @@ namespace apspie {
 
 namespace {
 
-constexpr int kBlock = 256;
 constexpr float kInf = APSPIE_SSSP_INF;
-
-struct SsspOp {
-    const float *w;
-    int32_t *dist;  // float bits
-    int32_t *stamp;
-    int32_t round;  // index of the frontier being built
-    __device__ __forceinline__ bool edge(int32_t u, int32_t e, int32_t v) const {
-        const float du = dev::bitsf(dist[u]);
-        const int32_t nd = dev::fbits(du + w[e]);
-        if (nd >= dist[v]) return false;  // not better than what we see
-        const int32_t old = dev::atomic_min_dev(dist + v, nd);
-        if (nd >= old) return false;
-        return dev::atomic_exch_dev(stamp + v, round) != round;
-    }
-};
 
 template <int TILE>
 __global__ void k_sssp_seed(const int32_t *rowptr, int32_t src, int32_t *dist, int32_t *stamp,
@@ -58,6 +43,312 @@ __global__ void k_sssp_seed(const int32_t *rowptr, int32_t src, int32_t *dist, i
         f0.off[0] = 0;
     }
     for (int32_t t = threadIdx.x; (int64_t)t * TILE < d; t += blockDim.x) f0.tfirst[t] = 0;
+}
+
+// -------------------------------------------------------- pull rounds
+// Direction-optimising Bellman-Ford for a CSR equal to its transpose (values
+// included): a round whose frontier holds more than nnz / sssp_pull_div edges
+// is pulled instead of pushed -- every vertex takes the minimum of
+// dist[u] + w(v, u) over its neighbours u in the frontier (a bitmap), which
+// is the same set of relaxations as pushing the frontier, with far fewer
+// atomics.  Rows of at most kLight entries are scanned by one lane (8
+// vertices per thread, 8 loads in flight) and written by that lane alone.
+// Longer rows are walked in tiles of kHvTile entries over a list of all such
+// vertices built once per graph: each thread takes kHvPer consecutive
+// entries (at most two rows, since every long row has more than kHvPer
+// entries), the minimum per row is reduced in LDS and applied with one atomic
+// min per tile and row.  The next frontier's bitmap doubles as the claim: the
+// thread whose atomic OR sets a vertex's bit appends it to the next queue.
+constexpr int kDoBlock = 1024;
+constexpr int kDoTile = 2048;  // tiles of the pushed queue
+constexpr int32_t kLight = 32;
+constexpr int kDoPer = 8;
+constexpr int kDoIlp = 8;
+constexpr int kHvPer = 8;
+constexpr int kHvTile = kDoBlock * kHvPer;
+constexpr int kHvEnt = kHvTile / (kLight + 1) + 3;  // rows meeting one tile, with slack
+static_assert(kHvEnt < kDoBlock, "one claim per thread");
+static_assert(kHvPer <= kLight, "a thread's entries span at most two rows");
+
+__device__ __forceinline__ bool claim_bit(uint32_t *bits, int32_t v) {
+    const uint32_t bit = 1u << (v & 31);
+    return !(dev::atomic_or_dev(bits + (v >> 5), bit) & bit);
+}
+
+struct SsspPushOp {
+    const float *w;
+    int32_t *dist;  // float bits
+    uint32_t *fb_next;
+    __device__ __forceinline__ bool edge(int32_t u, int32_t e, int32_t v) const {
+        const int32_t nd = dev::fbits(dev::bitsf(dist[u]) + w[e]);
+        if (nd >= dist[v]) return false;
+        if (nd >= dev::atomic_min_dev(dist + v, nd)) return false;
+        return claim_bit(fb_next, v);
+    }
+};
+
+struct HeavyLds {
+    int32_t off[kHvEnt + 1], start[kHvEnt], v[kHvEnt], best[kHvEnt];
+    uint64_t scan[kDoBlock / 64 + 1];
+    uint64_t base;
+};
+
+union DoLds {
+    ExpandLds<kDoBlock, kDoTile, true> ex;
+    HeavyLds hv;
+};
+
+struct HeavyOp {
+    const int32_t *rowptr;
+    __device__ __forceinline__ bool keep(int32_t v) const {
+        return rowptr[v + 1] - rowptr[v] > kLight;
+    }
+};
+
+__global__ void k_do_seed(const int32_t *rowptr, int32_t src, int32_t *dist, uint32_t *fb0,
+                          FrontierBufs f0, uint64_t *ctl) {
+    const int32_t rs = rowptr[src];
+    const int32_t d = rowptr[src + 1] - rs;
+    if (threadIdx.x == 0) {
+        dist[src] = dev::fbits(0.0f);
+        fb0[src >> 5] = 1u << (src & 31);
+        ctl[0] = d > 0 ? ((uint64_t(1) << 32) | uint64_t(d)) : 0;
+        f0.v[0] = src;
+        f0.start[0] = rs;
+        f0.off[0] = 0;
+    }
+    for (int32_t t = threadIdx.x; (int64_t)t * kDoTile < d; t += blockDim.x) f0.tfirst[t] = 0;
+}
+
+// two 1024-thread blocks per CU resident: at most 64 VGPRs
+__global__ __launch_bounds__(kDoBlock) __attribute__((amdgpu_waves_per_eu(8))) void
+k_sssp_do_round(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                const float *__restrict__ w, int32_t nnz, int32_t m, int32_t *dist,
+                const uint32_t *__restrict__ fb_cur, uint32_t *fb_next, uint32_t *fb_clear,
+                FrontierBufs cur, FrontierBufs nxt, FrontierBufs heavy, uint64_t heavy_cnt,
+                int32_t pull_div, uint64_t *ctl, int32_t r) {
+    __shared__ DoLds lds;
+    const uint64_t cc = ctl[r % 3];
+    const int32_t F = (int32_t)cnt_entries(cc), E = (int32_t)cnt_edges(cc);
+    const bool pull = pull_div > 0 && (int64_t)E * pull_div > (int64_t)nnz;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl[(r + 2) % 3] = 0;  // the counter of round r + 1's output
+        if (F > 0) {
+            ctl[4] += (uint64_t)E;
+            ctl[5] += (uint64_t)F;
+            ctl[6] += 1;
+            ctl[7] += pull ? (uint64_t)nnz : (uint64_t)E;  // CSR entries read
+        }
+    }
+    if (F == 0) return;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m / 32 + 1;
+         i += (int64_t)gridDim.x * blockDim.x)
+        fb_clear[i] = 0;
+    uint64_t *cnt_next = ctl + (r + 1) % 3;
+    if (!pull) {
+        SsspPushOp op{w, dist, fb_next};
+        expand_tiles<kDoBlock, kDoTile, true, true, true>(lds.ex, op, rowptr, col, nnz, cur, F,
+                                                          E, nxt, cnt_next);
+        return;
+    }
+    const int lane = dev::lane_id();
+    // short rows: one lane per vertex
+    for (int64_t pbase = (int64_t)blockIdx.x * kDoBlock * kDoPer; pbase < m;
+         pbase += (int64_t)gridDim.x * kDoBlock * kDoPer) {
+        int32_t crs[kDoPer], cd[kDoPer], best[kDoPer], d0[kDoPer];
+        uint32_t pend = 0;
+#pragma unroll
+        for (int k = 0; k < kDoPer; ++k) {
+            const int64_t v = pbase + (int64_t)k * kDoBlock + threadIdx.x;
+            const bool in = v < m;
+            crs[k] = in ? rowptr[v] : 0;
+            cd[k] = in ? rowptr[v + 1] - crs[k] : 0;
+            d0[k] = in ? dist[v] : 0;
+            best[k] = d0[k];
+            if (cd[k] > 0 && cd[k] <= kLight) pend |= 1u << k;
+        }
+        // each lane walks its short rows one after another, kDoIlp loads in flight
+        {
+            int32_t e = 0, lim = 0, kk = -1, bk = 0;
+            for (;;) {
+                if (e >= lim) {
+                    if (kk >= 0) {
+#pragma unroll
+                        for (int k = 0; k < kDoPer; ++k)
+                            if (k == kk) best[k] = bk;
+                    }
+                    if (!pend) break;
+                    kk = __ffs(pend) - 1;
+                    pend &= pend - 1;
+#pragma unroll
+                    for (int k = 0; k < kDoPer; ++k)
+                        if (k == kk) {
+                            e = crs[k];
+                            lim = crs[k] + cd[k];
+                            bk = best[k];
+                        }
+                }
+                int32_t u[kDoIlp];
+#pragma unroll
+                for (int q = 0; q < kDoIlp; ++q) u[q] = e + q < lim ? col[e + q] : -1;
+                bool f[kDoIlp];
+#pragma unroll
+                for (int q = 0; q < kDoIlp; ++q)
+                    f[q] = u[q] >= 0 && ((fb_cur[u[q] >> 5] >> (u[q] & 31)) & 1u);
+#pragma unroll
+                for (int q = 0; q < kDoIlp; ++q)
+                    if (f[q]) bk = min(bk, dev::fbits(dev::bitsf(dist[u[q]]) + w[e + q]));
+                e += kDoIlp;
+            }
+        }
+        int32_t pc[kDoPer], prs[kDoPer], pcd[kDoPer];
+#pragma unroll
+        for (int k = 0; k < kDoPer; ++k) {
+            const int32_t v = (int32_t)(pbase + (int64_t)k * kDoBlock + threadIdx.x);
+            const bool imp = cd[k] > 0 && cd[k] <= kLight && best[k] < d0[k];
+            pc[k] = v;
+            prs[k] = imp ? crs[k] : 0;
+            pcd[k] = imp ? cd[k] : 0;
+            if (imp) dist[v] = best[k];
+            const uint64_t bm = __ballot(imp);
+            const int64_t wb = (pbase + (int64_t)k * kDoBlock + (threadIdx.x & ~63)) >> 5;
+            if (lane == 0 && (uint32_t)bm) dev::atomic_or_dev(fb_next + wb, (uint32_t)bm);
+            if (lane == 32 && (uint32_t)(bm >> 32))
+                dev::atomic_or_dev(fb_next + wb + 1, (uint32_t)(bm >> 32));
+        }
+        produce<kDoBlock, kDoPer, kDoTile, true>(pc, prs, pcd, nxt, cnt_next, lds.hv.scan,
+                                                 &lds.hv.base);
+    }
+    // long rows: tiles over the long-row list
+    const int32_t HF = (int32_t)cnt_entries(heavy_cnt), HE = (int32_t)cnt_edges(heavy_cnt);
+    if (HF == 0) return;
+    HeavyLds &L = lds.hv;
+    const int32_t ntiles = (HE + kHvTile - 1) / kHvTile;
+    for (int32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int32_t i0 = min(max(heavy.tfirst[t], 0), HF - 1);
+        const int32_t i1 = (t + 1 < ntiles) ? min(max(heavy.tfirst[t + 1], i0), HF - 1) : HF - 1;
+        const int32_t nv = min(i1 - i0 + 1, kHvEnt);
+        // off[nv]: where the row after the tile's last one starts (or HE)
+        for (int32_t j = threadIdx.x; j <= nv; j += kDoBlock) {
+            L.off[j] = i0 + j < HF ? heavy.off[i0 + j] : HE;
+            if (j < nv) {
+                L.start[j] = heavy.start[i0 + j];
+                L.v[j] = heavy.v[i0 + j];
+                L.best[j] = 0x7fffffff;
+            }
+        }
+        __syncthreads();
+        const int32_t x0 = t * kHvTile + threadIdx.x * kHvPer;
+        if (x0 < HE) {
+            const int j = dev::upper_bound_m1(L.off, 0, nv, x0);
+            const int32_t split = L.off[j + 1];  // first entry of row j + 1
+            int32_t idx[kHvPer], u[kHvPer];
+#pragma unroll
+            for (int k = 0; k < kHvPer; ++k) {
+                const int32_t x = x0 + k;
+                const int jj = x >= split ? j + 1 : j;
+                idx[k] = -1;
+                if (x < HE && jj < nv) idx[k] = L.start[jj] + (x - L.off[jj]);
+                if ((uint32_t)idx[k] >= (uint32_t)nnz) idx[k] = -1;  // guard only
+                u[k] = idx[k] >= 0 ? col[idx[k]] : -1;
+            }
+            bool f[kHvPer];
+#pragma unroll
+            for (int k = 0; k < kHvPer; ++k)
+                f[k] = u[k] >= 0 && ((fb_cur[u[k] >> 5] >> (u[k] & 31)) & 1u);
+            int32_t b0 = 0x7fffffff, b1 = 0x7fffffff;
+#pragma unroll
+            for (int k = 0; k < kHvPer; ++k)
+                if (f[k]) {
+                    const int32_t c = dev::fbits(dev::bitsf(dist[u[k]]) + w[idx[k]]);
+                    if (x0 + k >= split) b1 = min(b1, c);
+                    else b0 = min(b0, c);
+                }
+            if (b0 != 0x7fffffff) atomicMin(&L.best[j], b0);
+            if (b1 != 0x7fffffff) atomicMin(&L.best[j + 1], b1);
+        }
+        __syncthreads();
+        int32_t hc[1] = {0}, hrs[1] = {0}, hcd[1] = {0};
+        if (threadIdx.x < nv && L.best[threadIdx.x] != 0x7fffffff) {
+            const int32_t j = threadIdx.x, v = L.v[j], b = L.best[j];
+            if (b < dev::atomic_min_dev(dist + v, b) && claim_bit(fb_next, v)) {
+                hc[0] = v;
+                hrs[0] = L.start[j];
+                hcd[0] = L.off[j + 1] - L.off[j];
+            }
+        }
+        produce<kDoBlock, 1, kDoTile, true>(hc, hrs, hcd, nxt, cnt_next, L.scan, &L.base);
+        __syncthreads();  // the tile's LDS is refilled next
+    }
+}
+
+// The long-row list of a graph (rows longer than kLight, with tile owners
+// every kHvTile entries), built once and kept with the graph.
+void graph_prepare_heavy(Ctx *c, Graph *g) {
+    std::lock_guard<std::mutex> lock(g->derived_mu);
+    if (g->heavy_mem || g->m == 0) return;
+    const int64_t ent = g->nnz / (kLight + 1) + 2, tiles = g->nnz / kHvTile + 2;
+    const size_t bytes = sizeof(int32_t) * (3 * ent + tiles) + 64;
+    // owned by the graph (freed with it), like its other derived arrays
+    APSPIE_HIP(hipMalloc(&g->heavy_mem, bytes));
+    int32_t *p = (int32_t *)g->heavy_mem;
+    g->heavy_v = p;
+    g->heavy_start = p + ent;
+    g->heavy_off = p + 2 * ent;
+    g->heavy_tfirst = p + 3 * ent;
+    uint64_t *cnt = (uint64_t *)scratch(c, "sssp_heavy_cnt", 64);
+    fill_i32(c, (int32_t *)cnt, 0, 2);
+    FrontierBufs hb{g->heavy_v, g->heavy_start, g->heavy_off, g->heavy_tfirst};
+    const int grid = fill_grid(c, kDoBlock, 2);
+    k_build<kDoBlock, 8, kHvTile, true><<<grid, kDoBlock, 0, c->stream>>>(
+        HeavyOp{g->rowptr}, g->rowptr, nullptr, g->m, nullptr, hb, cnt);
+    APSPIE_LAUNCH_CHECK();
+    APSPIE_HIP(hipMemcpyAsync(&g->heavy_cnt, cnt, 8, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+}
+
+void sssp_do(Ctx *c, const Graph *g0, int32_t source, float *dist_out, int32_t pull_div) {
+    Graph *g = const_cast<Graph *>(g0);
+    graph_prepare_heavy(c, g);
+    FrontierSet fs = frontier_alloc(c, "sssp", g->m, g->nnz, kDoTile, true);
+    int32_t *dist = (int32_t *)dist_out;
+    const int64_t words = g->m / 32 + 1;
+    uint32_t *fb = (uint32_t *)scratch(c, "sssp_fb", sizeof(uint32_t) * 3 * words);
+    fill_i32(c, dist, __builtin_bit_cast(int32_t, kInf), g->m);
+    fill_i32(c, (int32_t *)fb, 0, 3 * words);
+    fill_i32(c, (int32_t *)fs.ctl, 0, 2 * kCtlWords);
+    k_do_seed<<<1, 256, 0, c->stream>>>(g->rowptr, source, dist, fb, fs.f[0], fs.ctl);
+    APSPIE_LAUNCH_CHECK();
+    const FrontierBufs heavy{g->heavy_v, g->heavy_start, g->heavy_off, g->heavy_tfirst};
+    const int grid = fill_grid(
+        c, kDoBlock, (int)std::max<int64_t>(1, opt(c, "bfs_blocks_per_cu", 8) * 256 / kDoBlock));
+    int chunk = (int)std::max<int64_t>(1, opt(c, "bfs_levels_per_sync", 8));
+    KernelTimer kt(c);
+    uint64_t *h = (uint64_t *)c->pinned;
+    int r = 0;
+    for (;;) {
+        kt.begin();
+        for (int k = 0; k < chunk; ++k, ++r) {
+            k_sssp_do_round<<<grid, kDoBlock, 0, c->stream>>>(
+                g->rowptr, g->col, g->val, (int32_t)g->nnz, g->m, dist, fb + (r % 3) * words,
+                fb + ((r + 1) % 3) * words, fb + ((r + 2) % 3) * words, fs.f[r & 1],
+                fs.f[(r + 1) & 1], heavy, g->heavy_cnt, pull_div, fs.ctl, r);
+            APSPIE_LAUNCH_CHECK();
+        }
+        kt.end(chunk);
+        APSPIE_HIP(hipMemcpyAsync(h, fs.ctl, 16 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                  c->stream));
+        sync(c);
+        c->stats.host_syncs++;
+        if (cnt_entries(h[r % 3]) == 0) break;
+        APSPIE_CHECK(r <= 4 * g->m + 64, APSPIE_ERR_DEVICE, "SSSP did not converge");
+        chunk = std::min(chunk * 2, 256);
+    }
+    c->stats.edges_traversed = (int64_t)h[4];
+    c->stats.vertices_reached = (int64_t)h[5];
+    c->stats.iterations = (int32_t)h[6];
+    c->stats.edges_checked = (int64_t)h[7];
+    kt.collect();
 }
 
 // ---------------------------------------------------------------- near-far
@@ -225,50 +516,6 @@ void sssp_near_far(Ctx *c, const Graph *g, int32_t source, float *dist_out, floa
     kt.collect();
 }
 
-template <int TILE>
-void sssp_rounds(Ctx *c, const Graph *g, int32_t source, float *dist_out) {
-    FrontierSet fs = frontier_alloc(c, "sssp", g->m, g->nnz, TILE, true);
-    int32_t *dist = (int32_t *)dist_out;
-    int32_t *stamp = (int32_t *)scratch(c, "sssp_stamp", sizeof(int32_t) * g->m);
-    fill_i32(c, dist, __builtin_bit_cast(int32_t, kInf), g->m);
-    fill_i32(c, stamp, -1, g->m);
-    k_sssp_seed<TILE><<<1, 256, 0, c->stream>>>(g->rowptr, source, dist, stamp, fs.f[0], fs.ctl);
-    APSPIE_LAUNCH_CHECK();
-
-    const int grid = fill_grid(c, kBlock, (int)opt(c, "bfs_blocks_per_cu", 4));
-    int chunk = (int)std::max<int64_t>(1, opt(c, "bfs_levels_per_sync", 8));
-    KernelTimer kt(c);
-    int round = 0;
-    for (;;) {
-        kt.begin();
-        for (int k = 0; k < chunk; ++k, ++round) {
-            const int cur = round & 1;
-            k_expand<kBlock, TILE, true, true, true><<<grid, kBlock, 0, c->stream>>>(
-                SsspOp{g->val, dist, stamp, round + 1}, g->rowptr, g->col, (int32_t)g->nnz,
-                fs.f[cur], fs.ctl + (round % 3), fs.f[cur ^ 1], fs.ctl + ((round + 1) % 3),
-                fs.ctl + ((round + 2) % 3), fs.ctl + 4);
-            APSPIE_LAUNCH_CHECK();
-        }
-        kt.end(chunk);
-        uint64_t *h = (uint64_t *)c->pinned;
-        APSPIE_HIP(hipMemcpyAsync(h, fs.ctl + (round % 3), sizeof(uint64_t), hipMemcpyDeviceToHost,
-                                  c->stream));
-        APSPIE_HIP(hipMemcpyAsync(h + 1, fs.ctl + 4, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost,
-                                  c->stream));
-        sync(c);
-        c->stats.host_syncs++;
-        if (cnt_entries(h[0]) == 0) break;
-        // each round lowers at least one distance; integer weights bound the rounds
-        APSPIE_CHECK(round <= 4 * g->m + 64, APSPIE_ERR_DEVICE, "SSSP did not converge");
-        chunk = std::min(chunk * 2, 256);
-    }
-    uint64_t *h = (uint64_t *)c->pinned;
-    c->stats.edges_traversed = (int64_t)h[1];
-    c->stats.vertices_reached = (int64_t)h[2];
-    c->stats.iterations = (int32_t)h[3];
-    kt.collect();
-}
-
 }  // namespace
 
 void sssp_run(Ctx *c, const Graph *g, int32_t source, float *dist) {
@@ -280,11 +527,10 @@ void sssp_run(Ctx *c, const Graph *g, int32_t source, float *dist) {
         default: sssp_near_far<2048>(c, g, source, dist, (float)delta); return;
         }
     }
-    switch (opt(c, "bfs_tile", 2048)) {
-    case 1024: sssp_rounds<1024>(c, g, source, dist); break;
-    case 4096: sssp_rounds<4096>(c, g, source, dist); break;
-    default: sssp_rounds<2048>(c, g, source, dist); break;
-    }
+    // pulled rounds read w(v, u) for the edge u -> v: symmetric values only
+    int32_t pull_div = (int32_t)opt(c, "sssp_pull_div", 0);
+    if (pull_div > 0 && !graph_is_symmetric_weighted(c, const_cast<Graph *>(g))) pull_div = 0;
+    sssp_do(c, g, source, dist, pull_div);
 }
 
 }  // namespace apspie

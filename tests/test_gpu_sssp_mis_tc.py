@@ -78,6 +78,80 @@ def test_sssp_near_far_vs_oracle(ctx, key, delta):
         ctx.set_option("sssp_delta", 0)
 
 
+def wheel(n):
+    """hub n-1 joined to every rim vertex, rim a path: one row spans many tiles."""
+    rim = np.arange(n - 1)
+    u = np.concatenate([rim[:-1], rim])
+    v = np.concatenate([rim[1:], np.full(n - 1, n - 1)])
+    rp, c = O.edges_to_csr(n, u.astype(np.int32), v.astype(np.int32))
+    return rp, c
+
+
+@pytest.mark.parametrize("pull_div", [0, 1, 2, 1 << 20])
+@pytest.mark.parametrize("key", ["rmat10", "rmat14", "lattice64", "star", "wheel", "test_cc"])
+def test_sssp_pull_rounds_vs_oracle(ctx, key, pull_div):
+    """Rounds pulled over the frontier bitmap (symmetric structure and
+    weights) reach the oracle's distances: push only (0), pull on big rounds
+    (1, 2) and pull on every round (2^20); the star and the wheel put hubs of
+    3000 and 20000 entries into the tiled long-row pass, and a pulled round
+    reads every CSR entry."""
+    if key == "wheel":
+        rp, c = wheel(20000)
+        m, rowptr, col = 20000, rp, c
+        wt = O.assign_weights(m, rp, c)
+    else:
+        g = {"rmat10": lambda: T.rmat(10), "rmat14": lambda: T.rmat(14),
+             "lattice64": lambda: T.lattice(64), "star": lambda: T.star_graph(3000),
+             "test_cc": lambda: T.fixture("test_cc")}[key]()
+        m, rowptr, col, wt = g.m, g.rowptr, g.col, g.w
+        if key == "star":
+            wt = O.assign_weights(m, rowptr, col)
+    ctx.set_option("sssp_pull_div", pull_div)
+    try:
+        G = ctx.upload(rowptr.astype(np.int32), col, wt)
+        deg = np.diff(rowptr)
+        srcs = [int(s) for s in np.nonzero(deg > 0)[0][[0, -1]]] + [int(np.argmax(deg))]
+        for src in srcs:
+            want = to_ref_sentinel(O.sssp(m, rowptr, col, wt, src))
+            got = ctx.sssp(G, src)
+            assert np.array_equal(got, want), (key, pull_div, src)
+            st = ctx.stats()
+            if pull_div == 0:  # push only (the default)
+                assert st["edges_checked"] == st["edges_traversed"]
+            if pull_div == 1 << 20:
+                assert st["edges_checked"] == st["iterations"] * len(col)
+        G.close()
+    finally:
+        ctx.set_option("sssp_pull_div", 0)
+
+
+def test_sssp_pull_needs_symmetric_weights(ctx):
+    """A symmetric pattern with w(u, v) != w(v, u), and a directed graph, are
+    never pulled (a pull reads w(v, u) for the edge u -> v)."""
+    g = T.rmat(12)
+    rng = np.random.default_rng(5)
+    w = (1 + rng.integers(0, 63, len(g.col))).astype(np.float32)
+    rng = np.random.default_rng(17)
+    n = 3000
+    u = rng.integers(0, n, 30000)
+    v = rng.integers(0, n, 30000)
+    k = np.unique((u * n + v)[u != v])
+    drp, dc, _ = O.coo_to_csr(n, (k // n).astype(np.int32), (k % n).astype(np.int32))
+    dw = (1 + rng.integers(0, 255, len(dc))).astype(np.float32)
+    ctx.set_option("sssp_pull_div", 1 << 20)
+    try:
+        for m, rowptr, col, wt in [(g.m, g.rowptr, g.col, w), (n, drp, dc, dw)]:
+            G = ctx.upload(rowptr.astype(np.int32), col, wt)
+            for src in T.sources(T.G("x", m, rowptr, col, weights=False), 3):
+                want = to_ref_sentinel(O.sssp(m, rowptr, col, wt, src))
+                assert np.array_equal(ctx.sssp(G, src), want), (m, src)
+                st = ctx.stats()
+                assert st["edges_checked"] == st["edges_traversed"]
+            G.close()
+    finally:
+        ctx.set_option("sssp_pull_div", 0)
+
+
 def test_sssp_near_far_edge_cases(ctx):
     ctx.set_option("sssp_delta", 16)
     try:
