@@ -3,8 +3,10 @@
 // results equal apspie_bfs / apspie_sssp.
 //
 // Layout (vertex-major, HBM): W = ceil(B/32) words of bits per vertex for
-// visited / current-frontier / next-frontier, and for SSSP B contiguous FP32
-// distances per vertex, so one expanded edge touches contiguous words.
+// visited / current-frontier / next-frontier, B contiguous int32 depths (BFS)
+// or FP32 distances (SSSP) per vertex, so one expanded edge touches
+// contiguous words; the per-source [B][m] result is one tiled transpose at
+// the end instead of a scattered write per discovery.
 // A level is one load-balanced expansion over the vertices whose frontier row
 // is non-empty (the same tile machinery as the single-source path):
 //   BFS : new = front[u] & ~visited[v]; fetch-or into visited[v] decides which
@@ -31,37 +33,66 @@ constexpr int kBlock = 256;
 constexpr int kTile = 2048;
 constexpr int kMaxBatch = 512;  // sources per pass (larger batches are split)
 
+// the W words of a vertex's bit row (16-byte loads when W is a multiple of 4;
+// rows of W words are W * 4-byte aligned)
+template <int W>
+__device__ __forceinline__ void load_row(const uint32_t *p, uint32_t (&x)[W]) {
+    if constexpr (W % 4 == 0) {
+        const uint4 *q = reinterpret_cast<const uint4 *>(p);
+#pragma unroll
+        for (int i = 0; i < W / 4; ++i) {
+            const uint4 t = q[i];
+            x[4 * i] = t.x;
+            x[4 * i + 1] = t.y;
+            x[4 * i + 2] = t.z;
+            x[4 * i + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int w = 0; w < W; ++w) x[w] = p[w];
+    }
+}
+
+// Push form of one batched BFS level: the edge u -> v carries the frontier
+// bits of u not yet visited at v.  Every thread that finds such a bit sets it
+// (visited, next) and writes its depth: racing threads of one level write the
+// same values, so the ORs need no return value; one exchange on the vertex's
+// stamp admits v to the next frontier once.
 template <int W>
 struct BfsBatchOp {
     const uint32_t *front;
     uint32_t *vis;
     uint32_t *next;
     int32_t *stamp;
-    int32_t *depth;  // [nsrc][m]
+    int32_t *depth;  // [m][32W], vertex-major (transposed at the end)
     int32_t m;
     const int32_t *lvl;  // device counter: depth of this level's discoveries
     int32_t nsrc;
     __device__ __forceinline__ bool edge(int32_t u, int32_t, int32_t v) const {
+        uint32_t f[W];
+        load_row<W>(front + (int64_t)u * W, f);
+        uint32_t anyf = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) anyf |= f[w];
+        if (!anyf) return false;
+        uint32_t seen[W];
+        load_row<W>(vis + (int64_t)v * W, seen);
         const int32_t level = *lvl;
         bool any = false;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            const uint32_t f = front[(int64_t)u * W + w];
-            if (!f) continue;
-            uint32_t *pv = vis + (int64_t)v * W + w;
-            uint32_t cand = f & ~*pv;
+            uint32_t cand = f[w] & ~seen[w];
             if (!cand) continue;
-            const uint32_t old = dev::atomic_or_dev(pv, cand);
-            uint32_t mine = cand & ~old;
-            if (!mine) continue;
-            __hip_atomic_fetch_or(next + (int64_t)v * W + w, mine, __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_AGENT);
             any = true;
-            while (mine) {
-                const int b = __ffs(mine) - 1;
-                mine &= mine - 1;
+            __hip_atomic_fetch_or(vis + (int64_t)v * W + w, cand, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_or(next + (int64_t)v * W + w, cand, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+            while (cand) {
+                const int b = __ffs(cand) - 1;
+                cand &= cand - 1;
                 const int s = w * 32 + b;
-                if (s < nsrc) depth[(int64_t)s * m + v] = level;
+                if (s < nsrc) depth[(int64_t)v * (32 * W) + s] = level;
             }
         }
         return any && dev::atomic_exch_dev(stamp + v, level) != level;
@@ -152,7 +183,7 @@ __device__ __forceinline__ void batch_pull(PullQueue &q, const int32_t *__restri
                                            const int32_t *__restrict__ col, int32_t m,
                                            const uint32_t *__restrict__ front, uint32_t *vis,
                                            uint32_t *next, int32_t *depth, int32_t level,
-                                           FrontierBufs nxt, uint64_t *cnt_next) {
+                                           uint32_t *snap, FrontierBufs nxt, uint64_t *cnt_next) {
     constexpr int VPB = kBlock / W;  // vertices per block row
     constexpr int KP = 4;            // block rows per pass
     constexpr int Q = 4;             // neighbours loaded at once
@@ -203,15 +234,21 @@ __device__ __forceinline__ void batch_pull(PullQueue &q, const int32_t *__restri
 #pragma unroll
         for (int k = 0; k < KP; ++k) {
             uint32_t nw = acc[k] & uv[k];
+            const int64_t i = (int64_t)cv[k] * W + w;
+            // with a snapshot the level's discoveries are one full bit row per
+            // vertex (the depths are read back from it at the end)
+            if (cv[k] < m) {
+                if (snap) snap[i] = nw;
+                next[i] = nw;  // every row: the buffer needs no clearing before a pull
+            }
             if (nw) {
-                const int64_t i = (int64_t)cv[k] * W + w;
                 vis[i] = ~uv[k] | nw;
-                next[i] = nw;
-                while (nw) {
-                    const int b = __ffs(nw) - 1;
-                    nw &= nw - 1;
-                    depth[(int64_t)(w * 32 + b) * m + cv[k]] = level;
-                }
+                if (!snap)
+                    while (nw) {
+                        const int b = __ffs(nw) - 1;
+                        nw &= nw - 1;
+                        depth[(int64_t)cv[k] * (32 * W) + w * 32 + b] = level;
+                    }
             }
             const uint64_t bm = __ballot((acc[k] & uv[k]) != 0);
             if (w == 0 && (bm & gmask)) {
@@ -242,20 +279,43 @@ __device__ __forceinline__ void batch_pull(PullQueue &q, const int32_t *__restri
     if (q.n > 0) queue_flush(q, nxt, cnt_next);
 }
 
+// Snapshots of pulled levels: the j-th pulled level writes its discovered
+// bits for every vertex to snap + j * stride and its level number to
+// level_of[j], up to cap levels (later pulled levels write depths directly):
+// one full bit row per vertex instead of a scattered 4-byte depth store per
+// discovered (vertex, source) pair.  npull[l & 1] holds the number of
+// snapshots taken before level l (two slots: a level reads one and block 0
+// writes the other).
+struct BatchSnap {
+    uint32_t *snap;
+    int64_t stride;
+    int32_t cap;
+    int32_t *level_of;
+    uint64_t *npull;
+};
+
 // One batched BFS level: push (load-balanced expansion of the union
 // frontier) or pull, chosen identically by every block from the union
 // frontier's edge count: pull when E * pull_div > nnz (pull_div 0: never).
-template <int W>
-__global__ __launch_bounds__(kBlock) void k_batch_bfs_level(
-    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col, int32_t nnz, int32_t m,
-    BfsBatchOp<W> op, uint32_t *next_w, int32_t pull_div, FrontierBufs cur,
-    const uint64_t *cnt_cur, FrontierBufs nxt, uint64_t *cnt_next, uint64_t *cnt_clear,
-    uint64_t *stats) {
-    __shared__ BatchLds lds;
+// The two forms are separate kernels (each level launches both; the one
+// whose form was not chosen returns at once), so the pull kernel is not held
+// to the push expansion's register count.
+template <int W, bool PULL>
+__device__ __forceinline__ void batch_level(BatchLds &lds, const int32_t *__restrict__ rowptr,
+                                            const int32_t *__restrict__ col, int32_t nnz,
+                                            int32_t m, BfsBatchOp<W> &op, uint32_t *next_w,
+                                            int32_t pull_div, FrontierBufs cur,
+                                            const uint64_t *cnt_cur, FrontierBufs nxt,
+                                            uint64_t *cnt_next, uint64_t *cnt_clear,
+                                            uint64_t *stats, BatchSnap sn) {
     const uint64_t cc = *cnt_cur;
     const int32_t F = (int32_t)cnt_entries(cc);
     const int32_t E = (int32_t)cnt_edges(cc);
     const bool pull = pull_div > 0 && (int64_t)E * pull_div > (int64_t)nnz;
+    if (pull != PULL) return;
+    const int32_t level = *op.lvl;
+    const uint64_t j = sn.npull[level & 1];
+    const bool take = pull && (int64_t)j < (int64_t)sn.cap;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *cnt_clear = 0;
         if (F > 0) {
@@ -263,16 +323,39 @@ __global__ __launch_bounds__(kBlock) void k_batch_bfs_level(
             stats[1] += (uint64_t)F;  // union-frontier entries
             stats[2] += 1;            // levels with work
             if (pull) stats[3] += 1;  // of which pulled
+            sn.npull[(level + 1) & 1] = j + (take ? 1 : 0);
+            if (take) sn.level_of[j] = level;
         }
     }
     if (F == 0) return;
-    if (pull) {
-        batch_pull<W>(lds.pq, rowptr, col, m, op.front, op.vis, next_w, op.depth, *op.lvl, nxt,
-                      cnt_next);
-        return;
-    }
-    expand_tiles<kBlock, kTile, true, true, true>(lds.ex, op, rowptr, col, nnz, cur, F, E, nxt,
-                                                   cnt_next);
+    if constexpr (PULL)
+        batch_pull<W>(lds.pq, rowptr, col, m, op.front, op.vis, next_w, op.depth, level,
+                      take ? sn.snap + (int64_t)j * sn.stride : nullptr, nxt, cnt_next);
+    else
+        expand_tiles<kBlock, kTile, true, true, true>(lds.ex, op, rowptr, col, nnz, cur, F, E,
+                                                       nxt, cnt_next);
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_batch_bfs_push(
+    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col, int32_t nnz, int32_t m,
+    BfsBatchOp<W> op, uint32_t *next_w, int32_t pull_div, FrontierBufs cur,
+    const uint64_t *cnt_cur, FrontierBufs nxt, uint64_t *cnt_next, uint64_t *cnt_clear,
+    uint64_t *stats, BatchSnap sn) {
+    __shared__ BatchLds lds;
+    batch_level<W, false>(lds, rowptr, col, nnz, m, op, next_w, pull_div, cur, cnt_cur, nxt,
+                          cnt_next, cnt_clear, stats, sn);
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_batch_bfs_pull(
+    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col, int32_t nnz, int32_t m,
+    BfsBatchOp<W> op, uint32_t *next_w, int32_t pull_div, FrontierBufs cur,
+    const uint64_t *cnt_cur, FrontierBufs nxt, uint64_t *cnt_next, uint64_t *cnt_clear,
+    uint64_t *stats, BatchSnap sn) {
+    __shared__ BatchLds lds;
+    batch_level<W, true>(lds, rowptr, col, nnz, m, op, next_w, pull_div, cur, cnt_cur, nxt,
+                         cnt_next, cnt_clear, stats, sn);
 }
 
 // Near-far rounds for the batch (the single-source scheme with one threshold
@@ -460,12 +543,16 @@ __global__ void k_batch_vis_init(const int32_t *__restrict__ rowptr, int32_t m, 
 
 // zero the frontier rows of the entries just expanded; advance the level
 // (every round when `every`: near-far rounds number their claims by round,
-// split rounds included)
+// split rounds included).  The rows are left alone when the next level pulls
+// (pull_div > 0 and its union frontier has more than nnz / pull_div edges):
+// that level writes every row of this buffer as its next frontier.
 template <int W>
 __global__ void k_clear_rows(FrontierBufs cur, const uint64_t *cnt, uint32_t *front,
-                             int32_t *lvl, bool every) {
+                             int32_t *lvl, bool every, const uint64_t *cnt_next,
+                             int32_t pull_div, int32_t nnz) {
     const int32_t F = (int32_t)cnt_entries(*cnt);
     if (blockIdx.x == 0 && threadIdx.x == 0 && (F > 0 || every)) *lvl += 1;
+    if (pull_div > 0 && (int64_t)cnt_edges(*cnt_next) * pull_div > (int64_t)nnz) return;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)F * W;
          i += (int64_t)gridDim.x * blockDim.x)
         front[(int64_t)cur.v[i / W] * W + (i % W)] = 0;
@@ -479,7 +566,7 @@ __global__ void k_batch_seed(const int32_t *srcs, int32_t nsrc, uint32_t *vis, u
         const int32_t s = srcs[i];
         if (vis) atomicOr(vis + (int64_t)s * W + i / 32, 1u << (i % 32));
         atomicOr(front + (int64_t)s * W + i / 32, 1u << (i % 32));
-        if (depth) depth[(int64_t)i * m + s] = 0;
+        if (depth) depth[(int64_t)s * (32 * W) + i] = 0;
         if (dist) dist[(int64_t)s * (32 * W) + i] = 0;
     }
 }
@@ -505,9 +592,63 @@ __global__ void k_transpose_dist(const int32_t *dist, int32_t m, int32_t B, int3
     }
 }
 
+__global__ void k_batch_npull(uint64_t *npull, const int32_t *lvl) { npull[0] = npull[*lvl & 1]; }
+
+// Depths of one pass into out [nsrc][m], T vertices per block through LDS
+// (T * W = 256 words, one per thread): pairs discovered in a snapshotted
+// level take that level's number; the others are reached (visited bit set)
+// with the depth written where they were discovered (vertex-major dep
+// [m][32W]), or unreached (-1).  With iso set, isolated vertices were marked
+// visited up front, so there only the source itself counts as reached.  The
+// snapshots are read kSnapRun at a time (independent loads in flight).
+constexpr int kSnapRun = 16;
+template <int W>
+__global__ __launch_bounds__(256) void k_batch_depths(
+    const int32_t *dep, const uint32_t *vis, const int32_t *rowptr, const int32_t *srcs, bool iso,
+    int32_t m, int32_t nsrc, const uint32_t *snap, int64_t stride, const uint64_t *npull,
+    const int32_t *level_of, int32_t *out) {
+    constexpr int B = 32 * W, T = 256 / W;
+    __shared__ int32_t d[T][B + 1];
+    const int64_t v0 = (int64_t)blockIdx.x * T;
+    const int t = threadIdx.x / W, w = threadIdx.x % W;
+    const int64_t v = v0 + t;
+    for (int b = 0; b < 32; ++b) d[t][w * 32 + b] = -2;
+    const int32_t ns = (int32_t)npull[0];
+    for (int32_t j0 = 0; j0 < ns; j0 += kSnapRun) {
+        uint32_t bits[kSnapRun];
+#pragma unroll
+        for (int q = 0; q < kSnapRun; ++q)
+            bits[q] = (j0 + q < ns && v < m) ? snap[(int64_t)(j0 + q) * stride + v * W + w] : 0u;
+#pragma unroll
+        for (int q = 0; q < kSnapRun; ++q) {
+            if (!bits[q]) continue;
+            const int32_t L = level_of[j0 + q];
+            while (bits[q]) {
+                const int b = __ffs(bits[q]) - 1;
+                bits[q] &= bits[q] - 1;
+                d[t][w * 32 + b] = L;
+            }
+        }
+    }
+    __syncthreads();
+    for (int x = threadIdx.x; x < T * nsrc; x += 256) {
+        const int s = x / T, tt = x % T;
+        const int64_t vv = v0 + tt;
+        if (vv >= m) continue;
+        int32_t r = d[tt][s];
+        if (r == -2) {
+            r = -1;
+            if ((vis[vv * W + s / 32] >> (s % 32)) & 1u)
+                if (!iso || rowptr[vv + 1] > rowptr[vv] || srcs[s] == (int32_t)vv)
+                    r = dep[vv * B + s];
+        }
+        out[(int64_t)s * m + vv] = r;
+    }
+}
+
 // One batched traversal pass for up to 32*W sources.
 template <int W, bool SSSP>
-void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, int32_t *depth,
+void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, int32_t *depth_out,
                 float *dist_out) {
     constexpr int B = 32 * W;
     const int32_t m = g->m;
@@ -519,6 +660,22 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
     int32_t *stamp = (int32_t *)scratch(c, "batch_stamp", sizeof(int32_t) * m);
     int32_t *dist = SSSP ? (int32_t *)scratch(c, "batch_dist", sizeof(int32_t) * m * (int64_t)B)
                          : nullptr;
+    int32_t *depth = SSSP ? nullptr
+                          : (int32_t *)scratch(c, "batch_dep", sizeof(int32_t) * m * (int64_t)B);
+    // snapshots of pulled BFS levels, within batch_snapshot_mb (and half the
+    // context's remaining HBM budget, if it has one)
+    BatchSnap sn{};
+    if (!SSSP) {
+        int64_t bytes = opt(c, "batch_snapshot_mb", 24576) << 20;
+        if (c->budget > 0) bytes = std::min<int64_t>(bytes, std::max<int64_t>(0, (c->budget - c->used) / 2));
+        sn.stride = words;
+        sn.cap = (int32_t)std::min<int64_t>(bytes / (int64_t)(sizeof(uint32_t) * words), 4096);
+        sn.snap = sn.cap > 0 ? (uint32_t *)scratch(c, "batch_snap",
+                                                   sizeof(uint32_t) * words * (int64_t)sn.cap)
+                             : nullptr;
+        sn.level_of = (int32_t *)scratch(c, "batch_snap_lv", sizeof(int32_t) * 4096);
+        sn.npull = fs.ctl + 12;
+    }
     int32_t *srcs = (int32_t *)scratch(c, "batch_srcs", sizeof(int32_t) * (B + 1));
     APSPIE_HIP(hipMemcpyAsync(srcs, src_host, sizeof(int32_t) * nsrc, hipMemcpyHostToDevice,
                               c->stream));
@@ -540,7 +697,6 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
     const int32_t pull_div = sym ? (int32_t)opt(c, "batch_pull_div", 4) : 0;
     if (!SSSP) {
         k_batch_vis_init<W><<<grid, kBlock, 0, c->stream>>>(g->rowptr, m, nsrc, sym, vis);
-        fill_i32(c, depth, -1, (int64_t)nsrc * m);
     } else {
         fill_i32(c, dist, __builtin_bit_cast(int32_t, APSPIE_SSSP_INF), (int64_t)m * B);
     }
@@ -571,14 +727,21 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
                 SsspBatchOp<W>{front, next, stamp, dist, g->val, lvl}, g->rowptr, g->col,
                 (int32_t)g->nnz, fs.f[cur], fs.ctl + (l % 3), fs.f[cur ^ 1],
                 fs.ctl + ((l + 1) % 3), fs.ctl + ((l + 2) % 3), fs.ctl + 4);
-        else
-            k_batch_bfs_level<W><<<grid, kBlock, 0, c->stream>>>(
-                g->rowptr, g->col, (int32_t)g->nnz, m,
-                BfsBatchOp<W>{front, vis, next, stamp, depth, m, lvl, nsrc}, next, pull_div,
-                fs.f[cur], fs.ctl + (l % 3), fs.f[cur ^ 1], fs.ctl + ((l + 1) % 3),
-                fs.ctl + ((l + 2) % 3), fs.ctl + 4);
+        else {
+            const BfsBatchOp<W> op{front, vis, next, stamp, depth, m, lvl, nsrc};
+            k_batch_bfs_push<W><<<grid, kBlock, 0, c->stream>>>(
+                g->rowptr, g->col, (int32_t)g->nnz, m, op, next, pull_div, fs.f[cur],
+                fs.ctl + (l % 3), fs.f[cur ^ 1], fs.ctl + ((l + 1) % 3), fs.ctl + ((l + 2) % 3),
+                fs.ctl + 4, sn);
+            if (pull_div > 0)  // six 256-thread blocks per CU fit its registers and LDS
+                k_batch_bfs_pull<W><<<fill_grid(c, kBlock, 6), kBlock, 0, c->stream>>>(
+                    g->rowptr, g->col, (int32_t)g->nnz, m, op, next, pull_div, fs.f[cur],
+                    fs.ctl + (l % 3), fs.f[cur ^ 1], fs.ctl + ((l + 1) % 3),
+                    fs.ctl + ((l + 2) % 3), fs.ctl + 4, sn);
+        }
         k_clear_rows<W><<<grid, kBlock, 0, c->stream>>>(fs.f[cur], fs.ctl + (l % 3), front, lvl,
-                                                        near_far);
+                                                        near_far, fs.ctl + ((l + 1) % 3),
+                                                        SSSP ? 0 : pull_div, (int32_t)g->nnz);
     };
 
     // chunks of K levels captured once and replayed: K is a multiple of 6 so
@@ -633,8 +796,15 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
     if (SSSP) {
         dim3 grd((unsigned)ceil_div(m, 32), (unsigned)ceil_div(nsrc, 32));
         k_transpose_dist<<<grd, 256, 0, c->stream>>>(dist, m, B, nsrc, dist_out);
-        APSPIE_LAUNCH_CHECK();
+    } else {
+        // the snapshot count after the last level sits in the slot of the level
+        // that found the frontier empty; copy it to slot 0 for the reader
+        k_batch_npull<<<1, 1, 0, c->stream>>>(sn.npull, lvl);
+        k_batch_depths<W><<<(unsigned)ceil_div(m, 256 / W), 256, 0, c->stream>>>(
+            depth, vis, g->rowptr, srcs, sym, m, nsrc, sn.snap, sn.stride, sn.npull, sn.level_of,
+            depth_out);
     }
+    APSPIE_LAUNCH_CHECK();
 }
 
 template <bool SSSP>

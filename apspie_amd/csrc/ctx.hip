@@ -25,6 +25,7 @@ static const char *kOptionNames[] = {
     "batch_graph",       // 1 = replay batched level chunks from a captured hipGraph
     "batch_pull_div",    // batched BFS pulls a level when union-frontier edges * this > nnz (0 = never)
     "batch_sssp_delta",  // batched SSSP near-far threshold step (0 = Bellman-Ford frontier)
+    "batch_snapshot_mb", // HBM for the bit rows of pulled batched BFS levels (0 = none)
     "profiling",         // 1 = time the dominant kernel with hipEvents
     nullptr};
 
@@ -45,6 +46,7 @@ static int64_t option_default(const std::string &n) {
     if (n == "batch_graph") return 1;
     if (n == "batch_pull_div") return 4;
     if (n == "batch_sssp_delta") return 256;
+    if (n == "batch_snapshot_mb") return 24576;
     if (n == "profiling") return 0;
     return -1;
 }
@@ -264,6 +266,7 @@ int apspie_ctx_set_option(apspie_ctx *c, const char *name, int64_t v) {
         return APSPIE_ERR_INVALID;
     if (n == "bfs_levels_per_sync" && (v < 1 || v > 4096)) return APSPIE_ERR_INVALID;
     if (n == "batch_pull_div" && (v < 0 || v > (1 << 20))) return APSPIE_ERR_INVALID;
+    if (n == "batch_snapshot_mb" && (v < 0 || v > (1 << 22))) return APSPIE_ERR_INVALID;
     if ((n == "batch_sssp_delta" || n == "sssp_delta") && (v < 0 || v > (1 << 30)))
         return APSPIE_ERR_INVALID;
     if (n == "sssp_pull_div" && (v < 0 || v > (1 << 20))) return APSPIE_ERR_INVALID;

@@ -213,14 +213,27 @@ __device__ __forceinline__ void for_products(int32_t r, const int32_t *arp, cons
         if (threadIdx.x == 0) pre[cn] = (int32_t)carry;
         __syncthreads();
         const int64_t W = carry;
+        // kRun consecutive products per lane: the loads of B go out together,
+        // then the products are applied
         for (int64_t x0 = (int64_t)threadIdx.x * kRun; x0 < W; x0 += (int64_t)BLOCK * kRun) {
             int j = dev::upper_bound_m1(pre, 0, cn, (int32_t)x0);
-            const int64_t xe = min<int64_t>(x0 + kRun, W);
-            for (int64_t x = x0; x < xe; ++x) {
-                while (pre[j + 1] <= x) ++j;
-                const int64_t e = (int64_t)rs[j] + (x - pre[j]);
-                f(bcol[e], av[j] * (bval ? bval[e] : 1.0f));
+            int32_t key[kRun];
+            float val[kRun];
+#pragma unroll
+            for (int q = 0; q < kRun; ++q) {
+                const int64_t x = x0 + q;
+                key[q] = -1;
+                val[q] = 0.0f;
+                if (x < W) {
+                    while (pre[j + 1] <= x) ++j;
+                    const int64_t e = (int64_t)rs[j] + (x - pre[j]);
+                    key[q] = bcol[e];
+                    val[q] = av[j] * (bval ? bval[e] : 1.0f);
+                }
             }
+#pragma unroll
+            for (int q = 0; q < kRun; ++q)
+                if (key[q] >= 0) f(key[q], val[q]);
         }
         __syncthreads();  // pre / rs / av are refilled by the next chunk
     }

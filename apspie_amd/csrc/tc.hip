@@ -32,6 +32,7 @@ constexpr int kSmallRow = 128;     // wave bin: |L_r| <= 128, table 256
 constexpr int kSmallTable = 256;
 constexpr int kMediumRow = 2048;   // block bin: |L_r| <= 2048, LDS table 4096
 constexpr int kMediumTable = 4096;
+constexpr int kMediumChunk = 1024;  // staging of the medium tier (LDS: 5 blocks per CU)
 constexpr int kBigRow = 16384;     // 1024-thread block, LDS table 32768 (128 KB)
 constexpr int kBigTable = 32768;
 constexpr int kBigBlock = 1024;    // also for the longest rows (table in global memory)
@@ -169,11 +170,23 @@ __global__ __launch_bounds__(kBlock) void k_tc_small(const int32_t *rowptr, cons
         dev::wave_sync();
         const int32_t W = (int32_t)carry;
         uint32_t c = 0;
-        for (int32_t x = lane; x < W; x += 64) {
-            const int j = dev::upper_bound_m1(s_pre[w], 0, n, x);
-            const int32_t e = s_rs[w][j] + (x - s_pre[w][j]);
-            if ((uint32_t)e >= (uint32_t)nnz) continue;  // guard only
-            c += table_has(s_table[w], kSmallTable - 1, col[e]) ? 1u : 0u;
+        // kRun consecutive wedges per lane, their column loads issued together
+        for (int32_t x0 = lane * kRun; x0 < W; x0 += 64 * kRun) {
+            int j = dev::upper_bound_m1(s_pre[w], 0, n, x0);
+            int32_t key[kRun];
+#pragma unroll
+            for (int q = 0; q < kRun; ++q) {
+                const int32_t x = x0 + q;
+                key[q] = -1;
+                if (x < W) {
+                    while (s_pre[w][j + 1] <= x) ++j;
+                    const int32_t e = s_rs[w][j] + (x - s_pre[w][j]);
+                    if ((uint32_t)e < (uint32_t)nnz) key[q] = col[e];  // guard only
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kRun; ++q)
+                if (key[q] >= 0) c += table_has(s_table[w], kSmallTable - 1, key[q]) ? 1u : 0u;
         }
         const uint64_t rc = dev::wave_sum_u64(c);
         if (lane == 0) {
@@ -193,31 +206,70 @@ __global__ __launch_bounds__(kBlock) void k_tc_small(const int32_t *rowptr, cons
 // The wedges of one chunk of entries j in [0, cn): entry j owns the wedges
 // [pre[j], pre[j + 1]) starting at CSR position rsk[j]; pre[cn] = W.  A lane
 // takes kRun consecutive wedges, finds the first one's entry by binary search
-// and walks on from there; f(j, e) is called for every wedge.
+// and walks on from there, issuing the kRun column loads back to back before
+// probing; f(j, key) is called for every wedge with the column it reaches.
 template <int BLOCK, class F>
-__device__ __forceinline__ void for_wedges(const int32_t *pre, const int32_t *rsk, int32_t cn,
-                                           int32_t W, int32_t nnz, F &&f) {
+__device__ __forceinline__ void for_wedges(const int32_t *col, const int32_t *pre,
+                                           const int32_t *rsk, int32_t cn, int32_t W,
+                                           int32_t nnz, F &&f) {
     for (int64_t x0 = (int64_t)threadIdx.x * kRun; x0 < W; x0 += (int64_t)BLOCK * kRun) {
         int j = dev::upper_bound_m1(pre, 0, cn, (int32_t)x0);
-        const int32_t xe = (int32_t)min<int64_t>(x0 + kRun, W);
-        for (int32_t x = (int32_t)x0; x < xe; ++x) {
-            while (pre[j + 1] <= x) ++j;
-            const int32_t e = rsk[j] + (x - pre[j]);
-            if ((uint32_t)e < (uint32_t)nnz) f(j, e);  // guard only
+        int32_t key[kRun], jj[kRun];
+#pragma unroll
+        for (int q = 0; q < kRun; ++q) {
+            const int32_t x = (int32_t)x0 + q;
+            key[q] = -1;
+            jj[q] = j;
+            if (x < W) {
+                while (pre[j + 1] <= x) ++j;
+                const int32_t e = rsk[j] + (x - pre[j]);
+                jj[q] = j;
+                if ((uint32_t)e < (uint32_t)nnz) key[q] = col[e];  // guard only
+            }
         }
+#pragma unroll
+        for (int q = 0; q < kRun; ++q)
+            if (key[q] >= 0) f(jj[q], key[q]);
     }
 }
 
-// one block per row of the block bins: the rows of list a (cnt_a[0] of them,
-// taken first) then of list b.  A row with |L_r| <= TABLE / 2 keeps its table
-// in LDS; a longer one in the block's region of global memory (g_cap slots).
-// The scanned lengths are staged in LDS, CHUNK entries at a time, for both
-// pair kinds.
+// Work items of the block bins: a row's own pairs in chunks of CHUNK entries
+// of L_r, then its upper pairs in chunks of CHUNK entries of U_r; item =
+// (row, chunk index), so the chunks of one long row spread over many blocks
+// instead of queueing behind each other in one.  Rows of list a come first.
+template <int CHUNK>
+__global__ void k_tc_items(const int32_t *rowptr, const int32_t *lowcnt, const int32_t *rows_a,
+                           const uint32_t *cnt_a, const int32_t *rows_b, const uint32_t *cnt_b,
+                           int2 *items, uint32_t *nitems) {
+    const int32_t na = (int32_t)cnt_a[0], nr = na + (cnt_b ? (int32_t)cnt_b[0] : 0);
+    const int lane = dev::lane_id();
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < nr;
+         i0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = i0 + threadIdx.x;
+        uint32_t k = 0;
+        int32_t r = 0;
+        if (i < nr) {
+            r = i < na ? rows_a[i] : rows_b[i - na];
+            const int32_t n = lowcnt[r], u = rowptr[r + 1] - rowptr[r] - n;
+            k = (uint32_t)((n + CHUNK - 1) / CHUNK + (u + CHUNK - 1) / CHUNK);
+        }
+        const uint32_t incl = dev::wave_inclusive_add(k);
+        uint32_t base = 0;
+        if (lane == 63 && incl) base = dev::atomic_add_dev(nitems, incl);
+        base = dev::wave_read_lane(base, 63) + incl - k;
+        for (uint32_t q = 0; q < k; ++q) items[base + q] = make_int2(r, (int32_t)q);
+    }
+}
+
+// One block per work item of the block bins (see k_tc_items); consecutive
+// items of one row in a block keep the row's table.  A row with
+// |L_r| <= TABLE / 2 keeps its table in LDS; a longer one in the block's
+// region of global memory (g_cap slots).  The scanned lengths are staged in
+// LDS, CHUNK entries at a time, for both pair kinds.
 template <int TABLE, int BLOCK, int CHUNK>
 __global__ __launch_bounds__(BLOCK) void k_tc_block(const int32_t *rowptr, const int32_t *col,
                                                     int32_t nnz, const int32_t *lowcnt,
-                                                    const int32_t *rows_a, const uint32_t *cnt_a,
-                                                    const int32_t *rows_b, const uint32_t *cnt_b,
+                                                    const int2 *items, const uint32_t *nitems,
                                                     int32_t *per_row, uint64_t *total,
                                                     uint64_t *wedges, int32_t *g_table,
                                                     int64_t g_cap) {
@@ -227,12 +279,15 @@ __global__ __launch_bounds__(BLOCK) void k_tc_block(const int32_t *rowptr, const
     __shared__ int32_t cntu[CHUNK];
     __shared__ uint32_t s_scan[BLOCK / 64 + 1];
     __shared__ uint64_t s_red[BLOCK / 64];
-    const int32_t na = (int32_t)cnt_a[0], nrows = na + (cnt_b ? (int32_t)cnt_b[0] : 0);
+    const uint32_t nit = *nitems;
     uint64_t my_tri = 0, my_wedges = 0;
-    for (int32_t ri = blockIdx.x; ri < nrows; ri += gridDim.x) {
-        const int32_t r = ri < na ? rows_a[ri] : rows_b[ri - na];
+    int32_t table_row = -1;
+    for (uint32_t it = blockIdx.x; it < nit; it += gridDim.x) {
+        const int2 item = items[it];
+        const int32_t r = item.x;
         const int32_t n = lowcnt[r];
         const int32_t base = rowptr[r], rend = rowptr[r + 1];
+        const int32_t own_chunks = (n + CHUNK - 1) / CHUNK;
         const bool lds = 2 * n <= TABLE;
         int32_t *table = lds ? s_table : g_table + (int64_t)blockIdx.x * g_cap;
         uint32_t tsize = TABLE;
@@ -241,14 +296,18 @@ __global__ __launch_bounds__(BLOCK) void k_tc_block(const int32_t *rowptr, const
             while (tsize < 2u * (uint32_t)n) tsize <<= 1;
         }
         const uint32_t mask = tsize - 1;
-        for (uint32_t s = threadIdx.x; s < tsize; s += BLOCK) table[s] = -1;
-        __syncthreads();
-        for (int32_t idx = threadIdx.x; idx < n; idx += BLOCK)
-            table_insert(table, mask, col[base + idx]);
-        __syncthreads();
-        // own pairs: k in L_r with |L_k| <= |L_r|, CHUNK entries of L_r at a time
-        uint32_t c = 0;
-        for (int32_t k0 = 0; k0 < n; k0 += CHUNK) {
+        if (r != table_row) {  // uniform: every thread read the same item
+            for (uint32_t s = threadIdx.x; s < tsize; s += BLOCK) table[s] = -1;
+            __syncthreads();
+            for (int32_t idx = threadIdx.x; idx < n; idx += BLOCK)
+                table_insert(table, mask, col[base + idx]);
+            __syncthreads();
+            table_row = r;
+        }
+        if (item.y < own_chunks) {
+            // own pairs: k in L_r with |L_k| <= |L_r|, entries [k0, k0 + CHUNK) of L_r
+            uint32_t c = 0;
+            const int32_t k0 = item.y * CHUNK;
             const int32_t cn = min(CHUNK, n - k0);
             uint32_t carry = 0;
             for (int c0 = 0; c0 < cn; c0 += BLOCK) {
@@ -267,24 +326,23 @@ __global__ __launch_bounds__(BLOCK) void k_tc_block(const int32_t *rowptr, const
             }
             if (threadIdx.x == 0) pre[cn] = (int32_t)carry;
             __syncthreads();
-            for_wedges<BLOCK>(pre, rsk, cn, (int32_t)carry, nnz, [&](int, int32_t e) {
-                c += table_has(table, mask, col[e]) ? 1u : 0u;
+            for_wedges<BLOCK>(col, pre, rsk, cn, (int32_t)carry, nnz, [&](int, int32_t key) {
+                c += table_has(table, mask, key) ? 1u : 0u;
             });
             if (threadIdx.x == 0) my_wedges += carry;
-            __syncthreads();  // pre / rsk are refilled by the next chunk
-        }
-        const uint64_t wc = dev::wave_sum_u64(c);
-        if (dev::lane_id() == 0) s_red[threadIdx.x / 64] = wc;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint64_t rc = 0;
-            for (int q = 0; q < BLOCK / 64; ++q) rc += s_red[q];
-            if (rc && per_row) atomicAdd(per_row + r, (int32_t)rc);
-            my_tri += rc;
-        }
-        __syncthreads();
-        // upper pairs: i in U_r with |L_i| < |L_r|, CHUNK entries of U_r at a time
-        for (int32_t u0 = base + n; u0 < rend; u0 += CHUNK) {
+            const uint64_t wc = dev::wave_sum_u64(c);
+            if (dev::lane_id() == 0) s_red[threadIdx.x / 64] = wc;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint64_t rc = 0;
+                for (int q = 0; q < BLOCK / 64; ++q) rc += s_red[q];
+                if (rc && per_row) atomicAdd(per_row + r, (int32_t)rc);
+                my_tri += rc;
+            }
+            __syncthreads();  // pre / rsk / s_red are refilled by the next item
+        } else {
+            // upper pairs: i in U_r with |L_i| < |L_r|, entries [u0, u0 + CHUNK) of U_r
+            const int32_t u0 = base + n + (item.y - own_chunks) * CHUNK;
             const int32_t cn = min(CHUNK, rend - u0);
             uint32_t carry = 0;
             for (int c0 = 0; c0 < cn; c0 += BLOCK) {
@@ -305,8 +363,8 @@ __global__ __launch_bounds__(BLOCK) void k_tc_block(const int32_t *rowptr, const
             if (threadIdx.x == 0) pre[cn] = (int32_t)carry;
             __syncthreads();
             if (carry > 0) {  // uniform: most chunks of a hub's upper part scan nothing
-                for_wedges<BLOCK>(pre, rsk, cn, (int32_t)carry, nnz, [&](int j, int32_t e) {
-                    if (table_has(table, mask, col[e])) atomicAdd(cntu + j, 1);
+                for_wedges<BLOCK>(col, pre, rsk, cn, (int32_t)carry, nnz, [&](int j, int32_t key) {
+                    if (table_has(table, mask, key)) atomicAdd(cntu + j, 1);
                 });
                 __syncthreads();
                 uint64_t cu = 0;
@@ -384,22 +442,40 @@ void tc_run(Ctx *c, const Graph *g0, uint64_t *total_out, int32_t *per_row) {
     c->stats.host_syncs++;
     KernelTimer kt(c);
     kt.begin();
-    if (h[2] + h[3]) {  // longest rows first: they set the tail
-        int64_t cap = 1;
-        while (cap < 2 * (int64_t)h[4]) cap <<= 1;
-        const int blocks = (int)std::min<int64_t>(h[2] + h[3], c->cus);
-        int32_t *gt = h[3] ? (int32_t *)scratch(c, "tc_gtable", sizeof(int32_t) * cap * blocks)
-                           : nullptr;
-        k_tc_block<kBigTable, kBigBlock, kBigChunk><<<blocks, kBigBlock, 0, c->stream>>>(
-            g->rowptr, g->col, (int32_t)g->nnz, g->lowcnt, lists + 3 * (size_t)m, ctl + 3,
-            lists + 2 * (size_t)m, ctl + 2, per, acc, acc + 1, gt, cap);
+    // block-bin work items: ctl[5] counts those of the long rows (huge, then
+    // big), ctl[6] those of the medium rows
+    const int64_t nbig = (int64_t)h[2] + h[3];
+    int2 *items_big = nullptr, *items_med = nullptr;
+    if (nbig) {
+        items_big = (int2 *)scratch(c, "tc_items_big",
+                                    sizeof(int2) * (g->nnz / kBigChunk + 2 * nbig + 64));
+        k_tc_items<kBigChunk><<<grid, kBlock, 0, c->stream>>>(
+            g->rowptr, g->lowcnt, lists + 3 * (size_t)m, ctl + 3, lists + 2 * (size_t)m, ctl + 2,
+            items_big, ctl + 5);
         APSPIE_LAUNCH_CHECK();
     }
     if (h[1]) {
-        const int blocks = (int)std::min<int64_t>(h[1], 4 * c->cus);
-        k_tc_block<kMediumTable, kBlock, kMediumRow><<<blocks, kBlock, 0, c->stream>>>(
-            g->rowptr, g->col, (int32_t)g->nnz, g->lowcnt, lists + m, ctl + 1, nullptr, nullptr,
-            per, acc, acc + 1, nullptr, 0);
+        items_med = (int2 *)scratch(c, "tc_items_med",
+                                    sizeof(int2) * (g->nnz / kMediumChunk + 2 * (int64_t)h[1] + 64));
+        k_tc_items<kMediumChunk><<<grid, kBlock, 0, c->stream>>>(
+            g->rowptr, g->lowcnt, lists + m, ctl + 1, nullptr, nullptr, items_med, ctl + 6);
+        APSPIE_LAUNCH_CHECK();
+    }
+    if (nbig) {  // longest rows first: they set the tail
+        int64_t cap = 1;
+        while (cap < 2 * (int64_t)h[4]) cap <<= 1;
+        const int blocks = c->cus;
+        int32_t *gt = h[3] ? (int32_t *)scratch(c, "tc_gtable", sizeof(int32_t) * cap * blocks)
+                           : nullptr;
+        k_tc_block<kBigTable, kBigBlock, kBigChunk><<<blocks, kBigBlock, 0, c->stream>>>(
+            g->rowptr, g->col, (int32_t)g->nnz, g->lowcnt, items_big, ctl + 5, per, acc, acc + 1,
+            gt, cap);
+        APSPIE_LAUNCH_CHECK();
+    }
+    if (h[1]) {
+        k_tc_block<kMediumTable, kBlock, kMediumChunk><<<5 * c->cus, kBlock, 0, c->stream>>>(
+            g->rowptr, g->col, (int32_t)g->nnz, g->lowcnt, items_med, ctl + 6, per, acc, acc + 1,
+            nullptr, 0);
         APSPIE_LAUNCH_CHECK();
     }
     if (h[0]) {

@@ -58,6 +58,28 @@ def test_bfs_batch_directions(ctx, key, nsrc, pull_div):
         ctx.set_option("batch_pull_div", 4)
 
 
+@pytest.mark.parametrize("snap_mb", [0, 1, 24576])
+@pytest.mark.parametrize("key", ["rmat12", "lattice64"])
+def test_bfs_batch_level_snapshots(ctx, key, snap_mb):
+    """Pulled levels record their discoveries as bit-row snapshots within
+    batch_snapshot_mb: none (depths written directly), a few (1 MB: the
+    later pulled levels fall back to direct writes) or all."""
+    g = T.rmat(12) if key == "rmat12" else T.lattice(64)
+    ctx.set_option("batch_snapshot_mb", snap_mb)
+    ctx.set_option("batch_pull_div", 1 << 20)
+    try:
+        G = ctx.upload(g.rowptr32, g.col)
+        for nsrc in (7, 100, 300):
+            src = T.sources(g, nsrc, seed=41)
+            got = ctx.bfs_batch(G, src)
+            for i, s in enumerate(src):
+                assert np.array_equal(got[i], O.bfs(g.m, g.rowptr, g.col, s)[0]), (key, i, s)
+        G.close()
+    finally:
+        ctx.set_option("batch_snapshot_mb", 24576)
+        ctx.set_option("batch_pull_div", 4)
+
+
 def test_bfs_batch_directed_graph_pushes(ctx):
     """A non-symmetric CSR is never pulled (rows are out-edges only)."""
     rng = np.random.default_rng(9)
