@@ -99,42 +99,6 @@ struct BfsBatchOp {
     }
 };
 
-template <int W>
-struct SsspBatchOp {
-    const uint32_t *front;
-    uint32_t *next;
-    int32_t *stamp;
-    int32_t *dist;  // [m][32W] float bits
-    const float *wt;
-    const int32_t *lvl;  // device round counter
-    __device__ __forceinline__ bool edge(int32_t u, int32_t e, int32_t v) const {
-        const int32_t round = *lvl;
-        constexpr int B = 32 * W;
-        const float we = wt[e];
-        bool any = false;
-#pragma unroll
-        for (int w = 0; w < W; ++w) {
-            uint32_t f = front[(int64_t)u * W + w];
-            uint32_t improved = 0;
-            while (f) {
-                const int b = __ffs(f) - 1;
-                f &= f - 1;
-                const int s = w * 32 + b;
-                const int32_t nd = dev::fbits(dev::bitsf(dist[(int64_t)u * B + s]) + we);
-                int32_t *pv = dist + (int64_t)v * B + s;
-                if (nd >= *pv) continue;
-                if (nd < dev::atomic_min_dev(pv, nd)) improved |= 1u << b;
-            }
-            if (improved) {
-                __hip_atomic_fetch_or(next + (int64_t)v * W + w, improved, __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT);
-                any = true;
-            }
-        }
-        return any && dev::atomic_exch_dev(stamp + v, round) != round;
-    }
-};
-
 // Claims of a pull level are collected in an LDS queue and appended to the
 // next frontier kQueue at a time: one 64-bit reservation per flush instead of
 // one per pass (a pass covers only 256 / W * 4 vertices, and thousands of
@@ -371,45 +335,94 @@ template <int W>
 struct SsspBatchNearOp {
     const uint32_t *front;
     uint32_t *next;
-    uint32_t *pend;
+    uint32_t *pend;  // null: no threshold (every improvement is near)
     int32_t *stamp;
     int32_t *dist;  // [m][32W] float bits
     const float *wt;
     const int32_t *lvl;  // device round counter
     int32_t tbits;
+    // the edge u -> v (e = CSR position): relax every source of front[u]
+    // (the row read at once), four sources at a time with their distance
+    // loads in flight.  An improvement is judged against the distance read,
+    // and the atomic min is applied without waiting for it: a racing lower
+    // write can leave a spurious frontier bit, which costs a re-expansion,
+    // never a wrong result.
     __device__ __forceinline__ bool edge(int32_t u, int32_t e, int32_t v) const {
-        const int32_t round = *lvl;
         constexpr int B = 32 * W;
+        uint32_t fr[W];
+        load_row<W>(front + (int64_t)u * W, fr);
+        uint32_t anyf = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) anyf |= fr[w];
+        if (!anyf) return false;
         const float we = wt[e];
         bool any = false;
-#pragma unroll
         for (int w = 0; w < W; ++w) {
-            uint32_t f = front[(int64_t)u * W + w];
+            uint32_t f = fr[w];
+            if (!f) continue;
+            const int32_t *du_row = dist + (int64_t)u * B + w * 32;
+            int32_t *dv_row = dist + (int64_t)v * B + w * 32;
             uint32_t near = 0, far = 0;
             while (f) {
-                const int b = __ffs(f) - 1;
-                f &= f - 1;
-                const int s = w * 32 + b;
-                const int32_t nd = dev::fbits(dev::bitsf(dist[(int64_t)u * B + s]) + we);
-                int32_t *pv = dist + (int64_t)v * B + s;
-                if (nd >= *pv) continue;
-                if (nd < dev::atomic_min_dev(pv, nd)) {
-                    if (nd < tbits) near |= 1u << b;
-                    else far |= 1u << b;
+                int b[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    b[q] = f ? __ffs(f) - 1 : -1;
+                    f &= f - 1;
+                }
+                int32_t du[4], dv[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    du[q] = b[q] >= 0 ? du_row[b[q]] : 0;
+                    dv[q] = b[q] >= 0 ? dv_row[b[q]] : 0;
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (b[q] < 0) continue;
+                    const int32_t nd = dev::fbits(dev::bitsf(du[q]) + we);
+                    if (nd >= dv[q]) continue;
+                    __hip_atomic_fetch_min(dv_row + b[q], nd, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                    if (!pend || nd < tbits) near |= 1u << b[q];
+                    else far |= 1u << b[q];
                 }
             }
+            if (far)
+                __hip_atomic_fetch_or(pend + (int64_t)v * W + w, far, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
             if (near) {
                 __hip_atomic_fetch_or(next + (int64_t)v * W + w, near, __ATOMIC_RELAXED,
                                       __HIP_MEMORY_SCOPE_AGENT);
                 any = true;
             }
-            if (far)
-                __hip_atomic_fetch_or(pend + (int64_t)v * W + w, far, __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT);
         }
-        return any && dev::atomic_exch_dev(stamp + v, round) != round;
+        if (!any) return false;
+        const int32_t round = *lvl;
+        return dev::atomic_exch_dev(stamp + v, round) != round;
     }
 };
+
+// Bellman-Ford rounds of the batch (no threshold): every improvement is near.
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_batch_sssp_bf(
+    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col, int32_t nnz,
+    SsspBatchNearOp<W> op, FrontierBufs cur, const uint64_t *cnt_cur, FrontierBufs nxt,
+    uint64_t *cnt_next, uint64_t *cnt_clear, uint64_t *stats) {
+    __shared__ BatchLds lds;
+    const uint64_t cc = *cnt_cur;
+    const int32_t F = (int32_t)cnt_entries(cc), E = (int32_t)cnt_edges(cc);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *cnt_clear = 0;
+        if (F > 0) {
+            stats[0] += (uint64_t)E;
+            stats[1] += (uint64_t)F;
+            stats[2] += 1;
+        }
+    }
+    if (F == 0) return;
+    expand_tiles<kBlock, kTile, true, true, true>(lds.ex, op, rowptr, col, nnz, cur, F, E, nxt,
+                                                   cnt_next);
+}
 
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_batch_sssp_nf(
@@ -723,10 +736,11 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
                 SsspBatchNearOp<W>{front, next, pend, stamp, dist, g->val, lvl, 0}, delta,
                 fs.f[cur], fs.f[cur ^ 1], fs.ctl, l % 6);
         else if (SSSP)
-            k_expand<kBlock, kTile, true, true, true><<<grid, kBlock, 0, c->stream>>>(
-                SsspBatchOp<W>{front, next, stamp, dist, g->val, lvl}, g->rowptr, g->col,
-                (int32_t)g->nnz, fs.f[cur], fs.ctl + (l % 3), fs.f[cur ^ 1],
-                fs.ctl + ((l + 1) % 3), fs.ctl + ((l + 2) % 3), fs.ctl + 4);
+            k_batch_sssp_bf<W><<<grid, kBlock, 0, c->stream>>>(
+                g->rowptr, g->col, (int32_t)g->nnz,
+                SsspBatchNearOp<W>{front, next, nullptr, stamp, dist, g->val, lvl, 0}, fs.f[cur],
+                fs.ctl + (l % 3), fs.f[cur ^ 1], fs.ctl + ((l + 1) % 3), fs.ctl + ((l + 2) % 3),
+                fs.ctl + 4);
         else {
             const BfsBatchOp<W> op{front, vis, next, stamp, depth, m, lvl, nsrc};
             k_batch_bfs_push<W><<<grid, kBlock, 0, c->stream>>>(
