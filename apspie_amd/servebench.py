@@ -81,13 +81,14 @@ def run_stream(server, ids, graphs, stream, rate, seed=11):
 
 def serve_benchmark(graphs, n_queries=300, rates=None, slots=4, max_batch=128,
                     hbm_budget=64 << 30, log=None):
-    """Plain capacity first, then the served form at three offered rates
-    (x1, x4, x16 the plain capacity)."""
+    """Plain capacity first (the whole stream, one query per launch), then the
+    served form on the same stream at three offered rates (x1, x4, x16 the
+    plain capacity) and saturated."""
     res = {}
     stream = make_stream(graphs, n_queries)
     with Server(slots=1, max_batch=1, hbm_budget=hbm_budget) as plain:
         ids = [plain.add_graph(g[1], g[2], g[3]) for g in graphs]
-        r = run_stream(plain, ids, graphs, stream[: max(40, n_queries // 5)], rate=0)
+        r = run_stream(plain, ids, graphs, stream, rate=0)
         res["plain_saturated"] = r
         if log:
             log(f"  plain: {r['qps']:.1f} q/s p50 {r['p50_ms']:.1f} ms p99 {r['p99_ms']:.1f} ms")
