@@ -75,6 +75,10 @@ __device__ __forceinline__ bool claim_bit(uint32_t *bits, int32_t v) {
     return !(dev::atomic_or_dev(bits + (v >> 5), bit) & bit);
 }
 
+// A relaxation that beats the distance it read applies its atomic min without
+// waiting for the result: if a racing thread of the same round lowered v
+// further, v still improved in this round and belongs to the next queue, so
+// the claim (the bitmap OR) decides the append alone.
 struct SsspPushOp {
     const float *w;
     int32_t *dist;  // float bits
@@ -82,7 +86,7 @@ struct SsspPushOp {
     __device__ __forceinline__ bool edge(int32_t u, int32_t e, int32_t v) const {
         const int32_t nd = dev::fbits(dev::bitsf(dist[u]) + w[e]);
         if (nd >= dist[v]) return false;
-        if (nd >= dev::atomic_min_dev(dist + v, nd)) return false;
+        __hip_atomic_fetch_min(dist + v, nd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return claim_bit(fb_next, v);
     }
 };
