@@ -94,6 +94,11 @@ def main():
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--extras", default="sssp,mis,tc,spgemm,batched,served",
+                    help="comma list of the extra workloads to run after the main line")
+    ap.add_argument("--batch-graph", type=int, default=1,
+                    help="replay batched level chunks from a captured hipGraph (1) or launch "
+                         "them one by one (0)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the traversals "
@@ -161,7 +166,9 @@ def main():
            "graph500_gteps": gteps / 2}
 
     if not args.no_extra:
-        rec["extra"] = extras(ctx, G, rowptr, col, srcs, optr, cpu=not args.no_cpu)
+        ctx.set_option("batch_graph", args.batch_graph)
+        rec["extra"] = extras(ctx, G, rowptr, col, srcs, optr, cpu=not args.no_cpu,
+                              which=set(args.extras.split(",")))
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(rowptr, col, srcs)
     else:
@@ -171,16 +178,34 @@ def main():
     print(json.dumps(rec), flush=True)
 
 
-def extras(ctx, G, rowptr, col, srcs, optr, cpu=True):
+def extras(ctx, G, rowptr, col, srcs, optr, cpu=True, which=None):
     """The other workloads of the path, each with its time, the roofline
     fraction where the per-unit bytes are defined, and a bounded CPU sample of
     the oracle on the same graph; then the batched and served forms."""
-    from oracle import oracle as O
+    which = which or {"sssp", "mis", "tc", "spgemm", "batched", "served"}
     ex = {}
+    if "sssp" in which:
+        ex["sssp_rmat22"] = sssp_rmat22(ctx, G, rowptr, col, srcs, optr, cpu)
+    ex.update(other_workloads(ctx, cpu, which))
+    if "batched" in which:
+        ex["batched_lattice2048"] = batched(ctx)
+    if "served" in which:
+        from apspie_amd import servebench as SB
+        log("serving pool:")
+        pool = SB.build_pool(log=log)
+        ex["served"] = SB.serve_benchmark(pool, n_queries=240, log=log)
+    return ex
+
+
+def sssp_rmat22(ctx, G, rowptr, col, srcs, optr, cpu):
+    """SSSP on the same RMAT-22 graph with integer weights 1..255, 16 sources:
+    GTEPS over the edges of the reached vertices, the roofline fraction at
+    12 B per traversed edge (column, weight, distance probe) + 20 B per
+    reached vertex + 4 B per vertex (init) per source, and one source of the
+    oracle's Dijkstra on the host."""
+    from oracle import oracle as O
     m = len(rowptr) - 1
     rp64 = rowptr.astype(np.int64)
-    # SSSP on RMAT-22: 12 B per traversed edge (column, weight, distance
-    # probe) + 20 B per reached vertex + 4 B per vertex (init) per source
     w = A.gen_weights(rowptr, col)
     Gw = ctx.upload(rowptr, col, w)
     fout = ctx.dev_alloc(4 * m)
@@ -194,93 +219,92 @@ def extras(ctx, G, rowptr, col, srcs, optr, cpu=True):
         st = ctx.stats()
         relaxed += st["edges_traversed"]
         t += st["elapsed_ms"]
-    ex["sssp_rmat22"] = {"gteps": edges / (t / 1e3) / 1e9, "ms_per_source": t / 16,
-                         "sources": 16, "expanded_edges_per_traversed": relaxed / edges,
-                         "roofline_frac": alg / (t / 1e3) / HBM_PEAK}
+    res = {"gteps": edges / (t / 1e3) / 1e9, "ms_per_source": t / 16,
+           "sources": 16, "expanded_edges_per_traversed": relaxed / edges,
+           "roofline_frac": alg / (t / 1e3) / HBM_PEAK}
     if cpu:
         deg = np.diff(rowptr)
         t0 = time.perf_counter()
         d = O.sssp(m, rp64, col, w, srcs[0])
         tc_ = time.perf_counter() - t0
-        ex["sssp_rmat22"]["cpu_baseline"] = {
+        res["cpu_baseline"] = {
             "value": int(deg[d >= 0].sum()) / tc_ / 1e9, "unit": "GTEPS", "cores": 1,
             "kind": "port", "sample": f"oracle binary-heap SSSP (SimpleReferenceSssp restated), "
                                       f"1 source of the same graph, {tc_:.1f} s"}
     Gw.close()
     fout.free()
-    ex.update(other_workloads(ctx, cpu))
-    ex["batched_lattice2048"] = batched(ctx)
-    from apspie_amd import servebench as SB
-    log("serving pool:")
-    pool = SB.build_pool(log=log)
-    ex["served"] = SB.serve_benchmark(pool, n_queries=240, log=log)
-    return ex
+    return res
 
 
-def other_workloads(ctx, cpu):
+def other_workloads(ctx, cpu, which):
     """MIS and TC on R-MAT, the general SpGEMM on R-MAT-14 (A * A)."""
     from oracle import oracle as O
     res = {}
-    rp20, col20 = A.gen_rmat(20)
-    m20 = len(rp20) - 1
-    G20 = ctx.upload(rp20, col20)
-    ctx.mis(G20)
-    t0 = time.perf_counter()
-    ctx.mis(G20)
-    res["mis_rmat20"] = {"ms": (time.perf_counter() - t0) * 1e3}
-    if cpu:
-        prio = O.mis_priorities(m20)
+    if which & {"mis", "tc"}:
+        rp20, col20 = A.gen_rmat(20)
+        m20 = len(rp20) - 1
+        G20 = ctx.upload(rp20, col20)
+        if "mis" in which:
+            ctx.mis(G20)
+            t0 = time.perf_counter()
+            ctx.mis(G20)
+            res["mis_rmat20"] = {"ms": (time.perf_counter() - t0) * 1e3}
+            if cpu:
+                prio = O.mis_priorities(m20)
+                t0 = time.perf_counter()
+                O.mis_greedy(m20, rp20.astype(np.int64), col20, prio)
+                res["mis_rmat20"]["cpu_baseline"] = {
+                    "value": (time.perf_counter() - t0) * 1e3, "unit": "ms", "cores": 1, "kind": "port",
+                    "sample": "oracle greedy MIS by descending priority (the set Luby rounds converge "
+                              "to), same graph"}
+        if "tc" in which:
+            # TC: 4 B per row offset and per L entry, 4 B per wedge of L.L (the
+            # reference formulation's sum over (i,k) in L of |L_k|) and 4 B per row
+            rows = np.repeat(np.arange(m20), np.diff(rp20))
+            low = col20 < rows
+            lcnt = np.bincount(rows[low], minlength=m20).astype(np.int64)
+            wedges = int(lcnt[col20[low]].sum())
+            alg = 4.0 * (m20 + 1) + 4.0 * int(low.sum()) + 4.0 * wedges + 4.0 * m20
+            ctx.tc(G20, per_row=True)
+            t0 = time.perf_counter()
+            total, _ = ctx.tc(G20, per_row=True)
+            ms = (time.perf_counter() - t0) * 1e3
+            res["tc_rmat20"] = {"ms": ms, "triangles": int(total), "reference_wedges": wedges,
+                                "probes": ctx.stats()["edges_traversed"],
+                                "roofline_frac": alg / (ms / 1e3) / HBM_PEAK}
+        G20.close()
+    if "tc" in which:
+        rp18, col18 = A.gen_rmat(18)
+        G18 = ctx.upload(rp18, col18)
+        ctx.tc(G18)
         t0 = time.perf_counter()
-        O.mis_greedy(m20, rp20.astype(np.int64), col20, prio)
-        res["mis_rmat20"]["cpu_baseline"] = {
-            "value": (time.perf_counter() - t0) * 1e3, "unit": "ms", "cores": 1, "kind": "port",
-            "sample": "oracle greedy MIS by descending priority (the set Luby rounds converge "
-                      "to), same graph"}
-    # TC: 4 B per row offset and per L entry, 4 B per wedge of L.L (the
-    # reference formulation's sum over (i,k) in L of |L_k|) and 4 B per row
-    rows = np.repeat(np.arange(m20), np.diff(rp20))
-    low = col20 < rows
-    lcnt = np.bincount(rows[low], minlength=m20).astype(np.int64)
-    wedges = int(lcnt[col20[low]].sum())
-    alg = 4.0 * (m20 + 1) + 4.0 * int(low.sum()) + 4.0 * wedges + 4.0 * m20
-    ctx.tc(G20, per_row=True)
-    t0 = time.perf_counter()
-    total, _ = ctx.tc(G20, per_row=True)
-    ms = (time.perf_counter() - t0) * 1e3
-    res["tc_rmat20"] = {"ms": ms, "triangles": int(total), "reference_wedges": wedges,
-                        "probes": ctx.stats()["edges_traversed"],
-                        "roofline_frac": alg / (ms / 1e3) / HBM_PEAK}
-    G20.close()
-    rp18, col18 = A.gen_rmat(18)
-    G18 = ctx.upload(rp18, col18)
-    ctx.tc(G18)
-    t0 = time.perf_counter()
-    ctx.tc(G18)
-    res["tc_rmat18"] = {"ms": (time.perf_counter() - t0) * 1e3}
-    G18.close()
-    if cpu:
+        ctx.tc(G18)
+        res["tc_rmat18"] = {"ms": (time.perf_counter() - t0) * 1e3}
+        G18.close()
+        if cpu:
+            t0 = time.perf_counter()
+            O.tc(len(rp18) - 1, rp18.astype(np.int64), col18)
+            res["tc_rmat18"]["cpu_baseline"] = {
+                "value": (time.perf_counter() - t0) * 1e3, "unit": "ms", "cores": 1, "kind": "port",
+                "sample": "oracle merge-intersection TC (ewiseMultTc semantics), same graph"}
+    if "spgemm" in which:
+        # SpGEMM C = A * A on R-MAT-14 (unit values)
+        rp14, col14 = A.gen_rmat(14)
+        G14 = ctx.upload(rp14, col14, np.ones(len(col14), np.float32))
+        ctx.spgemm(G14, G14).close()
         t0 = time.perf_counter()
-        O.tc(len(rp18) - 1, rp18.astype(np.int64), col18)
-        res["tc_rmat18"]["cpu_baseline"] = {
-            "value": (time.perf_counter() - t0) * 1e3, "unit": "ms", "cores": 1, "kind": "port",
-            "sample": "oracle merge-intersection TC (ewiseMultTc semantics), same graph"}
-    # SpGEMM C = A * A on R-MAT-14 (unit values)
-    rp14, col14 = A.gen_rmat(14)
-    G14 = ctx.upload(rp14, col14, np.ones(len(col14), np.float32))
-    ctx.spgemm(G14, G14).close()
-    t0 = time.perf_counter()
-    C = ctx.spgemm(G14, G14)
-    res["spgemm_rmat14_AxA"] = {"ms": (time.perf_counter() - t0) * 1e3, "nnz_c": C.nnz}
-    C.close()
-    G14.close()
-    if cpu:
-        m14 = len(rp14) - 1
-        t0 = time.perf_counter()
-        O.spgemm(m14, m14, m14, rp14.astype(np.int64), col14, None, rp14.astype(np.int64),
-                 col14, None)
-        res["spgemm_rmat14_AxA"]["cpu_baseline"] = {
-            "value": (time.perf_counter() - t0) * 1e3, "unit": "ms", "cores": 1, "kind": "port",
-            "sample": "oracle Gustavson SpGEMM, same matrices"}
+        C = ctx.spgemm(G14, G14)
+        res["spgemm_rmat14_AxA"] = {"ms": (time.perf_counter() - t0) * 1e3, "nnz_c": C.nnz}
+        C.close()
+        G14.close()
+        if cpu:
+            m14 = len(rp14) - 1
+            t0 = time.perf_counter()
+            O.spgemm(m14, m14, m14, rp14.astype(np.int64), col14, None, rp14.astype(np.int64),
+                     col14, None)
+            res["spgemm_rmat14_AxA"]["cpu_baseline"] = {
+                "value": (time.perf_counter() - t0) * 1e3, "unit": "ms", "cores": 1, "kind": "port",
+                "sample": "oracle Gustavson SpGEMM, same matrices"}
     for k, v in res.items():
         log(f"  {k}: {v}")
     return res
