@@ -108,6 +108,21 @@ __device__ __forceinline__ void table_insert(int32_t *table, uint32_t mask, int3
     }
 }
 
+// A bit filter beside each table: bit h of the key's multiplicative hash
+// (its top BITS bits) is set on insert.  Most probes miss (85 % at RMAT-22),
+// and a clear bit settles a miss with one LDS read instead of a probe chain.
+template <int BITS>
+__device__ __forceinline__ void bloom_insert(uint32_t *bloom, int32_t key) {
+    const uint32_t h = ((uint32_t)key * 2654435769u) >> (32 - BITS);
+    atomicOr(bloom + (h >> 5), 1u << (h & 31));
+}
+
+template <int BITS>
+__device__ __forceinline__ bool bloom_may_have(const uint32_t *bloom, int32_t key) {
+    const uint32_t h = ((uint32_t)key * 2654435769u) >> (32 - BITS);
+    return (bloom[h >> 5] >> (h & 31)) & 1u;
+}
+
 __device__ __forceinline__ bool table_has(const int32_t *table, uint32_t mask, int32_t key) {
     uint32_t h = dev::fib_slot((uint32_t)key, mask);
     for (;;) {
@@ -138,7 +153,9 @@ __global__ __launch_bounds__(kBlock) void k_tc_small(const int32_t *rowptr, cons
                                                      int32_t *per_row, uint64_t *total,
                                                      uint64_t *wedges) {
     constexpr int NW = kBlock / 64;
+    constexpr int kSmallBloom = 11;  // 2048 bits per wave
     __shared__ int32_t s_table[NW][kSmallTable];
+    __shared__ uint32_t s_bloom[NW][(1 << kSmallBloom) / 32];
     __shared__ int32_t s_pre[NW][kSmallRow + 1];
     __shared__ int32_t s_rs[NW][kSmallRow];
     const int w = threadIdx.x / 64, lane = dev::lane_id();
@@ -149,6 +166,7 @@ __global__ __launch_bounds__(kBlock) void k_tc_small(const int32_t *rowptr, cons
         const int32_t n = lowcnt[r];
         const int32_t base = rowptr[r];
         for (int s = lane; s < kSmallTable; s += 64) s_table[w][s] = -1;
+        for (int s = lane; s < (1 << kSmallBloom) / 32; s += 64) s_bloom[w][s] = 0;
         dev::wave_sync();
         // own pairs: every k in L_r with |L_k| <= kSmallRow
         uint32_t carry = 0;
@@ -158,6 +176,7 @@ __global__ __launch_bounds__(kBlock) void k_tc_small(const int32_t *rowptr, cons
             if (idx < n) {
                 const int32_t k = col[base + idx];
                 table_insert(s_table[w], kSmallTable - 1, k);
+                bloom_insert<kSmallBloom>(s_bloom[w], k);
                 s_rs[w][idx] = rowptr[k];
                 const int32_t l = lowcnt[k];
                 lc = l <= kSmallRow ? (uint32_t)l : 0u;  // n <= kSmallRow
@@ -186,7 +205,8 @@ __global__ __launch_bounds__(kBlock) void k_tc_small(const int32_t *rowptr, cons
             }
 #pragma unroll
             for (int q = 0; q < kRun; ++q)
-                if (key[q] >= 0) c += table_has(s_table[w], kSmallTable - 1, key[q]) ? 1u : 0u;
+                if (key[q] >= 0 && bloom_may_have<kSmallBloom>(s_bloom[w], key[q]))
+                    c += table_has(s_table[w], kSmallTable - 1, key[q]) ? 1u : 0u;
         }
         const uint64_t rc = dev::wave_sum_u64(c);
         if (lane == 0) {
@@ -273,7 +293,9 @@ __global__ __launch_bounds__(BLOCK) void k_tc_block(const int32_t *rowptr, const
                                                     int32_t *per_row, uint64_t *total,
                                                     uint64_t *wedges, int32_t *g_table,
                                                     int64_t g_cap) {
+    constexpr int BLOOM = TABLE <= 4096 ? 14 : 16;  // 2 KB / 8 KB of filter bits
     __shared__ int32_t s_table[TABLE];
+    __shared__ uint32_t s_bloom[(1 << BLOOM) / 32];
     __shared__ int32_t pre[CHUNK + 1];
     __shared__ int32_t rsk[CHUNK];
     __shared__ int32_t cntu[CHUNK];
@@ -298,9 +320,13 @@ __global__ __launch_bounds__(BLOCK) void k_tc_block(const int32_t *rowptr, const
         const uint32_t mask = tsize - 1;
         if (r != table_row) {  // uniform: every thread read the same item
             for (uint32_t s = threadIdx.x; s < tsize; s += BLOCK) table[s] = -1;
+            for (int s = threadIdx.x; s < (1 << BLOOM) / 32; s += BLOCK) s_bloom[s] = 0;
             __syncthreads();
-            for (int32_t idx = threadIdx.x; idx < n; idx += BLOCK)
-                table_insert(table, mask, col[base + idx]);
+            for (int32_t idx = threadIdx.x; idx < n; idx += BLOCK) {
+                const int32_t k = col[base + idx];
+                table_insert(table, mask, k);
+                bloom_insert<BLOOM>(s_bloom, k);
+            }
             __syncthreads();
             table_row = r;
         }
@@ -327,7 +353,8 @@ __global__ __launch_bounds__(BLOCK) void k_tc_block(const int32_t *rowptr, const
             if (threadIdx.x == 0) pre[cn] = (int32_t)carry;
             __syncthreads();
             for_wedges<BLOCK>(col, pre, rsk, cn, (int32_t)carry, nnz, [&](int, int32_t key) {
-                c += table_has(table, mask, key) ? 1u : 0u;
+                c += (bloom_may_have<BLOOM>(s_bloom, key) && table_has(table, mask, key)) ? 1u
+                                                                                       : 0u;
             });
             if (threadIdx.x == 0) my_wedges += carry;
             const uint64_t wc = dev::wave_sum_u64(c);
@@ -364,7 +391,8 @@ __global__ __launch_bounds__(BLOCK) void k_tc_block(const int32_t *rowptr, const
             __syncthreads();
             if (carry > 0) {  // uniform: most chunks of a hub's upper part scan nothing
                 for_wedges<BLOCK>(col, pre, rsk, cn, (int32_t)carry, nnz, [&](int j, int32_t key) {
-                    if (table_has(table, mask, key)) atomicAdd(cntu + j, 1);
+                    if (bloom_may_have<BLOOM>(s_bloom, key) && table_has(table, mask, key))
+                        atomicAdd(cntu + j, 1);
                 });
                 __syncthreads();
                 uint64_t cu = 0;
