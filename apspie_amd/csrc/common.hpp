@@ -87,6 +87,11 @@ struct Ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // pinned host word for counters read back by the host
     void *pinned = nullptr;
+    // the default MIS priorities (mt19937 seeded 5489, one draw per vertex)
+    // for mis_prio_m vertices, kept between calls: the stream is sequential
+    // and generated on the host
+    Buffer mis_prio;
+    int32_t mis_prio_m = -1;
     // graphs whose memory is accounted to this context
     std::set<Graph *> graphs;
 };

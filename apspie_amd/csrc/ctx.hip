@@ -232,6 +232,7 @@ int apspie_ctx_destroy(apspie_ctx *c) {
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->mis_prio.ptr) (void)hipFree(c->mis_prio.ptr);
     // graphs outliving their context keep their memory but lose the accounting
     for (Graph *g : c->graphs) g->owner = nullptr;
     delete c;
@@ -565,9 +566,17 @@ int apspie_mis(apspie_ctx *c, const apspie_graph *g, const uint32_t *prio, int p
         set_device(c);
         const uint32_t *p = prio;
         if (!prio) {
-            uint32_t *d = (uint32_t *)scratch(c, "mis_prio", sizeof(uint32_t) * g->m);
-            mt19937_priorities(c, 5489u, g->m, d);
-            p = d;
+            if (c->mis_prio_m != g->m) {
+                if (c->mis_prio.ptr) dev_free(c, c->mis_prio.ptr, c->mis_prio.bytes);
+                c->mis_prio = Buffer{};
+                c->mis_prio_m = -1;
+                const size_t bytes = sizeof(uint32_t) * std::max<int32_t>(g->m, 1);
+                c->mis_prio.ptr = dev_alloc(c, bytes);
+                c->mis_prio.bytes = bytes;
+                mt19937_priorities(c, 5489u, g->m, (uint32_t *)c->mis_prio.ptr);
+                c->mis_prio_m = g->m;
+            }
+            p = (const uint32_t *)c->mis_prio.ptr;
         } else if (prio_mem == APSPIE_MEM_HOST) {
             uint32_t *d = (uint32_t *)scratch(c, "mis_prio", sizeof(uint32_t) * g->m);
             copy_to_device(c, d, prio, sizeof(uint32_t) * g->m, APSPIE_MEM_HOST);

@@ -245,10 +245,19 @@ def other_workloads(ctx, cpu, which):
         m20 = len(rp20) - 1
         G20 = ctx.upload(rp20, col20)
         if "mis" in which:
-            ctx.mis(G20)
+            mout = ctx.dev_alloc(4 * m20)  # result stays in HBM, as for the traversals
+            ctx.mis(G20, out=mout.ptr, mem=A.DEVICE)
             t0 = time.perf_counter()
-            ctx.mis(G20)
-            res["mis_rmat20"] = {"ms": (time.perf_counter() - t0) * 1e3}
+            ctx.mis(G20, out=mout.ptr, mem=A.DEVICE)
+            ms = (time.perf_counter() - t0) * 1e3
+            mout.free()
+            st = ctx.stats()
+            # per round: two passes over the candidates' rows (column + state
+            # probe, 8 B per entry each) and 24 B per candidate (row extent,
+            # queue entry, join test)
+            alg = 16.0 * st["edges_traversed"] + 24.0 * st["vertices_reached"]
+            res["mis_rmat20"] = {"ms": ms, "rounds": st["iterations"],
+                                 "roofline_frac": alg / (ms / 1e3) / HBM_PEAK}
             if cpu:
                 prio = O.mis_priorities(m20)
                 t0 = time.perf_counter()
@@ -294,7 +303,15 @@ def other_workloads(ctx, cpu, which):
         ctx.spgemm(G14, G14).close()
         t0 = time.perf_counter()
         C = ctx.spgemm(G14, G14)
-        res["spgemm_rmat14_AxA"] = {"ms": (time.perf_counter() - t0) * 1e3, "nnz_c": C.nnz}
+        ms = (time.perf_counter() - t0) * 1e3
+        # products sum_k deg(k)^2 (A symmetric): 4 B each in the symbolic pass
+        # (B's column), 8 B in the numeric pass (column + value); 8 B per
+        # output entry, 8 B per entry of A per pass
+        deg14 = np.diff(rp14).astype(np.int64)
+        prods = int((deg14 * deg14).sum())
+        alg = 12.0 * prods + 8.0 * C.nnz + 16.0 * len(col14)
+        res["spgemm_rmat14_AxA"] = {"ms": ms, "nnz_c": C.nnz, "products": prods,
+                                    "roofline_frac": alg / (ms / 1e3) / HBM_PEAK}
         C.close()
         G14.close()
         if cpu:
