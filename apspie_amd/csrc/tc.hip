@@ -38,6 +38,7 @@ constexpr int kBigTable = 32768;
 constexpr int kBigBlock = 1024;    // also for the longest rows (table in global memory)
 constexpr int kBigChunk = 1024;    // entries per enumeration chunk in the big tiers
 constexpr int kRun = 8;            // consecutive wedges per lane (one binary search)
+constexpr int kBlockRun = 8;       // the same in the block tiers (16 measured no faster)
 
 // lowcnt[r] = number of entries of row r below the diagonal (rows sorted)
 __global__ void k_lowcnt(const int32_t *rowptr, const int32_t *col, int32_t m, int32_t *lowcnt) {
@@ -228,15 +229,15 @@ __global__ __launch_bounds__(kBlock) void k_tc_small(const int32_t *rowptr, cons
 // takes kRun consecutive wedges, finds the first one's entry by binary search
 // and walks on from there, issuing the kRun column loads back to back before
 // probing; f(j, key) is called for every wedge with the column it reaches.
-template <int BLOCK, class F>
+template <int BLOCK, int RUN, class F>
 __device__ __forceinline__ void for_wedges(const int32_t *col, const int32_t *pre,
                                            const int32_t *rsk, int32_t cn, int32_t W,
                                            int32_t nnz, F &&f) {
-    for (int64_t x0 = (int64_t)threadIdx.x * kRun; x0 < W; x0 += (int64_t)BLOCK * kRun) {
+    for (int64_t x0 = (int64_t)threadIdx.x * RUN; x0 < W; x0 += (int64_t)BLOCK * RUN) {
         int j = dev::upper_bound_m1(pre, 0, cn, (int32_t)x0);
-        int32_t key[kRun], jj[kRun];
+        int32_t key[RUN], jj[RUN];
 #pragma unroll
-        for (int q = 0; q < kRun; ++q) {
+        for (int q = 0; q < RUN; ++q) {
             const int32_t x = (int32_t)x0 + q;
             key[q] = -1;
             jj[q] = j;
@@ -248,7 +249,7 @@ __device__ __forceinline__ void for_wedges(const int32_t *col, const int32_t *pr
             }
         }
 #pragma unroll
-        for (int q = 0; q < kRun; ++q)
+        for (int q = 0; q < RUN; ++q)
             if (key[q] >= 0) f(jj[q], key[q]);
     }
 }
@@ -352,7 +353,7 @@ __global__ __launch_bounds__(BLOCK) void k_tc_block(const int32_t *rowptr, const
             }
             if (threadIdx.x == 0) pre[cn] = (int32_t)carry;
             __syncthreads();
-            for_wedges<BLOCK>(col, pre, rsk, cn, (int32_t)carry, nnz, [&](int, int32_t key) {
+            for_wedges<BLOCK, kBlockRun>(col, pre, rsk, cn, (int32_t)carry, nnz, [&](int, int32_t key) {
                 c += (bloom_may_have<BLOOM>(s_bloom, key) && table_has(table, mask, key)) ? 1u
                                                                                        : 0u;
             });
@@ -390,7 +391,7 @@ __global__ __launch_bounds__(BLOCK) void k_tc_block(const int32_t *rowptr, const
             if (threadIdx.x == 0) pre[cn] = (int32_t)carry;
             __syncthreads();
             if (carry > 0) {  // uniform: most chunks of a hub's upper part scan nothing
-                for_wedges<BLOCK>(col, pre, rsk, cn, (int32_t)carry, nnz, [&](int j, int32_t key) {
+                for_wedges<BLOCK, kBlockRun>(col, pre, rsk, cn, (int32_t)carry, nnz, [&](int j, int32_t key) {
                     if (bloom_may_have<BLOOM>(s_bloom, key) && table_has(table, mask, key))
                         atomicAdd(cntu + j, 1);
                 });
