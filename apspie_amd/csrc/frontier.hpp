@@ -19,6 +19,8 @@
 // expansion whose cost does not depend on the degree skew.
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 #include "device.hpp"
 
@@ -98,6 +100,15 @@ struct ExpandLds {
     uint64_t base;
 };
 
+// An Op may declare `static constexpr bool kStreamColumns = true`: the column
+// indices are then read with non-temporal loads, so the streamed CSR does not
+// push the Op's randomly probed per-vertex data out of the L2.
+template <class Op, class = void>
+struct streams_columns : std::false_type {};
+template <class Op>
+struct streams_columns<Op, std::void_t<decltype(Op::kStreamColumns)>>
+    : std::bool_constant<Op::kStreamColumns> {};
+
 // prefetch loads the row extent of every edge's endpoint together with the
 // column instead of after a successful claim: one dependent round trip less
 // per level for 8 extra bytes per edge, worth it on small (latency-bound)
@@ -138,7 +149,9 @@ __device__ __forceinline__ void expand_tiles(ExpandLds<BLOCK, TILE, STORE_V_IN> 
                 const int j = dev::upper_bound_m1(s_off, 0, nv, e);
                 const int32_t idx = s_start[j] + (e - s_off[j]);
                 if ((uint32_t)idx >= (uint32_t)nnz) continue;
-                const int32_t v = col[idx];
+                int32_t v;
+                if constexpr (streams_columns<Op>::value) v = __builtin_nontemporal_load(col + idx);
+                else v = col[idx];
                 const int32_t u = STORE_V_IN ? s_v[j] : -1;
                 int32_t prs = 0, pre = 0;
                 if (PRODUCE && prefetch) {

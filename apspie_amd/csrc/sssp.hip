@@ -80,11 +80,12 @@ __device__ __forceinline__ bool claim_bit(uint32_t *bits, int32_t v) {
 // further, v still improved in this round and belongs to the next queue, so
 // the claim (the bitmap OR) decides the append alone.
 struct SsspPushOp {
+    static constexpr bool kStreamColumns = true;  // columns and weights stream past the L2
     const float *w;
     int32_t *dist;  // float bits
     uint32_t *fb_next;
     __device__ __forceinline__ bool edge(int32_t u, int32_t e, int32_t v) const {
-        const int32_t nd = dev::fbits(dev::bitsf(dist[u]) + w[e]);
+        const int32_t nd = dev::fbits(dev::bitsf(dist[u]) + __builtin_nontemporal_load(w + e));
         if (nd >= dist[v]) return false;
         __hip_atomic_fetch_min(dist + v, nd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return claim_bit(fb_next, v);
