@@ -129,6 +129,9 @@ def _declare(L):
         "apspie_server_add_graph": ([vp, C.POINTER(_Csr), C.POINTER(i32)], C.c_int),
         "apspie_server_submit": ([vp, i32, C.c_int, i32, C.POINTER(i64)], C.c_int),
         "apspie_server_wait": ([vp, i64, vp, i64, C.POINTER(f64)], C.c_int),
+        "apspie_server_wait_view": ([vp, i64, C.POINTER(vp), C.POINTER(i64), C.POINTER(f64)],
+                                    C.c_int),
+        "apspie_server_release": ([vp, i64], C.c_int),
         "apspie_server_drain": ([vp], C.c_int),
     }
     for name, (args, res) in sig.items():

@@ -101,6 +101,8 @@ struct apspie_server {
     std::condition_variable cv_work, cv_done;
     std::deque<std::shared_ptr<Request>> queue;
     std::unordered_map<int64_t, std::shared_ptr<Request>> live;
+    // results handed out by apspie_server_wait_view, until released
+    std::unordered_map<int64_t, std::shared_ptr<Request>> viewed;
     std::vector<std::thread> workers;
     std::vector<apspie_ctx *> ctxs;
     PinnedPool pinned;
@@ -337,6 +339,46 @@ int apspie_server_wait(apspie_server *s, int64_t ticket, void *out, int64_t out_
     s->pinned.put(r->buf);
     r->buf = nullptr;
     return st;
+}
+
+int apspie_server_wait_view(apspie_server *s, int64_t ticket, const void **data, int64_t *bytes,
+                            double *latency_ms) {
+    if (!s || !data || !bytes) return APSPIE_ERR_INVALID;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->live.find(ticket);
+    if (it == s->live.end()) return APSPIE_ERR_INVALID;
+    auto r = it->second;
+    s->cv_done.wait(lk, [&] { return r->done; });
+    s->live.erase(ticket);
+    if (latency_ms)
+        *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+    *data = nullptr;
+    *bytes = 0;
+    if (r->status != APSPIE_OK) {  // nothing to hold: the buffer goes back now
+        lk.unlock();
+        s->pinned.put(r->buf);
+        r->buf = nullptr;
+        return r->status;
+    }
+    *bytes = result_bytes(s, r->kind, r->graph);
+    *data = r->kind == APSPIE_Q_TC ? (const void *)&r->triangles : r->buf;
+    s->viewed[ticket] = r;
+    return APSPIE_OK;
+}
+
+int apspie_server_release(apspie_server *s, int64_t ticket) {
+    if (!s) return APSPIE_ERR_INVALID;
+    std::shared_ptr<Request> r;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->viewed.find(ticket);
+        if (it == s->viewed.end()) return APSPIE_ERR_INVALID;
+        r = it->second;
+        s->viewed.erase(it);
+    }
+    s->pinned.put(r->buf);
+    r->buf = nullptr;
+    return APSPIE_OK;
 }
 
 int apspie_server_drain(apspie_server *s) {

@@ -1,7 +1,8 @@
 """Served query stream over a resident graph pool (the serving workload):
 Poisson arrivals of mixed queries (60% BFS, 30% SSSP, 10% TC) with a
 uniformly random graph and source, submitted to the continuous-batching
-server; reports queries/s and p50/p99 latency (submit -> result on the host).
+server; reports queries/s and p50/p99 latency (submit -> result in pinned host
+memory, handed out without a copy).
 
 The same stream is replayed against the plain form (one worker, one query per
 launch, no coalescing) for comparison.
@@ -69,7 +70,9 @@ def run_stream(server, ids, graphs, stream, rate, seed=11):
     for i, (gi, kind, src) in enumerate(stream):
         while tickets[i] is None:
             time.sleep(0.0002)
-        _, ms = server.wait(tickets[i], kind, len(graphs[gi][1]) - 1)
+        # results as views of the server's pinned buffers (no host copy)
+        res, ms = server.wait(tickets[i], kind, len(graphs[gi][1]) - 1, copy=False)
+        del res
         lat.append(ms)
     th.join()
     wall = time.perf_counter() - t0

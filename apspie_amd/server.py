@@ -3,6 +3,7 @@ in include/apspie.h)."""
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -55,16 +56,36 @@ class Server:
         _check(self._L.apspie_server_submit(self.h, graph_id, kind, int(source), C.byref(t)))
         return t.value
 
-    def wait(self, ticket, kind, m):
-        """Returns (result, latency_ms): int32[m] depths, float32[m] distances or an int."""
+    def wait(self, ticket, kind, m, copy=True):
+        """Returns (result, latency_ms): int32[m] depths, float32[m] distances or an int.
+        With copy=False a depth / distance result is a read-only view of the
+        server's pinned host buffer (no copy); the buffer returns to the pool
+        when the array is garbage-collected, and the view must not outlive
+        the server."""
         lat = C.c_double()
         if kind == Q_TC:
             out = C.c_uint64()
             _check(self._L.apspie_server_wait(self.h, ticket, C.byref(out), 8, C.byref(lat)))
             return int(out.value), lat.value
-        out = np.empty(m, np.int32 if kind == Q_BFS else np.float32)
-        _check(self._L.apspie_server_wait(self.h, ticket, _p(out), out.nbytes, C.byref(lat)))
-        return out, lat.value
+        dtype = np.int32 if kind == Q_BFS else np.float32
+        if copy:
+            out = np.empty(m, dtype)
+            _check(self._L.apspie_server_wait(self.h, ticket, _p(out), out.nbytes, C.byref(lat)))
+            return out, lat.value
+        data, nbytes = C.c_void_p(), C.c_int64()
+        _check(self._L.apspie_server_wait_view(self.h, ticket, C.byref(data), C.byref(nbytes),
+                                               C.byref(lat)))
+        buf = (C.c_char * nbytes.value).from_address(data.value)
+        view = np.frombuffer(buf, dtype=dtype, count=nbytes.value // 4)
+        view.flags.writeable = False
+        weakref.finalize(view, Server._release, weakref.ref(self), ticket)
+        return view, lat.value
+
+    @staticmethod
+    def _release(server_ref, ticket):
+        srv = server_ref()
+        if srv is not None and srv.h:  # a closed server already freed its buffers
+            srv._L.apspie_server_release(srv.h, ticket)
 
     def drain(self):
         _check(self._L.apspie_server_drain(self.h))

@@ -257,6 +257,13 @@ int apspie_server_submit(apspie_server *s, int32_t graph_id, int kind, int32_t s
  * 1 uint64 count) into `out` (host) and reports arrival->completion latency. */
 int apspie_server_wait(apspie_server *s, int64_t ticket, void *out, int64_t out_bytes,
                        double *latency_ms);
+/* Zero-copy wait: *data points at the result in the server's pinned host
+ * memory (*bytes long, layout as above) and stays valid until
+ * apspie_server_release(ticket) or apspie_server_destroy.  No reference
+ * counterpart (the reference has no server). */
+int apspie_server_wait_view(apspie_server *s, int64_t ticket, const void **data, int64_t *bytes,
+                            double *latency_ms);
+int apspie_server_release(apspie_server *s, int64_t ticket);
 int apspie_server_drain(apspie_server *s);
 
 #ifdef __cplusplus

@@ -211,6 +211,32 @@ def test_server_mixed_stream_equals_alone():
         srv.drain()
 
 
+def test_server_zero_copy_views():
+    """wait(copy=False) hands out read-only views of pinned result buffers:
+    equal to the copying wait, intact while later requests run, returned to
+    the pool when collected (a released ticket cannot be released again)."""
+    import gc
+    g = T.lattice(64)
+    with S.Server(slots=2, max_batch=8, hbm_budget=1 << 30) as srv:
+        gi = srv.add_graph(g.rowptr32, g.col, g.w)
+        src = T.sources(g, 12, seed=5)
+        tickets = [srv.submit(gi, A.Q_BFS, s) for s in src]
+        views = [srv.wait(t, A.Q_BFS, g.m, copy=False)[0] for t in tickets]
+        more = [srv.submit(gi, A.Q_SSSP, s) for s in src]  # reuse pressure on the pool
+        for t, s in zip(more, src):
+            d, _ = srv.wait(t, A.Q_SSSP, g.m, copy=False)
+            assert np.array_equal(d, oracle_sssp(g, s))
+        for v, s in zip(views, src):
+            assert not v.flags.writeable
+            assert np.array_equal(v, O.bfs(g.m, g.rowptr, g.col, s)[0])
+        t = srv.submit(gi, A.Q_TC)
+        assert srv.wait(t, A.Q_TC, g.m, copy=False)[0] == O.tc(g.m, g.rowptr, g.col)[0]
+        del views, d
+        gc.collect()
+        with pytest.raises(A.ApspieError):  # already released by the collector
+            A._check(srv._L.apspie_server_release(srv.h, tickets[0]))
+
+
 def test_server_budget_and_bad_requests():
     g = T.rmat(12)
     with S.Server(slots=1, max_batch=4, hbm_budget=100_000) as srv:
