@@ -117,33 +117,41 @@ struct BfsOp {
 //     levels then skip it after one bitmap read;
 //   frontier bitmaps fb0 = {source}, fb1 = {} (fb2 is cleared by level 0);
 //   the counter block and the level-0 frontier (the source's row).
-// One thread per vertex; a wave writes its two bitmap words from a ballot.
+// Depths -1 (the source 0) with 16-byte stores where the output is aligned,
+// then the bitmap words: visited = the graph's isolated-vertex bits (given
+// for a symmetric CSR, where such a vertex is never reached) | the source,
+// the level-0 frontier bitmap = the source, the next one cleared.
 template <int TILE>
-__global__ void k_bfs_init(const int32_t *__restrict__ rowptr, int32_t m, int32_t src,
-                           bool iso_visited, int32_t *depth, uint32_t *vis, uint32_t *fb0,
+__global__ void k_bfs_init(const int32_t *__restrict__ rowptr, const uint32_t *__restrict__ iso,
+                           int32_t m, int32_t src, int32_t *depth, uint32_t *vis, uint32_t *fb0,
                            uint32_t *fb1, FrontierBufs f0, uint64_t *ctl) {
-    const int lane = dev::lane_id();
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < m;
-         base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t v = base + threadIdx.x;
-        bool iso = false;
-        if (v < m) {
-            depth[v] = v == src ? 0 : -1;
-            iso = iso_visited && rowptr[v] == rowptr[v + 1];
-        }
-        const uint64_t sm = __ballot(v == src);
-        const uint64_t vm = __ballot(iso) | sm;
-        const int64_t w = (base + (threadIdx.x & ~63)) >> 5;
-        if (lane == 0 || lane == 32) {
-            const int64_t wi = w + (lane >> 5);
-            const int sh = lane;  // 0 or 32
-            if (wi * 32 < m) {
-                vis[wi] = (uint32_t)(vm >> sh);
-                if (fb0) {
-                    fb0[wi] = (uint32_t)(sm >> sh);
-                    fb1[wi] = 0;
-                }
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t done = 0;
+    if (((uintptr_t)depth & 15) == 0) {
+        int4 *d4 = reinterpret_cast<int4 *>(depth);
+        const int64_t n4 = m / 4;
+        for (int64_t q = tid; q < n4; q += stride) {
+            int4 x = make_int4(-1, -1, -1, -1);
+            if (q == src / 4) {
+                const int r = src % 4;
+                x.x = r == 0 ? 0 : -1;
+                x.y = r == 1 ? 0 : -1;
+                x.z = r == 2 ? 0 : -1;
+                x.w = r == 3 ? 0 : -1;
             }
+            d4[q] = x;
+        }
+        done = n4 * 4;
+    }
+    for (int64_t v = done + tid; v < m; v += stride) depth[v] = v == src ? 0 : -1;
+    const int64_t words = ((int64_t)m + 31) / 32;
+    for (int64_t wi = tid; wi < words; wi += stride) {
+        const uint32_t sbit = (src >> 5) == wi ? 1u << (src & 31) : 0u;
+        vis[wi] = (iso ? iso[wi] : 0u) | sbit;
+        if (fb0) {
+            fb0[wi] = sbit;
+            fb1[wi] = 0;
         }
     }
     if (blockIdx.x == 0) {
@@ -361,8 +369,9 @@ void bfs_levels(Ctx *c, const Graph *g, int32_t source, int32_t *depth) {
     uint32_t *fb = do_bfs ? (uint32_t *)scratch(c, "bfs_fb", sizeof(uint32_t) * 3 * words) : nullptr;
     // symmetric: a vertex without neighbours is never reached, so it starts
     // out visited; fb[2 * words ..] is cleared by level 0 itself
-    k_bfs_init<TILE><<<(int)std::min<int64_t>(ceil_div(g->m, kBlock), 16 * c->cus), kBlock, 0,
-                       c->stream>>>(g->rowptr, g->m, source, do_bfs, depth, vis, fb,
+    const uint32_t *iso = do_bfs ? graph_iso_bits(c, const_cast<Graph *>(g)) : nullptr;
+    k_bfs_init<TILE><<<(int)std::min<int64_t>(ceil_div(g->m, 4 * 256), 4 * c->cus), 256, 0,
+                       c->stream>>>(g->rowptr, iso, g->m, source, depth, vis, fb,
                                     fb ? fb + words : nullptr, fs.f[0], fs.ctl);
     APSPIE_LAUNCH_CHECK();
 

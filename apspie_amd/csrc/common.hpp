@@ -60,6 +60,7 @@ struct Graph {
     int64_t bytes = 0;
     // derived, built lazily and cached with the graph
     int32_t *lowcnt = nullptr;  // per row: number of cols < row (TC)
+    uint32_t *iso_bits = nullptr;  // bit per vertex without entries (BFS visited start)
     int symmetric = -1;         // 1 / 0 once known (pull traversal needs A == A^T)
     int sym_weights = -1;       // 1 / 0 once known: values equal to the transpose's too
     // rows longer than the SSSP pull's lane limit, as a frontier over those
@@ -159,6 +160,7 @@ bool graph_is_symmetric(Ctx *c, Graph *g);
 // symmetric structure and values (pulling a weighted relaxation reads w(v, u)
 // for w(u, v))
 bool graph_is_symmetric_weighted(Ctx *c, Graph *g);
+const uint32_t *graph_iso_bits(Ctx *c, Graph *g);
 
 }  // namespace apspie
 

@@ -460,6 +460,7 @@ int apspie_graph_destroy(apspie_graph *g) {
         }
     }
     if (g->lowcnt) (void)hipFree(g->lowcnt);
+    if (g->iso_bits) (void)hipFree(g->iso_bits);
     if (g->heavy_mem) (void)hipFree(g->heavy_mem);
     delete g;
     return APSPIE_OK;

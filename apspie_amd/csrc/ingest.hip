@@ -214,6 +214,39 @@ static void check_symmetry(Ctx *c, Graph *g) {
     g->sym_weights = (g->symmetric == 1 && g->val && h[1] == 0) ? 1 : 0;
 }
 
+__global__ void k_iso_bits(const int32_t *rowptr, int32_t m, uint32_t *bits) {
+    const int64_t words = ((int64_t)m + 31) / 32;
+    for (int64_t wi = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; wi < words;
+         wi += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t x = 0;
+        int32_t prev = rowptr[wi * 32];
+        for (int b = 0; b < 32 && wi * 32 + b < m; ++b) {
+            const int32_t next = rowptr[wi * 32 + b + 1];
+            if (next == prev) x |= 1u << b;
+            prev = next;
+        }
+        bits[wi] = x;
+    }
+}
+
+// Bitmap of the vertices without entries, built once per graph.
+const uint32_t *graph_iso_bits(Ctx *c, Graph *g) {
+    std::lock_guard<std::mutex> lock(g->derived_mu);
+    if (g->iso_bits) return g->iso_bits;
+    const int64_t words = ((int64_t)g->m + 31) / 32;
+    uint32_t *bits = nullptr;
+    // owned by the graph (freed with it), like its other derived arrays
+    APSPIE_HIP(hipMalloc(&bits, sizeof(uint32_t) * std::max<int64_t>(words, 1)));
+    if (words > 0) {
+        k_iso_bits<<<(int)std::min<int64_t>(ceil_div(words, kBlock), 4 * c->cus), kBlock, 0,
+                     c->stream>>>(g->rowptr, g->m, bits);
+        APSPIE_LAUNCH_CHECK();
+    }
+    sync(c);
+    g->iso_bits = bits;
+    return bits;
+}
+
 bool graph_is_symmetric(Ctx *c, Graph *g) {
     std::lock_guard<std::mutex> lock(g->derived_mu);
     check_symmetry(c, g);
