@@ -5,8 +5,11 @@
 // the candidate settings on a few sources and keeps the fastest in the
 // context's option table (readable with apspie_ctx_get_option, so callers can
 // persist them per GPU and graph class):
-//   batch == 0: single-source BFS over tile size x blocks per CU;
-//   batch  > 0: batched BFS over blocks per CU x levels per hipGraph chunk.
+//   batch == 0: single-source BFS over tile size x blocks per CU, then the
+//               direction switch threshold alpha (push -> pull);
+//   batch  > 0: batched BFS over blocks per CU x levels per hipGraph chunk,
+//               then the pull threshold batch_pull_div, and for a weighted
+//               graph the batched SSSP threshold step batch_sssp_delta.
 #include <algorithm>
 #include <random>
 #include <vector>
@@ -58,6 +61,18 @@ extern "C" int apspie_autotune(apspie_ctx *c, const apspie_graph *g, int32_t bat
                     }
                 apspie_ctx_set_option(c, "bfs_tile", bt);
                 apspie_ctx_set_option(c, "bfs_blocks_per_cu", bb);
+                int64_t ba = 14;
+                best = 1e300;
+                for (int64_t alpha : {7, 14, 28}) {
+                    apspie_ctx_set_option(c, "bfs_alpha", alpha);
+                    double tot = 0;
+                    for (size_t i = 0; i < std::min<size_t>(3, src.size()); ++i)
+                        tot += time_of([&] {
+                            return apspie_bfs(c, g, src[i], out, APSPIE_MEM_DEVICE);
+                        });
+                    if (tot < best) { best = tot; ba = alpha; }
+                }
+                apspie_ctx_set_option(c, "bfs_alpha", ba);
             } else {
                 double best = 1e300;
                 int64_t bb = 4, bl = 12;
@@ -73,6 +88,29 @@ extern "C" int apspie_autotune(apspie_ctx *c, const apspie_graph *g, int32_t bat
                     }
                 apspie_ctx_set_option(c, "batch_blocks_per_cu", bb);
                 apspie_ctx_set_option(c, "bfs_levels_per_sync", bl);
+                int64_t bd = 4;
+                best = 1e300;
+                for (int64_t div : {2, 4, 8}) {
+                    apspie_ctx_set_option(c, "batch_pull_div", div);
+                    const double t = time_of([&] {
+                        return apspie_bfs_batch(c, g, src.data(), batch, out, APSPIE_MEM_DEVICE);
+                    });
+                    if (t < best) { best = t; bd = div; }
+                }
+                apspie_ctx_set_option(c, "batch_pull_div", bd);
+                if (g->val) {
+                    int64_t bs = 256;
+                    best = 1e300;
+                    for (int64_t delta : {64, 256, 1024}) {
+                        apspie_ctx_set_option(c, "batch_sssp_delta", delta);
+                        const double t = time_of([&] {
+                            return apspie_sssp_batch(c, g, src.data(), batch, (float *)out,
+                                                     APSPIE_MEM_DEVICE);
+                        });
+                        if (t < best) { best = t; bs = delta; }
+                    }
+                    apspie_ctx_set_option(c, "batch_sssp_delta", bs);
+                }
             }
         } catch (...) {
             (void)hipFree(out);

@@ -172,15 +172,21 @@ def test_batch_many_sources_split(ctx):
 def test_autotune_sets_valid_options():
     g = T.rmat(12)
     with A.Context(0) as c:
-        G = c.upload(g.rowptr32, g.col)
+        G = c.upload(g.rowptr32, g.col, g.w)
         c.autotune(G, 0)
         assert c.get_option("bfs_tile") in (1024, 2048, 4096)
         assert c.get_option("bfs_blocks_per_cu") in (2, 4, 8)
+        assert c.get_option("bfs_alpha") in (7, 14, 28)
         c.autotune(G, 32)
         assert c.get_option("batch_blocks_per_cu") in (2, 4, 8)
+        assert c.get_option("batch_pull_div") in (2, 4, 8)
+        assert c.get_option("batch_sssp_delta") in (64, 256, 1024)
         src = T.sources(g, 4)
         for s in src:
             assert np.array_equal(c.bfs(G, s), O.bfs(g.m, g.rowptr, g.col, s)[0])
+        got = c.sssp_batch(G, src)
+        for i, s in enumerate(src):
+            assert np.array_equal(got[i], oracle_sssp(g, s))
         G.close()
 
 
