@@ -19,11 +19,10 @@ static const char *kOptionNames[] = {
                          // symmetric weights only)
     "spmspv_path",       // 0 auto, 1 sort, 2 atomic
     "spmspv_sort_max",   // auto: use the sort path up to this many expanded edges
-    "tc_small_row",      // rows with |L_r| <= this use a wave + LDS hash
-    "tc_block_row",      // rows with |L_r| <= this use a block + LDS hash
     "batch_blocks_per_cu",
     "batch_graph",       // 1 = replay batched level chunks from a captured hipGraph
-    "batch_pull_div",    // batched BFS pulls a level when union-frontier edges * this > nnz (0 = never)
+    "batch_pull_div",    // batched BFS pulls a level when union-frontier edges * this > nnz
+                         // (times 4 for batches of up to 128 sources; 0 = never)
     "batch_sssp_delta",  // batched SSSP near-far threshold step (0 = Bellman-Ford frontier)
     "batch_snapshot_mb", // HBM for the bit rows of pulled batched BFS levels (0 = none)
     "profiling",         // 1 = time the dominant kernel with hipEvents
@@ -40,8 +39,6 @@ static int64_t option_default(const std::string &n) {
     if (n == "sssp_pull_div") return 0;
     if (n == "spmspv_path") return 0;
     if (n == "spmspv_sort_max") return 1 << 16;
-    if (n == "tc_small_row") return 256;
-    if (n == "tc_block_row") return 4096;
     if (n == "batch_blocks_per_cu") return 4;
     if (n == "batch_graph") return 1;
     if (n == "batch_pull_div") return 4;
