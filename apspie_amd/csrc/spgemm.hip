@@ -41,16 +41,21 @@ constexpr int kRun = 8;             // consecutive products per lane
 constexpr int32_t kEmpty = 0x7fffffff;  // sorts after every column id
 
 // u[i] = min(products of row i, n)
+// one wave per row of A (the lanes stride its entries): a hub row is not
+// left to one thread
 __global__ void k_row_bound(const int32_t *arp, const int32_t *acol, const int32_t *brp,
                             int32_t m, int32_t n, int64_t *ub) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        int64_t s = 0;
-        for (int32_t e = arp[i]; e < arp[i + 1]; ++e) {
+    const int lane = dev::lane_id();
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
+    for (int64_t i = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; i < m;
+         i += waves) {
+        uint64_t s = 0;
+        for (int32_t e = arp[i] + lane; e < arp[i + 1]; e += 64) {
             const int32_t k = acol[e];
-            s += brp[k + 1] - brp[k];
+            s += (uint64_t)(brp[k + 1] - brp[k]);
         }
-        ub[i] = s == 0 ? 0 : std::min<int64_t>(s, n);
+        s = dev::wave_sum_u64(s);
+        if (lane == 0) ub[i] = s == 0 ? 0 : std::min<int64_t>((int64_t)s, n);
     }
 }
 
