@@ -708,12 +708,17 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
     // pulling reads a row as in-edges: only for a symmetric CSR
     const bool sym = !SSSP && graph_is_symmetric(c, const_cast<Graph *>(g));
     // A pulled level reads W words per vertex and a pushed one W words per
-    // expanded edge, so narrow batches (W <= 4) gain from pulling earlier:
-    // their threshold divisor is four times the option (measured on the 2048^2
-    // lattice: 32 sources 7.7x -> 10.1x the plain rate, 512 sources best at 4)
+    // expanded edge, so narrow batches (W <= 4) on graphs whose rows are all
+    // short gain from pulling earlier: their threshold divisor is four times
+    // the option (2048^2 lattice: 32 sources 7.7x -> 10.1x the plain rate).
+    // On a graph with long rows an early pull is a trap: a hub keeps scanning
+    // its whole row for sources whose frontier has not reached it (RMAT-22, 64
+    // sources: 0.29 -> 3.5 ms per query), so there the option applies as is.
     const int64_t div_opt = opt(c, "batch_pull_div", 4);
+    const bool short_rows = sym && graph_max_degree(c, const_cast<Graph *>(g)) <= 64;
     const int32_t pull_div =
-        sym ? (int32_t)std::min<int64_t>(div_opt * (W <= 4 ? 4 : 1), 1 << 22) : 0;
+        sym ? (int32_t)std::min<int64_t>(div_opt * (W <= 4 && short_rows ? 4 : 1), 1 << 22)
+            : 0;
     if (!SSSP) {
         k_batch_vis_init<W><<<grid, kBlock, 0, c->stream>>>(g->rowptr, m, nsrc, sym, vis);
     } else {

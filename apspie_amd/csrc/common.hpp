@@ -61,6 +61,7 @@ struct Graph {
     // derived, built lazily and cached with the graph
     int32_t *lowcnt = nullptr;  // per row: number of cols < row (TC)
     uint32_t *iso_bits = nullptr;  // bit per vertex without entries (BFS visited start)
+    int32_t max_deg = -1;          // longest row, once known
     int symmetric = -1;         // 1 / 0 once known (pull traversal needs A == A^T)
     int sym_weights = -1;       // 1 / 0 once known: values equal to the transpose's too
     // rows longer than the SSSP pull's lane limit, as a frontier over those
@@ -161,6 +162,7 @@ bool graph_is_symmetric(Ctx *c, Graph *g);
 // for w(u, v))
 bool graph_is_symmetric_weighted(Ctx *c, Graph *g);
 const uint32_t *graph_iso_bits(Ctx *c, Graph *g);
+int32_t graph_max_degree(Ctx *c, Graph *g);
 
 }  // namespace apspie
 

@@ -22,7 +22,8 @@ static const char *kOptionNames[] = {
     "batch_blocks_per_cu",
     "batch_graph",       // 1 = replay batched level chunks from a captured hipGraph
     "batch_pull_div",    // batched BFS pulls a level when union-frontier edges * this > nnz
-                         // (times 4 for batches of up to 128 sources; 0 = never)
+                         // (times 4 for batches of up to 128 sources on graphs whose rows
+                         // are at most 64 long; 0 = never)
     "batch_sssp_delta",  // batched SSSP near-far threshold step (0 = Bellman-Ford frontier)
     "batch_snapshot_mb", // HBM for the bit rows of pulled batched BFS levels (0 = none)
     "profiling",         // 1 = time the dominant kernel with hipEvents

@@ -229,6 +229,33 @@ __global__ void k_iso_bits(const int32_t *rowptr, int32_t m, uint32_t *bits) {
     }
 }
 
+__global__ void k_max_degree(const int32_t *rowptr, int32_t m, int32_t *out) {
+    int32_t best = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * blockDim.x)
+        best = max(best, rowptr[i + 1] - rowptr[i]);
+    for (int off = 32; off > 0; off >>= 1) best = max(best, __shfl_xor(best, off, 64));
+    if (dev::lane_id() == 0 && best > 0) atomicMax(out, best);
+}
+
+// Longest row, computed once per graph.
+int32_t graph_max_degree(Ctx *c, Graph *g) {
+    std::lock_guard<std::mutex> lock(g->derived_mu);
+    if (g->max_deg >= 0) return g->max_deg;
+    int32_t *d = (int32_t *)scratch(c, "max_deg", 256);
+    fill_i32(c, d, 0, 1);
+    if (g->m > 0) {
+        k_max_degree<<<(int)std::min<int64_t>(ceil_div(g->m, kBlock), 4 * c->cus), kBlock, 0,
+                       c->stream>>>(g->rowptr, g->m, d);
+        APSPIE_LAUNCH_CHECK();
+    }
+    int32_t h = 0;
+    APSPIE_HIP(hipMemcpyAsync(&h, d, 4, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    g->max_deg = h;
+    return h;
+}
+
 // Bitmap of the vertices without entries, built once per graph.
 const uint32_t *graph_iso_bits(Ctx *c, Graph *g) {
     std::lock_guard<std::mutex> lock(g->derived_mu);
