@@ -314,7 +314,7 @@ void graph_prepare_heavy(Ctx *c, Graph *g) {
 
 void sssp_do(Ctx *c, const Graph *g0, int32_t source, float *dist_out, int32_t pull_div) {
     Graph *g = const_cast<Graph *>(g0);
-    graph_prepare_heavy(c, g);
+    if (pull_div > 0) graph_prepare_heavy(c, g);  // only pulled rounds walk the long-row list
     FrontierSet fs = frontier_alloc(c, "sssp", g->m, g->nnz, kDoTile, true);
     int32_t *dist = (int32_t *)dist_out;
     const int64_t words = g->m / 32 + 1;
