@@ -80,6 +80,26 @@ def test_bfs_batch_level_snapshots(ctx, key, snap_mb):
         ctx.set_option("batch_pull_div", 4)
 
 
+@pytest.mark.parametrize("nsrc", [3, 40, 140])
+def test_batch_isolated_and_unreached_sources(ctx, nsrc):
+    """Sources that are isolated vertices (marked visited up front in a
+    symmetric graph) reach only themselves; vertices of other components stay
+    -1 / unreached; a source repeated in the batch gets the same row."""
+    g = T.random_graph(3000, 0.0008, 13)  # several components, isolated vertices
+    iso = np.nonzero(g.deg == 0)[0]
+    assert len(iso) >= 2
+    conn = T.sources(g, nsrc, seed=19)
+    src = (list(iso[:2]) + conn)[:nsrc]
+    src[-1] = src[0]  # a repeated (isolated) source
+    G = ctx.upload(g.rowptr32, g.col, g.w)
+    got = ctx.bfs_batch(G, src)
+    dist = ctx.sssp_batch(G, src)
+    for i, s in enumerate(src):
+        assert np.array_equal(got[i], O.bfs(g.m, g.rowptr, g.col, s)[0]), (i, s)
+        assert np.array_equal(dist[i], oracle_sssp(g, s)), (i, s)
+    G.close()
+
+
 def test_bfs_batch_directed_graph_pushes(ctx):
     """A non-symmetric CSR is never pulled (rows are out-edges only)."""
     rng = np.random.default_rng(9)
