@@ -112,6 +112,7 @@ def _declare(L):
         "apspie_spmspv": ([vp, vp, C.c_int, i64, vp, vp, vp, C.POINTER(i64), vp, vp, C.c_int,
                            C.c_int], C.c_int),
         "apspie_spgemm": ([vp, vp, vp, pp], C.c_int),
+        "apspie_mxm": ([vp, vp, vp, pp], C.c_int),
         "apspie_spgemm_masked": ([vp, vp, vp, vp, vp, C.POINTER(i64), C.c_int], C.c_int),
         "apspie_scan_exclusive": ([vp, vp, vp, i64, C.POINTER(i64), C.c_int], C.c_int),
         "apspie_sort_pairs": ([vp, vp, vp, i64, C.c_int, C.c_int], C.c_int),
@@ -324,6 +325,14 @@ class Context:
     def spgemm(self, a, b):
         h = C.c_void_p()
         _check(self._L.apspie_spgemm(self.h, a.h, b.h, C.byref(h)), self.h)
+        return Graph(self, h)
+
+    def mxm(self, a_csc, b_csc):
+        """C = A * B column by column (mXm): a_csc / b_csc are graphs whose
+        rowptr/col/val hold A's and B's CSC arrays (m = columns, n = rows);
+        the result is C in the same form."""
+        h = C.c_void_p()
+        _check(self._L.apspie_mxm(self.h, a_csc.h, b_csc.h, C.byref(h)), self.h)
         return Graph(self, h)
 
     def spgemm_masked(self, a, b, mask):

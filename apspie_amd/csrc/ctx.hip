@@ -668,6 +668,29 @@ int apspie_spgemm(apspie_ctx *c, const apspie_graph *a, const apspie_graph *b,
     });
 }
 
+// Column j of C = A * B is A * B[:,j], one sparse mXv per column of B
+// (mXm tc.cuh:314-375).  All columns advance at once as one frontier matrix:
+// a CSC array pair read as CSR is the transpose, so C^T = B^T * A^T is the
+// row-wise hash product of the two graphs in swapped order, and its CSR is
+// C's CSC with the rows of each column sorted.
+int apspie_mxm(apspie_ctx *c, const apspie_graph *a_csc, const apspie_graph *b_csc,
+               apspie_graph **c_csc) {
+    if (!c || !a_csc || !b_csc || !c_csc) return APSPIE_ERR_INVALID;
+    *c_csc = nullptr;
+    return guard(c, [&] {
+        check_graph(c, a_csc);
+        check_graph(c, b_csc);
+        // a_csc holds cols(A) columns over rows(A); b_csc cols(B) over rows(B)
+        APSPIE_CHECK(b_csc->n == a_csc->m, APSPIE_ERR_INVALID,
+                     "rows of B differ from columns of A");
+        set_device(c);
+        Graph *r = nullptr;
+        timed(c, [&] { r = spgemm_run(c, b_csc, a_csc); });
+        c->graphs.insert(r);
+        *c_csc = static_cast<apspie_graph *>(r);
+    });
+}
+
 int apspie_spgemm_masked(apspie_ctx *c, const apspie_graph *a, const apspie_graph *b,
                          const apspie_graph *mask, float *c_val, int64_t *c_nnz, int mem) {
     if (!c || !a || !b || !mask || !c_nnz) return APSPIE_ERR_INVALID;

@@ -191,6 +191,15 @@ int apspie_spmspv(apspie_ctx *ctx, const apspie_graph *g, int semiring, int64_t 
  * sorted rows. */
 int apspie_spgemm(apspie_ctx *ctx, const apspie_graph *a, const apspie_graph *b,
                   apspie_graph **c);
+/* C = A * B over (+,x) in CSC, column by column: column j of C is the sparse
+ * mXv A * B[:,j] (mXm, tc.cuh:314-375, over mXv mXv.cuh:249-389).  a_csc and
+ * b_csc hold CSC arrays (their rowptr/col/val are column offsets, row indices
+ * and values; a graph's m is then the column count and n the row count);
+ * *c_csc is a new device graph in the same form, rows sorted within each
+ * column, entries kept where their products cancel to 0.  All columns of B
+ * run together through the SpGEMM kernels instead of one launch each. */
+int apspie_mxm(apspie_ctx *ctx, const apspie_graph *a_csc, const apspie_graph *b_csc,
+               apspie_graph **c_csc);
 /* Masked product C = (A * B) o M over (+,x), evaluated on M's pattern only:
  * c_val[e] = sum_k A[i,k] * B[k,j] for every entry e = (i,j) of M in M's CSR
  * order (0 where no product lands; a column repeated within a row of M gets

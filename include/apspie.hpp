@@ -283,6 +283,27 @@ int mXv(const int *d_xInd, const T *d_xVal, int edge, int m, int nnz_x, const T 
     return (int)ny;
 }
 
+// mXm (tc.cuh:314-316): C = A * B over (+,x) with m x m A and B in CSC on the
+// device, column j of C being mXv of A with column j of B; edge = nnz(A),
+// nnz(B) = h_cscColPtrB[m].  The reference allocates C's arrays inside and
+// does not hand them back; here C comes back in a d_matrix whose
+// d_cscColPtr / d_cscRowInd / d_cscVal are C's column offsets, row indices
+// (sorted within each column) and values.  Returns nnz(C).
+template <typename T>
+int mXm(int edge, int m, const T *d_cscValA, const int *d_cscColPtrA, const int *d_cscRowIndA,
+        const T *d_cscValB, const int *h_cscColPtrB, const int *d_cscColPtrB,
+        const int *d_cscRowIndB, d_matrix *C, Context &ctx) {
+    static_assert(sizeof(T) == sizeof(float), "values are FP32");
+    GraphView a(ctx, m, m, edge, d_cscColPtrA, d_cscRowIndA,
+                reinterpret_cast<const float *>(d_cscValA));
+    GraphView b(ctx, m, m, h_cscColPtrB[m], d_cscColPtrB, d_cscRowIndB,
+                reinterpret_cast<const float *>(d_cscValB));
+    apspie_graph *c = nullptr;
+    check(apspie_mxm(ctx.get(), a.get(), b.get(), &c), ctx.get(), "mXm");
+    attach(C, c, ctx);
+    return C->nnz;
+}
+
 }  // namespace apspie
 
 #endif

@@ -672,3 +672,74 @@ int64_t or_spgemm(int32_t m, int32_t k, int32_t n,
     free(acc); free(mark); free(touched);
     return nnz;
 }
+
+/* ======================================================================== */
+/* Column-by-column SpMSpM (mXm, tc.cuh:314-375): for each column j of B,    */
+/* one sparse mXv (mXv.cuh:249-389) of A in CSC with that column as x:       */
+/* expand every (k, B[k,j]) over column k of A into (row r, B[k,j]*A[r,k])  */
+/* in column order (IntervalExpand/Gather, :300-307; elementMult :105), a    */
+/* stable sort of the pairs by row (SortPairs, :324) and a reduce by key     */
+/* with + (ReduceByKey, :332) -- a key whose products cancel stays, with     */
+/* value 0.  Column j of C holds that column's output (the running total_nnz */
+/* of tc.cuh:352-355 is the prefix of the per-column output counts).        */
+/* The sums run sequentially in sorted order; the reference's segmented      */
+/* reduce associates differently, which is exact on integer-valued inputs.   */
+/* ======================================================================== */
+typedef struct { int32_t row; int32_t pos; float val; } or_prod;
+
+static int prod_cmp(const void *x, const void *y) {
+    const or_prod *a = (const or_prod *)x, *b = (const or_prod *)y;
+    if (a->row != b->row) return a->row < b->row ? -1 : 1;
+    return a->pos < b->pos ? -1 : (a->pos > b->pos); /* stable: keep expansion order */
+}
+
+int64_t or_mxm(int32_t p, int32_t q, int32_t r,
+               const int64_t *a_colptr, const int32_t *a_row, const float *a_val,
+               const int64_t *b_colptr, const int32_t *b_row, const float *b_val,
+               int64_t *c_colptr, int32_t *c_row, float *c_val) {
+    (void)p; (void)q;
+    int64_t cap = 1;
+    or_prod *buf = (or_prod *)malloc(sizeof(or_prod));
+    if (!buf) return -1;
+    int64_t nnz = 0;
+    c_colptr[0] = 0;
+    for (int32_t j = 0; j < r; ++j) {
+        int64_t total = 0;
+        for (int64_t f = b_colptr[j]; f < b_colptr[j + 1]; ++f) {
+            const int32_t k = b_row[f];
+            total += a_colptr[k + 1] - a_colptr[k];
+        }
+        if (total > cap) {
+            free(buf);
+            cap = total;
+            buf = (or_prod *)malloc(sizeof(or_prod) * (size_t)cap);
+            if (!buf) return -1;
+        }
+        int64_t t = 0;
+        for (int64_t f = b_colptr[j]; f < b_colptr[j + 1]; ++f) {
+            const int32_t k = b_row[f];
+            const float xv = b_val ? b_val[f] : 1.0f;
+            for (int64_t e = a_colptr[k]; e < a_colptr[k + 1]; ++e) {
+                buf[t].row = a_row[e];
+                buf[t].pos = (int32_t)t;
+                buf[t].val = xv * (a_val ? a_val[e] : 1.0f);
+                ++t;
+            }
+        }
+        qsort(buf, (size_t)t, sizeof(or_prod), prod_cmp);
+        for (int64_t s = 0; s < t;) {
+            int64_t e = s + 1;
+            float sum = buf[s].val;
+            while (e < t && buf[e].row == buf[s].row) sum = sum + buf[e++].val;
+            if (c_row) {
+                c_row[nnz] = buf[s].row;
+                if (c_val) c_val[nnz] = sum;
+            }
+            ++nnz;
+            s = e;
+        }
+        c_colptr[j + 1] = nnz;
+    }
+    free(buf);
+    return nnz;
+}

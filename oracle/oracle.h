@@ -135,6 +135,15 @@ int64_t or_spgemm(int32_t m, int32_t k, int32_t n,
                   const int64_t *b_rowptr, const int32_t *b_col, const float *b_val,
                   int64_t *c_rowptr, int32_t *c_col, float *c_val);
 
+/* Column-by-column C = A*B in CSC (mXm, tc.cuh:314-375, one sparse mXv per
+ * column of B): A is p x q, B is q x r, both CSC; rows of each column of C
+ * sorted, cancelled sums kept as 0.  Two-call protocol as or_spgemm
+ * (c_row == NULL: c_colptr and the nnz only). */
+int64_t or_mxm(int32_t p, int32_t q, int32_t r,
+               const int64_t *a_colptr, const int32_t *a_row, const float *a_val,
+               const int64_t *b_colptr, const int32_t *b_row, const float *b_val,
+               int64_t *c_colptr, int32_t *c_row, float *c_val);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,6 +74,8 @@ def _declare(L):
     L.or_spmspv.restype = i64
     L.or_spgemm.argtypes = [i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.or_spgemm.restype = i64
+    L.or_mxm.argtypes = [i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.or_mxm.restype = i64
 
 
 # --------------------------------------------------------------------------- RNG
@@ -265,6 +267,28 @@ def spgemm(m, k, n, a_rowptr, a_col, a_val, b_rowptr, b_col, b_val):
     L.or_spgemm(m, k, n, _p(a_rowptr), _p(a_col), _p(a_val), _p(b_rowptr), _p(b_col),
                 _p(b_val), _p(c_rowptr), _p(c_col), _p(c_val))
     return c_rowptr, c_col, c_val
+
+
+def mxm(p, q, r, a_colptr, a_row, a_val, b_colptr, b_row, b_val):
+    """C = A*B column by column (mXm): A p x q and B q x r in CSC; returns C's
+    (colptr int64, row indices, values) in CSC."""
+    L = lib()
+    a_colptr = np.ascontiguousarray(a_colptr, np.int64)
+    b_colptr = np.ascontiguousarray(b_colptr, np.int64)
+    a_row = np.ascontiguousarray(a_row, np.int32)
+    b_row = np.ascontiguousarray(b_row, np.int32)
+    a_val = None if a_val is None else np.ascontiguousarray(a_val, np.float32)
+    b_val = None if b_val is None else np.ascontiguousarray(b_val, np.float32)
+    c_colptr = np.empty(r + 1, np.int64)
+    nnz = L.or_mxm(p, q, r, _p(a_colptr), _p(a_row), _p(a_val), _p(b_colptr), _p(b_row),
+                   _p(b_val), _p(c_colptr), None, None)
+    if nnz < 0:
+        raise MemoryError("or_mxm")
+    c_row = np.empty(nnz, np.int32)
+    c_val = np.empty(nnz, np.float32)
+    L.or_mxm(p, q, r, _p(a_colptr), _p(a_row), _p(a_val), _p(b_colptr), _p(b_row), _p(b_val),
+             _p(c_colptr), _p(c_row), _p(c_val))
+    return c_colptr, c_row, c_val
 
 
 def lower(m, rowptr, col):

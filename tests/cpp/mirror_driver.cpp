@@ -91,6 +91,26 @@ int main(int argc, char **argv) {
         std::printf("spgemm_nnz %d\n", C.nnz);
         apspie::matrix_delete(&C);
 
+        // mXm (tc.cuh:314): the weighted matrix times itself, column by column
+        apspie::d_matrix D;
+        apspie::matrix_new(&D, A.m, A.m);
+        const int dn = apspie::mXm<float>(A.nnz, A.m, d_w, A.d_cscColPtr, A.d_cscRowInd, d_w,
+                                          A.h_cscColPtr, A.d_cscColPtr, A.d_cscRowInd, &D, ctx);
+        std::vector<int> dcol(dn);
+        std::vector<float> dval(dn);
+        apspie::check(apspie_memcpy(ctx.get(), dcol.data(), D.d_cscRowInd, 4 * (int64_t)dn, 1),
+                      ctx.get(), "copy");
+        apspie::check(apspie_memcpy(ctx.get(), dval.data(), D.d_cscVal, 4 * (int64_t)dn, 1),
+                      ctx.get(), "copy");
+        std::printf("mxm_colptr");
+        for (int i = 0; i <= A.m; ++i) std::printf(" %d", D.h_cscColPtr[i]);
+        std::printf("\nmxm_rows");
+        for (int i = 0; i < dn; ++i) std::printf(" %d", dcol[i]);
+        std::printf("\nmxm_vals");
+        for (int i = 0; i < dn; ++i) std::printf(" %.9g", dval[i]);
+        std::printf("\n");
+        apspie::matrix_delete(&D);
+
         apspie_dev_free(ctx.get(), d_w, 4 * (int64_t)A.nnz);
         apspie_dev_free(ctx.get(), d_out, 4 * (int64_t)A.m);
         apspie_dev_free(ctx.get(), d_dist, 4 * (int64_t)A.m);

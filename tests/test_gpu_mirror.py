@@ -46,3 +46,7 @@ def test_mirror_driver_matches_oracle(name):
     assert [int(x) for x in out["tc_rows"]] == rows.tolist()
     c_rp, _, _ = O.spgemm(g.m, g.m, g.m, g.rowptr, g.col, None, g.rowptr, g.col, None)
     assert int(out["spgemm_nnz"][0]) == int(c_rp[-1])
+    cp, ri, cv = O.mxm(g.m, g.m, g.m, g.rowptr, g.col, g.w, g.rowptr, g.col, g.w)
+    assert [int(x) for x in out["mxm_colptr"]] == cp.tolist()
+    assert [int(x) for x in out["mxm_rows"]] == ri.tolist()
+    assert np.array_equal(np.array([float(x) for x in out["mxm_vals"]], np.float32), cv)

@@ -146,6 +146,95 @@ def test_spgemm_vs_oracle(ctx, case):
         G.close()
 
 
+def _csc_graph(ctx, M):
+    M = sp.csc_matrix(M)
+    M.sort_indices()
+    return ctx.upload(M.indptr.astype(np.int32), M.indices.astype(np.int32),
+                      M.data.astype(np.float32), n=M.shape[0])
+
+
+@pytest.mark.parametrize("case", ["rect_cancel", "empty_cols", "rmat10_weighted", "wide_col"])
+def test_mxm_vs_oracle(ctx, case):
+    """Column-by-column C = A*B in CSC (mXm) against the oracle's column loop:
+    non-square operands with signed values (sums that cancel keep their entry),
+    empty columns on both sides, a weighted R-MAT squared, and a column of B
+    whose products spill past the LDS tables."""
+    rng = np.random.default_rng(11)
+    if case == "rect_cancel":
+        Am = sp.random(300, 200, 0.05, random_state=1, format="csc")
+        Bm = sp.random(200, 400, 0.05, random_state=2, format="csc")
+        vals = lambda k: rng.integers(-3, 4, k)
+    elif case == "empty_cols":
+        Am = sp.random(120, 90, 0.04, random_state=3, format="lil")
+        Am[:, 10:30] = 0
+        Bm = sp.random(90, 150, 0.05, random_state=4, format="lil")
+        Bm[:, 50:80] = 0
+        Am, Bm = sp.csc_matrix(Am), sp.csc_matrix(Bm)
+        vals = lambda k: rng.integers(1, 6, k)
+    elif case == "rmat10_weighted":
+        g = T.rmat(10)
+        w = O.assign_weights(g.m, g.rowptr, g.col)
+        Am = sp.csr_matrix((w, g.col, g.rowptr), shape=(g.m, g.m)).tocsc()
+        Bm = Am
+        vals = None
+    else:
+        Am = sp.random(6000, 300, 0.02, random_state=5, format="csc")
+        Bm = sp.lil_matrix((300, 40))
+        Bm[:, 0] = 1
+        Bm[::2, 1] = 1
+        Bm = sp.csc_matrix(Bm)
+        vals = lambda k: rng.integers(1, 6, k)
+    if vals is not None:
+        for M in (Am, Bm):
+            M.data = vals(M.nnz).astype(np.float64)
+    Ga, Gb = _csc_graph(ctx, Am), _csc_graph(ctx, Bm)
+    Gc = ctx.mxm(Ga, Gb)
+    assert (Gc.m, Gc.n) == (Bm.shape[1], Am.shape[0])
+    cp, ri, cv = Gc.download(values=True)
+    A_ = sp.csc_matrix(Am); A_.sort_indices()
+    B_ = sp.csc_matrix(Bm); B_.sort_indices()
+    wcp, wri, wcv = O.mxm(Am.shape[0], Am.shape[1], Bm.shape[1], A_.indptr, A_.indices,
+                          A_.data.astype(np.float32), B_.indptr, B_.indices,
+                          B_.data.astype(np.float32))
+    assert np.array_equal(cp, wcp.astype(np.int32))
+    assert np.array_equal(ri, wri) and np.array_equal(cv, wcv)
+    if case == "rect_cancel":
+        assert (cv == 0).any()
+    for G in (Ga, Gb, Gc):
+        G.close()
+
+
+def test_mxm_columns_equal_spmspv(ctx):
+    """The bridge between the two engines: each column of mXm equals the SpMSpV
+    of A's CSC with that column of B, computed on the device by apspie_spmspv."""
+    g = T.rmat(9)
+    rng = np.random.default_rng(12)
+    a = rng.integers(1, 9, len(g.col)).astype(np.float32)
+    Ga = ctx.upload(g.rowptr.astype(np.int32), g.col, a)
+    Bm = sp.random(g.m, 24, 0.05, random_state=7, format="csc")
+    Bm.data = rng.integers(1, 9, Bm.nnz).astype(np.float64)
+    Bm.sort_indices()
+    Gb = _csc_graph(ctx, Bm)
+    Gc = ctx.mxm(Ga, Gb)
+    cp, ri, cv = Gc.download(values=True)
+    for j in range(24):
+        s, e = Bm.indptr[j], Bm.indptr[j + 1]
+        yi, yv = ctx.spmspv(Ga, A.SR_PLUS_TIMES, Bm.indices[s:e], Bm.data[s:e])
+        assert np.array_equal(ri[cp[j]:cp[j + 1]], yi)
+        assert np.array_equal(cv[cp[j]:cp[j + 1]], yv)
+    for G in (Ga, Gb, Gc):
+        G.close()
+
+
+def test_mxm_rejects_bad_shapes(ctx):
+    Ga = _csc_graph(ctx, sp.random(5, 4, 0.5, random_state=1))   # A is 5 x 4
+    Gb = _csc_graph(ctx, sp.random(3, 6, 0.5, random_state=2))   # B is 3 x 6
+    with pytest.raises(A.ApspieError):
+        ctx.mxm(Ga, Gb)
+    for G in (Ga, Gb):
+        G.close()
+
+
 def _masked_want(Am, Bm, Mm):
     """The oracle's full product, read at the mask's entries."""
     wrp, wc, wv = O.spgemm(Am.shape[0], Am.shape[1], Bm.shape[1], Am.indptr.astype(np.int64),

@@ -339,6 +339,69 @@ def test_spgemm_vs_scipy():
         assert vv.tolist() == Cm.data.astype(np.float32).tolist()
 
 
+def _csc(M):
+    M = sp.csc_matrix(M)
+    M.sort_indices()
+    return (M.indptr.astype(np.int64), M.indices.astype(np.int32),
+            M.data.astype(np.float32))
+
+
+def test_mxm_vs_scipy_with_cancellation():
+    """Column-by-column C = A*B: the pattern is that of |A|*|B| (a sum that
+    cancels keeps its entry, value 0, as the reduce by key does), the values
+    are scipy's; A is not square and not symmetric, so a transposed or
+    swapped operand fails."""
+    rng = np.random.default_rng(4)
+    for seed in range(3):
+        A = sp.random(50, 40, 0.1, random_state=seed, format="csc")
+        B = sp.random(40, 65, 0.12, random_state=seed + 5, format="csc")
+        A.data = rng.integers(-3, 4, A.nnz).astype(np.float64)  # zeros and signs: cancels
+        B.data = rng.integers(-3, 4, B.nnz).astype(np.float64)
+        cp, ri, cv = O.mxm(50, 40, 65, *_csc(A), *_csc(B))
+        ones = lambda M: sp.csc_matrix((np.ones(M.nnz), M.indices, M.indptr), shape=M.shape)
+        pat = (ones(A) @ ones(B)).tocsc()  # structural: stored zeros count
+        pat.sort_indices()
+        dense = (A @ B).toarray()
+        assert cp.tolist() == pat.indptr.tolist()
+        assert ri.tolist() == pat.indices.tolist()
+        want = np.array([dense[r, j] for j in range(65)
+                         for r in pat.indices[pat.indptr[j]:pat.indptr[j + 1]]], np.float32)
+        assert cv.tolist() == want.tolist()
+        assert (cv == 0).any()  # the cancellation case occurred
+
+
+def test_mxm_hand_cases():
+    # [1 1] * [1; -1]: one product pair that cancels -> one entry, value 0
+    cp, ri, cv = O.mxm(1, 2, 1, np.array([0, 1, 2]), np.array([0, 0]), np.array([1.0, 1.0]),
+                       np.array([0, 2]), np.array([0, 1]), np.array([1.0, -1.0]))
+    assert cp.tolist() == [0, 1] and ri.tolist() == [0] and cv.tolist() == [0.0]
+    # identity times B is B; an empty column of B gives an empty column of C
+    B = sp.csc_matrix(np.array([[0, 2, 0], [5, 0, 0], [0, 7, 0]], float))
+    cp, ri, cv = O.mxm(3, 3, 3, *_csc(sp.identity(3)), *_csc(B))
+    assert cp.tolist() == B.indptr.tolist() and ri.tolist() == B.indices.tolist()
+    assert cv.tolist() == B.data.tolist()
+
+
+def test_mxm_bridges_spmspv_and_spgemm():
+    """Column j of the column-by-column product is the SpMSpV of A's CSC with
+    column j of B (mXm's loop body), and the whole product read as CSR is the
+    row-wise SpGEMM of the transposes, B^T * A^T."""
+    m, rp, c = random_graph(80, 0.07, 6)
+    rng = np.random.default_rng(6)
+    a = rng.integers(1, 9, len(c)).astype(np.float32)
+    B = sp.random(80, 30, 0.1, random_state=2, format="csc")
+    B.data = rng.integers(1, 9, B.nnz).astype(np.float64)
+    bp, bi, bv = _csc(B)
+    cp, ri, cv = O.mxm(80, 80, 30, rp, c, a, bp, bi, bv)
+    for j in range(30):
+        yi, yv = O.spmspv(m, rp, c, a, O.SR_PLUS_TIMES, bi[bp[j]:bp[j + 1]], bv[bp[j]:bp[j + 1]])
+        assert ri[cp[j]:cp[j + 1]].tolist() == yi.tolist()
+        assert cv[cp[j]:cp[j + 1]].tolist() == yv.tolist()
+    srp, sc, sv = O.spgemm(30, 80, 80, bp, bi, bv, rp, c, a)
+    assert srp.tolist() == cp.tolist() and sc.tolist() == ri.tolist()
+    assert sv.tolist() == cv.tolist()
+
+
 # ----------------------------------------------- planted mistakes are caught
 def test_planted_bfs_mistake_is_caught():
     """An off-by-one in depth (a plausible bug) must fail the scipy pin."""
