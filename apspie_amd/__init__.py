@@ -113,6 +113,8 @@ def _declare(L):
                            C.c_int], C.c_int),
         "apspie_spgemm": ([vp, vp, vp, pp], C.c_int),
         "apspie_mxm": ([vp, vp, vp, pp], C.c_int),
+        "apspie_graph_to_dcsc": ([vp, vp, i32, i32, C.POINTER(i64), vp, vp, vp, vp, vp, C.c_int],
+                                 C.c_int),
         "apspie_spgemm_masked": ([vp, vp, vp, vp, vp, C.POINTER(i64), C.c_int], C.c_int),
         "apspie_scan_exclusive": ([vp, vp, vp, i64, C.POINTER(i64), C.c_int], C.c_int),
         "apspie_sort_pairs": ([vp, vp, vp, i64, C.c_int, C.c_int], C.c_int),
@@ -447,6 +449,23 @@ class Graph:
         _check(lib().apspie_graph_download(self.ctx.h, self.h, _p(rowptr), _p(col), _p(val)),
                self.ctx.h)
         return rowptr, col[:self.nnz], (None if val is None else val[:self.nnz])
+
+    def to_dcsc(self, part_size, part_num, values=True):
+        """Row-block DCSC (csr_to_dcsc): (part_ptr, col_ind, col_off, row_ind,
+        val or None) as host arrays."""
+        e = max(self.nnz, 1)
+        pp = np.empty(part_num + 1, np.int32)
+        ci = np.empty(e, np.int32)
+        co = np.empty(e + 1, np.int32)
+        ri = np.empty(e, np.int32)
+        vv = np.empty(e, np.float32) if values else None
+        L = C.c_int64()
+        _check(lib().apspie_graph_to_dcsc(self.ctx.h, self.h, part_size, part_num, C.byref(L),
+                                          _p(pp), _p(ci), _p(co), _p(ri), _p(vv), HOST),
+               self.ctx.h)
+        n = L.value
+        return (pp, ci[:n], co[:n + 1], ri[:self.nnz],
+                None if vv is None else vv[:self.nnz])
 
     def transpose(self):
         h = C.c_void_p()

@@ -153,6 +153,8 @@ void sssp_batch_run(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsr
 void graph_from_coo_dev(Ctx *c, Graph *g, const int32_t *row, const int32_t *col,
                         const float *val, int64_t nnz, int symmetrize);
 void graph_transpose_dev(Ctx *c, const Graph *g, Graph *t);
+int64_t graph_to_dcsc_dev(Ctx *c, const Graph *g, int32_t ps, int32_t pn, int32_t *part_ptr,
+                          int32_t *col_ind, int32_t *col_off, int32_t *row_ind, float *val);
 void mt19937_priorities(Ctx *c, uint32_t seed, int32_t m, uint32_t *out_dev);
 void fill_i32(Ctx *c, int32_t *p, int32_t v, int64_t n);
 void graph_prepare_tc(Ctx *c, Graph *g);

@@ -501,11 +501,52 @@ int apspie_graph_download(apspie_ctx *c, const apspie_graph *g, int32_t *rowptr,
     });
 }
 
+int apspie_graph_to_dcsc(apspie_ctx *c, const apspie_graph *g, int32_t part_size,
+                         int32_t part_num, int64_t *col_length, int32_t *part_ptr,
+                         int32_t *col_ind, int32_t *col_off, int32_t *row_ind, float *val,
+                         int mem) {
+    if (!c || !g || !col_length || !part_ptr || !col_off) return APSPIE_ERR_INVALID;
+    return guard(c, [&] {
+        check_graph(c, g);
+        APSPIE_CHECK(part_size >= 1 && part_num >= 1, APSPIE_ERR_INVALID,
+                     "part_size and part_num must be positive");
+        APSPIE_CHECK((int64_t)(part_num - 1) * part_size <= g->m, APSPIE_ERR_INVALID,
+                     "a part starts past the last row");
+        APSPIE_CHECK(g->nnz == 0 || (col_ind && row_ind), APSPIE_ERR_INVALID,
+                     "col_ind / row_ind are NULL");
+        APSPIE_CHECK(g->nnz == 0 || !val || g->val, APSPIE_ERR_INVALID, "graph has no values");
+        set_device(c);
+        const int64_t E = std::max<int64_t>(g->nnz, 1);
+        const bool host = mem == APSPIE_MEM_HOST;
+        const bool want_val = val && g->val;
+        int32_t *pp = host ? (int32_t *)scratch(c, "dco_pp", sizeof(int32_t) * (part_num + 1))
+                           : part_ptr;
+        int32_t *ci = host ? (int32_t *)scratch(c, "dco_ci", sizeof(int32_t) * E) : col_ind;
+        int32_t *co = host ? (int32_t *)scratch(c, "dco_co", sizeof(int32_t) * (E + 1)) : col_off;
+        int32_t *ri = host ? (int32_t *)scratch(c, "dco_ri", sizeof(int32_t) * E) : row_ind;
+        float *vv = !want_val ? nullptr
+                    : host    ? (float *)scratch(c, "dco_v", sizeof(float) * E)
+                              : val;
+        int64_t L = 0;
+        timed(c, [&] { L = graph_to_dcsc_dev(c, g, part_size, part_num, pp, ci, co, ri, vv); });
+        if (host) {
+            copy_from_device(c, part_ptr, pp, sizeof(int32_t) * (part_num + 1), mem);
+            copy_from_device(c, col_off, co, sizeof(int32_t) * (L + 1), mem);
+            if (L) copy_from_device(c, col_ind, ci, sizeof(int32_t) * L, mem);
+            if (g->nnz) copy_from_device(c, row_ind, ri, sizeof(int32_t) * g->nnz, mem);
+            if (want_val && g->nnz) copy_from_device(c, val, vv, sizeof(float) * g->nnz, mem);
+        }
+        sync(c);
+        *col_length = L;
+    });
+}
+
 int apspie_graph_transpose(apspie_ctx *c, const apspie_graph *g, apspie_graph **out) {
     if (!c || !g || !out) return APSPIE_ERR_INVALID;
     *out = nullptr;
     apspie_graph *t = new apspie_graph();
     int st = guard(c, [&] {
+        check_graph(c, g);
         set_device(c);
         graph_transpose_dev(c, g, t);
         sync(c);

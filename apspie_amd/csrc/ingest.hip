@@ -123,6 +123,62 @@ void finish_rowptr(Ctx *c, Graph *g, const int32_t *orow) {
     APSPIE_LAUNCH_CHECK();
 }
 
+
+// ---------------------------------------------------- row-block DCSC
+// Part of CSR row r when the rows are cut into blocks of ps, the last of the
+// pn parts taking every row from (pn - 1) * ps on.
+__device__ __forceinline__ uint32_t dcsc_part(int32_t r, int32_t ps, int32_t pn) {
+    return (uint32_t)min(r / ps, pn - 1);
+}
+
+__global__ void k_dcsc_part_keys(const int32_t *rows, const uint32_t *idx, int64_t n, int32_t ps,
+                                 int32_t pn, uint32_t *keys) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        keys[i] = dcsc_part(rows[idx[i]], ps, pn);
+}
+
+// Entry j of the (part, column, row) order: its row and value, and whether
+// it opens a (part, column) segment.
+__global__ void k_dcsc_entries(const int32_t *rows, const int32_t *col, const float *val,
+                               const uint32_t *part, const uint32_t *idx, int64_t n,
+                               int32_t *row_ind, float *val_out, int32_t *opens) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t e = idx[j];
+        row_ind[j] = rows[e];
+        if (val_out) val_out[j] = val[e];
+        opens[j] = j == 0 || part[j] != part[j - 1] || col[e] != col[idx[j - 1]];
+    }
+}
+
+// Segment k starts at off[k]: its column and part; off[L] = n closes the last.
+__global__ void k_dcsc_segments(const int32_t *col, const uint32_t *idx, const uint32_t *part,
+                                int32_t *off, int64_t L, int64_t n, int32_t *col_ind,
+                                uint32_t *seg_part) {
+    const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    for (int64_t k = gid; k < L; k += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t j = off[k];
+        col_ind[k] = col[idx[j]];
+        seg_part[k] = part[j];
+    }
+    if (gid == 0) off[L] = (int32_t)n;
+}
+
+// part_ptr[p] = segments in parts before p (first k with seg_part[k] >= p)
+__global__ void k_dcsc_part_ptr(const uint32_t *seg_part, int64_t L, int32_t pn,
+                                int32_t *part_ptr) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p <= pn;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        int64_t lo = 0, hi = L;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if ((int64_t)seg_part[mid] < p) lo = mid + 1; else hi = mid;
+        }
+        part_ptr[p] = (int32_t)lo;
+    }
+}
+
 }  // namespace
 
 void graph_from_coo_dev(Ctx *c, Graph *g, const int32_t *row, const int32_t *col,
@@ -316,6 +372,47 @@ void graph_transpose_dev(Ctx *c, const Graph *g, Graph *t) {
         APSPIE_LAUNCH_CHECK();
     }
     finish_rowptr(c, t, orow);
+}
+
+
+// Row-block DCSC (csr_to_dcsc, matrix.cpp:286-468): two stable radix passes
+// order the entries by (part, column) with rows ascending inside, the
+// segment openings are compacted into the column offsets, and each part's
+// first segment is found by binary search.  Outputs are device pointers;
+// returns the number of (part, column) segments.
+int64_t graph_to_dcsc_dev(Ctx *c, const Graph *g, int32_t ps, int32_t pn, int32_t *part_ptr,
+                          int32_t *col_ind, int32_t *col_off, int32_t *row_ind, float *val) {
+    const int grid = fill_grid(c, kBlock, 4);
+    const int64_t E = g->nnz, Ea = std::max<int64_t>(E, 1);
+    int64_t L = 0;
+    uint32_t *seg_part = (uint32_t *)scratch(c, "dc_sp", sizeof(uint32_t) * Ea);
+    if (E > 0) {
+        int32_t *rows = (int32_t *)scratch(c, "dc_rows", sizeof(int32_t) * Ea);
+        uint32_t *keys = (uint32_t *)scratch(c, "dc_k", sizeof(uint32_t) * Ea);
+        uint32_t *idx = (uint32_t *)scratch(c, "dc_i", sizeof(uint32_t) * Ea);
+        int32_t *opens = (int32_t *)scratch(c, "dc_open", sizeof(int32_t) * Ea);
+        k_entry_rows<<<grid, kBlock, 0, c->stream>>>(g->rowptr, g->m, E, rows);
+        k_iota_key<<<grid, kBlock, 0, c->stream>>>(g->col, E, keys, idx);
+        APSPIE_LAUNCH_CHECK();
+        radix_sort_pairs(c, keys, idx, E, bits_for(g->n));  // by column, rows kept ascending
+        k_dcsc_part_keys<<<grid, kBlock, 0, c->stream>>>(rows, idx, E, ps, pn, keys);
+        APSPIE_LAUNCH_CHECK();
+        radix_sort_pairs(c, keys, idx, E, bits_for(pn - 1));  // by part, stable
+        k_dcsc_entries<<<grid, kBlock, 0, c->stream>>>(rows, g->col, g->val, keys, idx, E,
+                                                       row_ind, g->val ? val : nullptr, opens);
+        APSPIE_LAUNCH_CHECK();
+        L = compact_flags(c, opens, E, col_off);
+        k_dcsc_segments<<<grid, kBlock, 0, c->stream>>>(g->col, idx, keys, col_off, L, E,
+                                                        col_ind, seg_part);
+    } else {
+        k_dcsc_segments<<<1, 64, 0, c->stream>>>(g->col, nullptr, nullptr, col_off, 0, 0,
+                                                 col_ind, seg_part);
+    }
+    APSPIE_LAUNCH_CHECK();
+    k_dcsc_part_ptr<<<std::max(1, std::min(grid, (pn + kBlock) / kBlock)), kBlock, 0,
+                      c->stream>>>(seg_part, L, pn, part_ptr);
+    APSPIE_LAUNCH_CHECK();
+    return L;
 }
 
 }  // namespace apspie

@@ -153,6 +153,21 @@ int apspie_graph_download(apspie_ctx *ctx, const apspie_graph *g, int32_t *rowpt
                           int32_t *col, float *val);
 /* Transpose (csr2csc, util.cuh:473-511; cusparseScsr2csc). */
 int apspie_graph_transpose(apspie_ctx *ctx, const apspie_graph *g, apspie_graph **out);
+/* Row-block doubly compressed storage (csr_to_dcsc, matrix.cpp:286-468; the
+ * d_dcscPartPtr / d_dcscColPtr_ind / d_dcscColPtr_off / d_dcscRowInd /
+ * d_dcscVal members of d_matrix, matrix.hpp:7-14).  The rows are cut into
+ * part_num parts of part_size rows, the last part taking every row from
+ * (part_num-1)*part_size on (which must not exceed m).  Inside part p the
+ * entries are grouped by column: segments k in [part_ptr[p], part_ptr[p+1])
+ * are its non-empty columns col_ind[k] (ascending), and segment k's entries
+ * [col_off[k], col_off[k+1]) hold their rows (row_ind, ascending) and
+ * values (val; NULL skips them).  *col_length (host) = number of segments.
+ * Capacities: part_ptr part_num+1, col_ind nnz, col_off nnz+1, row_ind and
+ * val nnz; all arrays in `mem`. */
+int apspie_graph_to_dcsc(apspie_ctx *ctx, const apspie_graph *g, int32_t part_size,
+                         int32_t part_num, int64_t *col_length, int32_t *part_ptr,
+                         int32_t *col_ind, int32_t *col_off, int32_t *row_ind, float *val,
+                         int mem);
 
 /* --------------------------------------------------------------- algorithms */
 /* Level-synchronous BFS, depth[v] = hops from source, -1 unreached.

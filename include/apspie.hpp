@@ -125,7 +125,33 @@ struct d_matrix {
     int *d_cscRowInd = nullptr;  // device column indices
     float *d_cscVal = nullptr;   // device values
     apspie_graph *g = nullptr;
+    // row-block DCSC (matrix.hpp:7-14), filled by csr_to_dcsc
+    int part = 0, col_length = 0;
+    int *h_dcscPartPtr = nullptr;     // host copy of d_dcscPartPtr (copy_part)
+    int *d_dcscPartPtr = nullptr;     // part+1
+    int *d_dcscColPtr_ind = nullptr;  // non-empty columns per part (nnz capacity)
+    int *d_dcscColPtr_off = nullptr;  // their entry offsets (nnz+1 capacity)
+    int *d_dcscRowInd = nullptr;      // nnz
+    float *d_dcscVal = nullptr;       // nnz
+    apspie_ctx *dcsc_ctx = nullptr;   // the context the DCSC arrays came from
+    int64_t dcsc_cap = 0;             // entries the DCSC arrays were sized for
 };
+
+inline void dcsc_free(d_matrix *A) {
+    if (!A->dcsc_ctx) return;
+    const int64_t e = A->dcsc_cap;
+    apspie_dev_free(A->dcsc_ctx, A->d_dcscPartPtr, 4 * ((int64_t)A->part + 1));
+    apspie_dev_free(A->dcsc_ctx, A->d_dcscColPtr_ind, 4 * e);
+    apspie_dev_free(A->dcsc_ctx, A->d_dcscColPtr_off, 4 * (e + 1));
+    apspie_dev_free(A->dcsc_ctx, A->d_dcscRowInd, 4 * e);
+    apspie_dev_free(A->dcsc_ctx, A->d_dcscVal, 4 * e);
+    std::free(A->h_dcscPartPtr);
+    A->h_dcscPartPtr = nullptr;
+    A->d_dcscPartPtr = A->d_dcscColPtr_ind = A->d_dcscColPtr_off = A->d_dcscRowInd = nullptr;
+    A->d_dcscVal = nullptr;
+    A->dcsc_ctx = nullptr;
+    A->dcsc_cap = 0;
+}
 
 inline void matrix_new(d_matrix *A, int m, int n) {
     A->m = m;
@@ -135,6 +161,7 @@ inline void matrix_new(d_matrix *A, int m, int n) {
 }
 
 inline void matrix_delete(d_matrix *A) {
+    dcsc_free(A);
     if (A->g) apspie_graph_destroy(A->g);
     std::free(A->h_cscColPtr);
     *A = d_matrix{};
@@ -181,6 +208,47 @@ void buildSymmetricMatrix(d_matrix *A, int numEdge, const int *h_cooRowInd,
     check(apspie_graph_from_coo(ctx.get(), &coo, APSPIE_MEM_HOST, 1, &g), ctx.get(),
           "makeSymmetric");
     attach(A, g, ctx);
+}
+
+// csr_to_dcsc (matrix.cpp:286-289): with alloc the DCSC arrays are sized
+// for partNum parts and A's nnz; without it A's CSR is converted into them
+// (allocating first if that was not done).  The rows are cut into partNum
+// parts of partSize rows, the last taking the rest.
+template <typename typeVal>
+void csr_to_dcsc(d_matrix *A, int partSize, int partNum, Context &ctx, bool alloc = false) {
+    static_assert(sizeof(typeVal) == sizeof(float), "values are FP32");
+    const int64_t e = (int64_t)A->nnz > 0 ? A->nnz : 1;
+    if (alloc || !A->dcsc_ctx || A->part != partNum || A->dcsc_cap < e) {
+        dcsc_free(A);
+        A->part = partNum;
+        A->dcsc_ctx = ctx.get();
+        A->dcsc_cap = e;
+        A->h_dcscPartPtr = static_cast<int *>(std::calloc((size_t)partNum + 1, sizeof(int)));
+        check(apspie_dev_alloc(ctx.get(), 4 * ((int64_t)partNum + 1), (void **)&A->d_dcscPartPtr),
+              ctx.get(), "csr_to_dcsc");
+        check(apspie_dev_alloc(ctx.get(), 4 * e, (void **)&A->d_dcscColPtr_ind), ctx.get(),
+              "csr_to_dcsc");
+        check(apspie_dev_alloc(ctx.get(), 4 * (e + 1), (void **)&A->d_dcscColPtr_off), ctx.get(),
+              "csr_to_dcsc");
+        check(apspie_dev_alloc(ctx.get(), 4 * e, (void **)&A->d_dcscRowInd), ctx.get(),
+              "csr_to_dcsc");
+        check(apspie_dev_alloc(ctx.get(), 4 * e, (void **)&A->d_dcscVal), ctx.get(),
+              "csr_to_dcsc");
+        if (alloc) return;
+    }
+    int64_t len = 0;
+    check(apspie_graph_to_dcsc(ctx.get(), A->g, partSize, partNum, &len, A->d_dcscPartPtr,
+                               A->d_dcscColPtr_ind, A->d_dcscColPtr_off, A->d_dcscRowInd,
+                               A->d_cscVal ? A->d_dcscVal : nullptr, APSPIE_MEM_DEVICE),
+          ctx.get(), "csr_to_dcsc");
+    A->col_length = (int)len;
+}
+
+// copy_part (matrix.cpp:470-473): the part offsets to the host
+inline void copy_part(d_matrix *A) {
+    check(apspie_memcpy(A->dcsc_ctx, A->h_dcscPartPtr, A->d_dcscPartPtr,
+                        4 * ((int64_t)A->part + 1), 1),
+          A->dcsc_ctx, "copy_part");
 }
 
 // csr2csc (util.cuh:473-511)

@@ -743,3 +743,63 @@ int64_t or_mxm(int32_t p, int32_t q, int32_t r,
     free(buf);
     return nnz;
 }
+
+/* ======================================================================== */
+/* Row-block DCSC (csr_to_dcsc, matrix.cpp:286-468), restated step by step: */
+/* part offsets from the row offsets (:313-315), a per-part stable sort of   */
+/* the entries by column (:327, :413), segment ends where the column changes */
+/* (lookRight mXv.cuh:36-41) or a part ends (specialScatter spgemm.cuh:      */
+/* 34-38), an exclusive scan of the ends (:349), their positions (:351) and  */
+/* columns (:358), ends turned into starts (shiftRightAdd spgemm.cuh:27-31,  */
+/* :359-361), part offsets read from the scan at each part start (:369-371)  */
+/* and the rows / values gathered through the sort's permutation (:405-424). */
+/* Parts that start at or past the last entry read the segment count.        */
+/* ======================================================================== */
+typedef struct { int32_t key; int32_t pos; } or_kp;
+
+static int kp_cmp(const void *x, const void *y) {
+    const or_kp *a = (const or_kp *)x, *b = (const or_kp *)y;
+    if (a->key != b->key) return a->key < b->key ? -1 : 1;
+    return a->pos < b->pos ? -1 : (a->pos > b->pos);
+}
+
+int64_t or_dcsc(int32_t m, const int64_t *rowptr, const int32_t *col, const float *val,
+                int32_t ps, int32_t pn, int64_t *part_ptr, int32_t *col_ind,
+                int64_t *col_off, int32_t *row_ind, float *val_out) {
+    const int64_t nnz = rowptr[m];
+    int64_t *off = (int64_t *)malloc(sizeof(int64_t) * ((size_t)pn + 1));
+    or_kp *kp = (or_kp *)malloc(sizeof(or_kp) * (size_t)(nnz ? nnz : 1));
+    int32_t *rows = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
+    unsigned char *end = (unsigned char *)calloc((size_t)(nnz ? nnz : 1), 1);
+    int64_t *scan = (int64_t *)malloc(sizeof(int64_t) * ((size_t)nnz + 1));
+    if (!off || !kp || !rows || !end || !scan) return -1;
+    for (int32_t p = 0; p < pn; ++p) off[p] = rowptr[(int64_t)p * ps];
+    off[pn] = nnz;
+    for (int32_t r = 0; r < m; ++r)
+        for (int64_t e = rowptr[r]; e < rowptr[r + 1]; ++e) rows[e] = r;
+    for (int64_t e = 0; e < nnz; ++e) { kp[e].key = col[e]; kp[e].pos = (int32_t)e; }
+    for (int32_t p = 0; p < pn; ++p)
+        qsort(kp + off[p], (size_t)(off[p + 1] - off[p]), sizeof(or_kp), kp_cmp);
+    for (int64_t i = 0; i + 1 < nnz; ++i) end[i] = kp[i].key != kp[i + 1].key;
+    for (int32_t p = 1; p <= pn; ++p)
+        if (off[p] > 0) end[off[p] - 1] = 1;
+    scan[0] = 0;
+    for (int64_t i = 0; i < nnz; ++i) scan[i + 1] = scan[i] + end[i];
+    const int64_t L = scan[nnz];
+    int64_t k = 0;
+    col_off[0] = 0;
+    for (int64_t i = 0; i < nnz; ++i) {
+        if (!end[i]) continue;
+        col_ind[k] = kp[i].key;
+        col_off[k + 1] = i + 1;
+        ++k;
+    }
+    for (int32_t p = 0; p < pn; ++p) part_ptr[p] = scan[off[p]];
+    part_ptr[pn] = L;
+    for (int64_t i = 0; i < nnz; ++i) {
+        row_ind[i] = rows[kp[i].pos];
+        if (val_out) val_out[i] = val ? val[kp[i].pos] : 1.0f;
+    }
+    free(off); free(kp); free(rows); free(end); free(scan);
+    return L;
+}

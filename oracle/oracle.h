@@ -144,6 +144,15 @@ int64_t or_mxm(int32_t p, int32_t q, int32_t r,
                const int64_t *b_colptr, const int32_t *b_row, const float *b_val,
                int64_t *c_colptr, int32_t *c_row, float *c_val);
 
+/* Row-block DCSC of a CSR (csr_to_dcsc, matrix.cpp:286-468): rows cut into pn
+ * parts of ps rows (the last takes the rest); per part the non-empty columns
+ * col_ind[part_ptr[p]..part_ptr[p+1]) ascending, segment k's rows and values
+ * at [col_off[k], col_off[k+1]).  Capacities: part_ptr pn+1, col_ind nnz,
+ * col_off nnz+1, row_ind / val_out nnz.  Returns the segment count. */
+int64_t or_dcsc(int32_t m, const int64_t *rowptr, const int32_t *col, const float *val,
+                int32_t ps, int32_t pn, int64_t *part_ptr, int32_t *col_ind,
+                int64_t *col_off, int32_t *row_ind, float *val_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -76,6 +76,8 @@ def _declare(L):
     L.or_spgemm.restype = i64
     L.or_mxm.argtypes = [i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.or_mxm.restype = i64
+    L.or_dcsc.argtypes = [i32, vp, vp, vp, i32, i32, vp, vp, vp, vp, vp]
+    L.or_dcsc.restype = i64
 
 
 # --------------------------------------------------------------------------- RNG
@@ -289,6 +291,26 @@ def mxm(p, q, r, a_colptr, a_row, a_val, b_colptr, b_row, b_val):
     L.or_mxm(p, q, r, _p(a_colptr), _p(a_row), _p(a_val), _p(b_colptr), _p(b_row), _p(b_val),
              _p(c_colptr), _p(c_row), _p(c_val))
     return c_colptr, c_row, c_val
+
+
+def dcsc(m, rowptr, col, val, part_size, part_num):
+    """Row-block DCSC (csr_to_dcsc): (part_ptr, col_ind, col_off, row_ind, val)."""
+    if part_size < 1 or part_num < 1 or (part_num - 1) * part_size > m:
+        raise ValueError("parts do not fit the rows")
+    rowptr = np.ascontiguousarray(rowptr, np.int64)
+    col = np.ascontiguousarray(col, np.int32)
+    val = None if val is None else np.ascontiguousarray(val, np.float32)
+    nnz = int(rowptr[-1])
+    pp = np.empty(part_num + 1, np.int64)
+    ci = np.empty(max(nnz, 1), np.int32)
+    co = np.empty(nnz + 1, np.int64)
+    ri = np.empty(max(nnz, 1), np.int32)
+    vo = np.empty(max(nnz, 1), np.float32)
+    L = lib().or_dcsc(m, _p(rowptr), _p(col), _p(val), part_size, part_num, _p(pp), _p(ci),
+                      _p(co), _p(ri), _p(vo))
+    if L < 0:
+        raise MemoryError("or_dcsc")
+    return pp, ci[:L].copy(), co[:L + 1].copy(), ri[:nnz].copy(), vo[:nnz].copy()
 
 
 def lower(m, rowptr, col):

@@ -111,6 +111,28 @@ int main(int argc, char **argv) {
         std::printf("\n");
         apspie::matrix_delete(&D);
 
+        // csr_to_dcsc (spgemm.cuh:138-139 call pattern) + copy_part
+        const int pn = A.m < 4 ? A.m : 4, ps = A.m / pn;
+        apspie::csr_to_dcsc<float>(&A, ps, pn, ctx, true);
+        apspie::csr_to_dcsc<float>(&A, ps, pn, ctx);
+        apspie::copy_part(&A);
+        std::vector<int> cind(A.col_length), coff(A.col_length + 1), rind(A.nnz);
+        apspie::check(apspie_memcpy(ctx.get(), cind.data(), A.d_dcscColPtr_ind,
+                                    4 * (int64_t)A.col_length, 1), ctx.get(), "copy");
+        apspie::check(apspie_memcpy(ctx.get(), coff.data(), A.d_dcscColPtr_off,
+                                    4 * ((int64_t)A.col_length + 1), 1), ctx.get(), "copy");
+        apspie::check(apspie_memcpy(ctx.get(), rind.data(), A.d_dcscRowInd, 4 * (int64_t)A.nnz, 1),
+                      ctx.get(), "copy");
+        std::printf("dcsc_parts %d %d\ndcsc_part_ptr", ps, pn);
+        for (int i = 0; i <= pn; ++i) std::printf(" %d", A.h_dcscPartPtr[i]);
+        std::printf("\ndcsc_col_ind");
+        for (int v : cind) std::printf(" %d", v);
+        std::printf("\ndcsc_col_off");
+        for (int v : coff) std::printf(" %d", v);
+        std::printf("\ndcsc_row_ind");
+        for (int v : rind) std::printf(" %d", v);
+        std::printf("\n");
+
         apspie_dev_free(ctx.get(), d_w, 4 * (int64_t)A.nnz);
         apspie_dev_free(ctx.get(), d_out, 4 * (int64_t)A.m);
         apspie_dev_free(ctx.get(), d_dist, 4 * (int64_t)A.m);

@@ -50,3 +50,9 @@ def test_mirror_driver_matches_oracle(name):
     assert [int(x) for x in out["mxm_colptr"]] == cp.tolist()
     assert [int(x) for x in out["mxm_rows"]] == ri.tolist()
     assert np.array_equal(np.array([float(x) for x in out["mxm_vals"]], np.float32), cv)
+    ps, pn = (int(x) for x in out["dcsc_parts"])
+    pp, ci, co, ri, _ = O.dcsc(g.m, g.rowptr, g.col, None, ps, pn)
+    assert [int(x) for x in out["dcsc_part_ptr"]] == pp.tolist()
+    assert [int(x) for x in out["dcsc_col_ind"]] == ci.tolist()
+    assert [int(x) for x in out["dcsc_col_off"]] == co.tolist()
+    assert [int(x) for x in out["dcsc_row_ind"]] == ri.tolist()

@@ -235,6 +235,53 @@ def test_mxm_rejects_bad_shapes(ctx):
         G.close()
 
 
+@pytest.mark.parametrize("case,ps,pn", [("rmat10", 128, 8), ("rmat10", 100, 11),
+                                       ("random_empty", 7, 40), ("lattice32", 1024, 1),
+                                       ("lattice32", 1, 1024), ("no_values", 64, 4),
+                                       ("no_entries", 3, 2)])
+def test_dcsc_vs_oracle(ctx, case, ps, pn):
+    """Row-block DCSC on the device against the oracle's step-by-step
+    restatement of csr_to_dcsc: equal parts, a last part taking the rest
+    (11 parts of 100 rows over 1024), rows with no entries, one part (the
+    compressed transpose), one row per part, a graph without values and one
+    without entries."""
+    if case == "rmat10":
+        g = T.rmat(10)
+        m, rp, c = g.m, g.rowptr, g.col
+        v = O.assign_weights(m, rp, c)
+    elif case == "random_empty":
+        g = T.random_graph(300, 0.01, 5)
+        m, rp, c = g.m, g.rowptr, g.col
+        v = np.arange(1, len(c) + 1, dtype=np.float32)
+    elif case == "lattice32":
+        g = T.lattice(32)
+        m, rp, c = g.m, g.rowptr, g.col
+        v = np.arange(len(c), dtype=np.float32)
+    elif case == "no_values":
+        g = T.rmat(8)
+        m, rp, c, v = g.m, g.rowptr, g.col, None
+    else:
+        m, rp, c, v = 5, np.zeros(6, np.int64), np.zeros(0, np.int32), None
+    G = ctx.upload(np.asarray(rp, np.int32), np.asarray(c, np.int32), v, n=m)
+    pp, ci, co, ri, vv = G.to_dcsc(ps, pn, values=v is not None)
+    wpp, wci, wco, wri, wvv = O.dcsc(m, rp, c, v, ps, pn)
+    assert pp.tolist() == wpp.tolist()
+    assert ci.tolist() == wci.tolist() and co.tolist() == wco.tolist()
+    assert ri.tolist() == wri.tolist()
+    if v is not None:
+        assert np.array_equal(vv, wvv)
+    G.close()
+
+
+def test_dcsc_rejects_bad_parts(ctx):
+    g = T.rmat(8)
+    G = ctx.upload(g.rowptr.astype(np.int32), g.col)
+    for ps, pn in ((0, 4), (4, 0), (64, 6)):  # 5 * 64 > 256 rows: a part past the end
+        with pytest.raises(A.ApspieError):
+            G.to_dcsc(ps, pn, values=False)
+    G.close()
+
+
 def _masked_want(Am, Bm, Mm):
     """The oracle's full product, read at the mask's entries."""
     wrp, wc, wv = O.spgemm(Am.shape[0], Am.shape[1], Bm.shape[1], Am.indptr.astype(np.int64),

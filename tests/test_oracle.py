@@ -402,6 +402,52 @@ def test_mxm_bridges_spmspv_and_spgemm():
     assert sv.tolist() == cv.tolist()
 
 
+def dcsc_by_blocks(A, ps, pn):
+    """Row-block DCSC from scipy: each block of rows converted to CSC, its
+    empty columns dropped (an independent route to the same arrays)."""
+    m = A.shape[0]
+    pp, ci, co, ri, vv = [0], [], [0], [], []
+    for p in range(pn):
+        lo, hi = p * ps, (m if p == pn - 1 else (p + 1) * ps)
+        B = sp.csc_matrix(A[lo:hi])
+        B.sort_indices()
+        for j in np.nonzero(np.diff(B.indptr))[0]:
+            s, e = B.indptr[j], B.indptr[j + 1]
+            ci.append(j)
+            ri.extend((B.indices[s:e] + lo).tolist())
+            vv.extend(B.data[s:e].tolist())
+            co.append(co[-1] + e - s)
+        pp.append(len(ci))
+    return pp, ci, co, ri, vv
+
+
+@pytest.mark.parametrize("ps,pn", [(16, 8), (10, 13), (128, 1), (1, 128), (40, 4)])
+def test_dcsc_vs_scipy_blocks(ps, pn):
+    """Parts of equal rows, a last part taking the rest (10 x 13 over 128 rows
+    leaves 8 rows to the last part), one part (the transpose, compressed) and
+    one row per part; a random graph with empty rows and distinct values."""
+    m, rp, c = random_graph(128, 0.04, 8)
+    v = np.arange(1, len(c) + 1, dtype=np.float32)
+    A = sp.csr_matrix((v.astype(np.float64), c, rp), shape=(m, m))
+    got = O.dcsc(m, rp, c, v, ps, pn)
+    want = dcsc_by_blocks(A, ps, pn)
+    for g, w in zip(got, want):
+        assert g.tolist() == list(w)
+
+
+def test_dcsc_hand_case():
+    # rows 0-1 | 2-3 of [[0 5 0 6], [7 0 0 8], [0 0 0 0], [0 9 1 0]]
+    rp = np.array([0, 2, 4, 4, 6])
+    c = np.array([1, 3, 0, 3, 1, 2], np.int32)
+    v = np.array([5, 6, 7, 8, 9, 1], np.float32)
+    pp, ci, co, ri, vv = O.dcsc(4, rp, c, v, 2, 2)
+    assert pp.tolist() == [0, 3, 5]
+    assert ci.tolist() == [0, 1, 3, 1, 2]
+    assert co.tolist() == [0, 1, 2, 4, 5, 6]
+    assert ri.tolist() == [1, 0, 0, 1, 3, 3]
+    assert vv.tolist() == [7, 5, 6, 8, 9, 1]
+
+
 # ----------------------------------------------- planted mistakes are caught
 def test_planted_bfs_mistake_is_caught():
     """An off-by-one in depth (a plausible bug) must fail the scipy pin."""
