@@ -62,6 +62,8 @@ struct Graph {
     int32_t *lowcnt = nullptr;  // per row: number of cols < row (TC)
     uint32_t *iso_bits = nullptr;  // bit per vertex without entries (BFS visited start)
     int32_t max_deg = -1;          // longest row, once known
+    int w_range = -1;              // 1 once w_lo / w_hi are known
+    float w_lo = 0.0f, w_hi = 0.0f; // smallest and largest value
     int symmetric = -1;         // 1 / 0 once known (pull traversal needs A == A^T)
     int sym_weights = -1;       // 1 / 0 once known: values equal to the transpose's too
     // rows longer than the SSSP pull's lane limit, as a frontier over those
@@ -165,6 +167,11 @@ bool graph_is_symmetric(Ctx *c, Graph *g);
 bool graph_is_symmetric_weighted(Ctx *c, Graph *g);
 const uint32_t *graph_iso_bits(Ctx *c, Graph *g);
 int32_t graph_max_degree(Ctx *c, Graph *g);
+// Smallest and largest value of a weighted graph with entries, computed once
+// (false: no values or no entries).
+bool graph_weight_range(Ctx *c, Graph *g, float *lo, float *hi);
+// Near-far split width for sssp_delta = -1 (0 = Bellman-Ford rounds).
+int64_t sssp_auto_delta(Ctx *c, Graph *g);
 
 }  // namespace apspie
 

@@ -1,6 +1,7 @@
 // Context, graph residency and the C ABI glue.
 #include <algorithm>
 #include <cstdlib>
+#include <utility>
 
 #include "common.hpp"
 #include "device.hpp"
@@ -14,7 +15,8 @@ static const char *kOptionNames[] = {
     "bfs_direction",     // 0 push only, 1 direction-optimising
     "bfs_alpha",         // push->pull switch: frontier edges * alpha > unexplored edges
     "bfs_beta",          // pull->push switch: frontier vertices * beta < m
-    "sssp_delta",        // near/far split width (0 = plain Bellman-Ford frontier)
+    "sssp_delta",        // near/far split width (0 = plain Bellman-Ford frontier,
+                         // -1 = chosen per graph from its weights and mean row length)
     "sssp_pull_div",     // pull an SSSP round when frontier edges * this > nnz (0 = never;
                          // symmetric weights only)
     "spmspv_path",       // 0 auto, 1 sort, 2 atomic
@@ -29,24 +31,30 @@ static const char *kOptionNames[] = {
     "profiling",         // 1 = time the dominant kernel with hipEvents
     nullptr};
 
-static int64_t option_default(const std::string &n) {
-    if (n == "bfs_tile") return 2048;
-    if (n == "bfs_blocks_per_cu") return 8;
-    if (n == "bfs_levels_per_sync") return 8;
-    if (n == "bfs_direction") return 1;
-    if (n == "bfs_alpha") return 14;
-    if (n == "bfs_beta") return 24;
-    if (n == "sssp_delta") return 0;
-    if (n == "sssp_pull_div") return 0;
-    if (n == "spmspv_path") return 0;
-    if (n == "spmspv_sort_max") return 1 << 16;
-    if (n == "batch_blocks_per_cu") return 4;
-    if (n == "batch_graph") return 1;
-    if (n == "batch_pull_div") return 4;
-    if (n == "batch_sssp_delta") return 256;
-    if (n == "batch_snapshot_mb") return 24576;
-    if (n == "profiling") return 0;
-    return -1;
+static bool option_default(const std::string &n, int64_t *v) {
+    static const std::pair<const char *, int64_t> kDefaults[] = {
+        {"bfs_tile", 2048},
+        {"bfs_blocks_per_cu", 8},
+        {"bfs_levels_per_sync", 8},
+        {"bfs_direction", 1},
+        {"bfs_alpha", 14},
+        {"bfs_beta", 24},
+        {"sssp_delta", -1},
+        {"sssp_pull_div", 0},
+        {"spmspv_path", 0},
+        {"spmspv_sort_max", 1 << 16},
+        {"batch_blocks_per_cu", 4},
+        {"batch_graph", 1},
+        {"batch_pull_div", 4},
+        {"batch_sssp_delta", 256},
+        {"batch_snapshot_mb", 24576},
+        {"profiling", 0}};
+    for (const auto &d : kDefaults)
+        if (n == d.first) {
+            *v = d.second;
+            return true;
+        }
+    return false;
 }
 
 static bool option_known(const std::string &n) {
@@ -58,8 +66,8 @@ static bool option_known(const std::string &n) {
 int64_t opt(Ctx *c, const char *name, int64_t dflt) {
     auto it = c->options.find(name);
     if (it != c->options.end()) return it->second;
-    int64_t d = option_default(name);
-    return d >= 0 ? d : dflt;
+    int64_t d = 0;
+    return option_default(name, &d) ? d : dflt;
 }
 
 void set_device(Ctx *c) { APSPIE_HIP(hipSetDevice(c->device)); }
@@ -266,8 +274,8 @@ int apspie_ctx_set_option(apspie_ctx *c, const char *name, int64_t v) {
     if (n == "bfs_levels_per_sync" && (v < 1 || v > 4096)) return APSPIE_ERR_INVALID;
     if (n == "batch_pull_div" && (v < 0 || v > (1 << 20))) return APSPIE_ERR_INVALID;
     if (n == "batch_snapshot_mb" && (v < 0 || v > (1 << 22))) return APSPIE_ERR_INVALID;
-    if ((n == "batch_sssp_delta" || n == "sssp_delta") && (v < 0 || v > (1 << 30)))
-        return APSPIE_ERR_INVALID;
+    if (n == "batch_sssp_delta" && (v < 0 || v > (1 << 30))) return APSPIE_ERR_INVALID;
+    if (n == "sssp_delta" && (v < -1 || v > (1 << 30))) return APSPIE_ERR_INVALID;
     if (n == "sssp_pull_div" && (v < 0 || v > (1 << 20))) return APSPIE_ERR_INVALID;
     if (n == "profiling") c->profiling = v != 0;
     c->options[n] = v;

@@ -294,6 +294,56 @@ __global__ void k_max_degree(const int32_t *rowptr, int32_t m, int32_t *out) {
     if (dev::lane_id() == 0 && best > 0) atomicMax(out, best);
 }
 
+// Floats as unsigned keys that order like the floats (negative ones reversed).
+__device__ __forceinline__ uint32_t f_order(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__global__ void k_weight_range(const float *w, int64_t n, uint32_t *out) {
+    uint32_t lo = 0xffffffffu, hi = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t k = f_order(w[i]);
+        lo = min(lo, k);
+        hi = max(hi, k);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = min(lo, (uint32_t)__shfl_xor((int)lo, off, 64));
+        hi = max(hi, (uint32_t)__shfl_xor((int)hi, off, 64));
+    }
+    if (dev::lane_id() == 0) {
+        atomicMin(out, lo);
+        atomicMax(out + 1, hi);
+    }
+}
+
+bool graph_weight_range(Ctx *c, Graph *g, float *lo, float *hi) {
+    if (!g->val || g->nnz == 0) return false;
+    std::lock_guard<std::mutex> lock(g->derived_mu);
+    if (g->w_range < 0) {
+        uint32_t *d = (uint32_t *)scratch(c, "w_range", 256);
+        fill_i32(c, (int32_t *)d, -1, 1);
+        fill_i32(c, (int32_t *)d + 1, 0, 1);
+        k_weight_range<<<(int)std::min<int64_t>(ceil_div(g->nnz, kBlock), 4 * c->cus), kBlock, 0,
+                         c->stream>>>(g->val, g->nnz, d);
+        APSPIE_LAUNCH_CHECK();
+        uint32_t h[2];
+        APSPIE_HIP(hipMemcpyAsync(h, d, 8, hipMemcpyDeviceToHost, c->stream));
+        sync(c);
+        auto back = [](uint32_t k) {
+            const uint32_t b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+            return __builtin_bit_cast(float, b);
+        };
+        g->w_lo = back(h[0]);
+        g->w_hi = back(h[1]);
+        g->w_range = 1;
+    }
+    *lo = g->w_lo;
+    *hi = g->w_hi;
+    return true;
+}
+
 // Longest row, computed once per graph.
 int32_t graph_max_degree(Ctx *c, Graph *g) {
     std::lock_guard<std::mutex> lock(g->derived_mu);

@@ -12,7 +12,7 @@
 // atomic min on the distance bits (non-negative floats order like their bit
 // patterns), and the thread whose relaxation lowered a vertex first in this
 // round appends it to the next frontier (the atomic OR that sets its bit in
-// the next frontier's bitmap; near-far rounds use a per-vertex round stamp).
+// the next frontier's bitmap).
 // With integer-valued weights every sum is exact, so the fixed point does not
 // depend on the relaxation order.
 #include "common.hpp"
@@ -27,13 +27,12 @@ namespace {
 constexpr float kInf = APSPIE_SSSP_INF;
 
 template <int TILE>
-__global__ void k_sssp_seed(const int32_t *rowptr, int32_t src, int32_t *dist, int32_t *stamp,
-                            FrontierBufs f0, uint64_t *ctl) {
+__global__ void k_sssp_seed(const int32_t *rowptr, int32_t src, int32_t *dist, FrontierBufs f0,
+                            uint64_t *ctl) {
     const int32_t rs = rowptr[src];
     const int32_t d = rowptr[src + 1] - rs;
     if (threadIdx.x == 0) {
         dist[src] = dev::fbits(0.0f);
-        stamp[src] = 0;
         ctl[0] = d > 0 ? ((uint64_t(1) << 32) | uint64_t(d)) : 0;
         ctl[1] = 0;
         ctl[2] = 0;
@@ -362,36 +361,39 @@ void sssp_do(Ctx *c, const Graph *g0, int32_t source, float *dist_out, int32_t p
 // frontier as above; one that leaves it at or above T only marks v pending.
 // When a round finds the frontier empty it is a split round instead: T grows
 // by delta and a sweep over all vertices moves the pending ones now below T
-// into the frontier.  Vertices are expanded fewer times (mostly once, near
-// their final distance); the fixed point is the same.
+// into the frontier.  A pending vertex whose distance is already below the
+// old T was lowered again by a near relaxation, which queued it with that
+// distance: the sweep drops it (so near relaxations never clear the mark,
+// and a far relaxation that lost the race to a near one costs nothing).  Vertices are expanded fewer times (mostly once, near
+// their final distance); the fixed point is the same.  Relaxations and
+// claims are those of the Bellman-Ford rounds above (a non-returning atomic
+// min, the next bitmap's OR as the claim); a split's sweep meets every vertex
+// once and appends without a claim.
 //
 // ctl words (every word a round reads was written by the previous round, and
 // no word a round reads is written in that round): [0..2] frontier counter
-// ring, [4..6] stats, [12 + (r & 1)] T of round r (float bits), [14 + (r & 1)]
+// ring, [4..7] stats, [12 + (r & 1)] T of round r (float bits), [14 + (r & 1)]
 // 1 if round r - 1 was a split, and the pending-vertex count left by the
 // split of round r in the spread slots [16 + 16 i + r % 3].
 constexpr int kNfBlock = 1024;
 constexpr int kNfPer = 8;  // vertices per thread per sweep pass
 
 struct SsspNearOp {
+    static constexpr bool kStreamColumns = true;  // columns and weights stream past the L2
     const float *w;
     int32_t *dist;  // float bits
-    int32_t *stamp;
+    uint32_t *fb_next;
     uint8_t *pend;
-    int32_t round;  // index of the frontier being built
     int32_t tbits;  // threshold T (float bits; distances are non-negative)
     __device__ __forceinline__ bool edge(int32_t u, int32_t e, int32_t v) const {
-        const float du = dev::bitsf(dist[u]);
-        const int32_t nd = dev::fbits(du + w[e]);
+        const int32_t nd = dev::fbits(dev::bitsf(dist[u]) + __builtin_nontemporal_load(w + e));
         if (nd >= dist[v]) return false;
-        const int32_t old = dev::atomic_min_dev(dist + v, nd);
-        if (nd >= old) return false;
+        __hip_atomic_fetch_min(dist + v, nd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (nd >= tbits) {  // far: expanded by a later split round
             pend[v] = 1;
             return false;
         }
-        pend[v] = 0;  // near now (a racing far store only costs a re-expansion)
-        return dev::atomic_exch_dev(stamp + v, round) != round;
+        return claim_bit(fb_next, v);
     }
 };
 
@@ -403,9 +405,9 @@ __device__ __forceinline__ uint64_t *pend_slot(uint64_t *ctl, int r) {
 template <int TILE>
 __global__ __launch_bounds__(kNfBlock) __attribute__((amdgpu_waves_per_eu(8))) void
 k_sssp_nf_round(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
-                const float *w, int32_t nnz, int32_t m, int32_t *dist, int32_t *stamp,
-                uint8_t *pend, float delta, FrontierBufs cur, FrontierBufs nxt, uint64_t *ctl,
-                int32_t r) {
+                const float *w, int32_t nnz, int32_t m, int32_t *dist, uint32_t *fb_next,
+                uint32_t *fb_clear, uint8_t *pend, float delta, FrontierBufs cur,
+                FrontierBufs nxt, uint64_t *ctl, int32_t r) {
     __shared__ ExpandLds<kNfBlock, TILE, true> lds;
     __shared__ uint64_t s_part[kNfBlock / 64];
     const uint64_t cc = ctl[r % 3];
@@ -427,11 +429,16 @@ k_sssp_nf_round(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ 
             ctl[4] += (uint64_t)E;
             ctl[5] += (uint64_t)F;
             ctl[6] += 1;
+            ctl[7] += (uint64_t)E;  // CSR entries read
         }
     }
     if (done) return;
+    // the bitmap round r + 1 claims into (round r claims into another)
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m / 32 + 1;
+         i += (int64_t)gridDim.x * blockDim.x)
+        fb_clear[i] = 0;
     if (!split) {
-        SsspNearOp op{w, dist, stamp, pend, r + 1, tbits};
+        SsspNearOp op{w, dist, fb_next, pend, tbits};
         expand_tiles<kNfBlock, TILE, true, true, true>(lds, op, rowptr, col, nnz, cur, F, E, nxt,
                                                        ctl + ((r + 1) % 3));
         return;
@@ -441,27 +448,29 @@ k_sssp_nf_round(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ 
     for (int64_t pbase = (int64_t)blockIdx.x * kNfBlock * kNfPer; pbase < m;
          pbase += (int64_t)gridDim.x * kNfBlock * kNfPer) {
         int32_t cv[kNfPer], crs[kNfPer], cd[kNfPer];
-        bool take[kNfPer];
+        bool take[kNfPer], drop[kNfPer];
 #pragma unroll
         for (int k = 0; k < kNfPer; ++k) {
             const int64_t v = pbase + (int64_t)k * kNfBlock + threadIdx.x;
             cv[k] = (int32_t)v;
             crs[k] = 0;
             cd[k] = 0;
-            take[k] = false;
+            take[k] = drop[k] = false;
             if (v < m && pend[v]) {
-                if (dist[v] < tnext) take[k] = true;
+                const int32_t dv = dist[v];
+                if (dv < tbits) drop[k] = true;  // lowered below T since: already expanded
+                else if (dv < tnext) take[k] = true;
                 else ++left;
             }
         }
 #pragma unroll
-        for (int k = 0; k < kNfPer; ++k)
-            if (take[k]) {
-                pend[cv[k]] = 0;
-                stamp[cv[k]] = r + 1;
+        for (int k = 0; k < kNfPer; ++k) {
+            if (take[k] || drop[k]) pend[cv[k]] = 0;
+            if (take[k]) {  // the sweep meets each vertex once: no claim needed
                 crs[k] = rowptr[cv[k]];
                 cd[k] = rowptr[cv[k] + 1] - crs[k];
             }
+        }
         produce<kNfBlock, kNfPer, TILE, true>(cv, crs, cd, nxt, ctl + ((r + 1) % 3), lds.scan,
                                               &lds.base);
     }
@@ -479,13 +488,14 @@ template <int TILE>
 void sssp_near_far(Ctx *c, const Graph *g, int32_t source, float *dist_out, float delta) {
     FrontierSet fs = frontier_alloc(c, "sssp", g->m, g->nnz, TILE, true);
     int32_t *dist = (int32_t *)dist_out;
-    int32_t *stamp = (int32_t *)scratch(c, "sssp_stamp", sizeof(int32_t) * g->m);
+    const int64_t words = g->m / 32 + 1;
+    uint32_t *fb = (uint32_t *)scratch(c, "sssp_fb", sizeof(uint32_t) * 3 * words);
     uint8_t *pend = (uint8_t *)scratch(c, "sssp_pend", (size_t)g->m + 4);
     fill_i32(c, dist, __builtin_bit_cast(int32_t, kInf), g->m);
-    fill_i32(c, stamp, -1, g->m);
+    fill_i32(c, (int32_t *)fb, 0, 3 * words);
     fill_i32(c, (int32_t *)pend, 0, ceil_div(g->m + 4, 4));
     fill_i32(c, (int32_t *)fs.ctl, 0, 2 * kCtlWords);
-    k_sssp_seed<TILE><<<1, 256, 0, c->stream>>>(g->rowptr, source, dist, stamp, fs.f[0], fs.ctl);
+    k_sssp_seed<TILE><<<1, 256, 0, c->stream>>>(g->rowptr, source, dist, fs.f[0], fs.ctl);
     fill_i32(c, (int32_t *)(fs.ctl + 12), __builtin_bit_cast(int32_t, delta), 1);  // T of round 0
     APSPIE_LAUNCH_CHECK();
     const int grid = fill_grid(
@@ -498,7 +508,8 @@ void sssp_near_far(Ctx *c, const Graph *g, int32_t source, float *dist_out, floa
         kt.begin();
         for (int k = 0; k < chunk; ++k, ++r) {
             k_sssp_nf_round<TILE><<<grid, kNfBlock, 0, c->stream>>>(
-                g->rowptr, g->col, g->val, (int32_t)g->nnz, g->m, dist, stamp, pend, delta,
+                g->rowptr, g->col, g->val, (int32_t)g->nnz, g->m, dist,
+                fb + ((r + 1) % 3) * words, fb + ((r + 2) % 3) * words, pend, delta,
                 fs.f[r & 1], fs.f[(r + 1) & 1], fs.ctl, r);
             APSPIE_LAUNCH_CHECK();
         }
@@ -518,13 +529,33 @@ void sssp_near_far(Ctx *c, const Graph *g, int32_t source, float *dist_out, floa
     c->stats.edges_traversed = (int64_t)h[4];
     c->stats.vertices_reached = (int64_t)h[5];
     c->stats.iterations = (int32_t)h[6];
+    c->stats.edges_checked = (int64_t)h[7];
     kt.collect();
 }
 
 }  // namespace
 
+// The near-far split width when sssp_delta is left at -1: ten times the
+// largest weight over the mean row length, rounded to the nearest power of
+// two in [16, 4096] -- at weights 1..255 that is 64 on R-MAT (mean row 30:
+// 48-64 measured best at scale 22, 8 % under Bellman-Ford) and 512 on 2D
+// lattices (mean row 3.6: 1.4-2.5x under Bellman-Ford at 2048^2 and 4096^2).
+// Bellman-Ford rounds (0) when the weights are not positive or lie within a
+// factor of two of each other: with near-equal weights a vertex settles in
+// the round that first reaches it and the splits would only add rounds.
+int64_t sssp_auto_delta(Ctx *c, Graph *g) {
+    float lo = 0.0f, hi = 0.0f;
+    if (g->m == 0 || !graph_weight_range(c, g, &lo, &hi) || !(lo > 0.0f) || !(hi >= 2.0f * lo))
+        return 0;
+    const double want = 10.0 * hi / std::max(1.0, (double)g->nnz / g->m);
+    int64_t d = 16;
+    while (d < 4096 && (double)d * 1.4142135623730951 < want) d *= 2;
+    return d;
+}
+
 void sssp_run(Ctx *c, const Graph *g, int32_t source, float *dist) {
-    const int64_t delta = opt(c, "sssp_delta", 0);
+    int64_t delta = opt(c, "sssp_delta", -1);
+    if (delta < 0) delta = sssp_auto_delta(c, const_cast<Graph *>(g));
     if (delta > 0) {
         switch (opt(c, "bfs_tile", 2048)) {
         case 1024: sssp_near_far<1024>(c, g, source, dist, (float)delta); return;

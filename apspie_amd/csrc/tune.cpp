@@ -6,7 +6,9 @@
 // context's option table (readable with apspie_ctx_get_option, so callers can
 // persist them per GPU and graph class):
 //   batch == 0: single-source BFS over tile size x blocks per CU, then the
-//               direction switch threshold alpha (push -> pull);
+//               direction switch threshold alpha (push -> pull), and for a
+//               weighted graph the SSSP near-far width sssp_delta (Bellman-Ford,
+//               and half, once and twice the width chosen from the weights);
 //   batch  > 0: batched BFS over blocks per CU x levels per hipGraph chunk,
 //               then the pull threshold batch_pull_div, and for a weighted
 //               graph the batched SSSP threshold step batch_sssp_delta.
@@ -73,6 +75,23 @@ extern "C" int apspie_autotune(apspie_ctx *c, const apspie_graph *g, int32_t bat
                     if (tot < best) { best = tot; ba = alpha; }
                 }
                 apspie_ctx_set_option(c, "bfs_alpha", ba);
+                if (g->val) {
+                    const int64_t d = std::max<int64_t>(
+                        apspie::sssp_auto_delta(c, const_cast<apspie_graph *>(g)), 64);
+                    int64_t bs = -1;
+                    best = 1e300;
+                    for (int64_t delta : {(int64_t)0, d / 2, d, 2 * d}) {
+                        apspie_ctx_set_option(c, "sssp_delta", delta);
+                        double tot = 0;
+                        for (size_t i = 0; i < std::min<size_t>(3, src.size()); ++i)
+                            tot += time_of([&] {
+                                return apspie_sssp(c, g, src[i], (float *)out,
+                                                   APSPIE_MEM_DEVICE);
+                            });
+                        if (tot < best) { best = tot; bs = delta; }
+                    }
+                    apspie_ctx_set_option(c, "sssp_delta", bs);
+                }
             } else {
                 double best = 1e300;
                 int64_t bb = 4, bl = 12;

@@ -193,10 +193,21 @@ def test_autotune_sets_valid_options():
     g = T.rmat(12)
     with A.Context(0) as c:
         G = c.upload(g.rowptr32, g.col, g.w)
+        assert c.get_option("sssp_delta") == -1  # chosen per graph until tuned
         c.autotune(G, 0)
         assert c.get_option("bfs_tile") in (1024, 2048, 4096)
         assert c.get_option("bfs_blocks_per_cu") in (2, 4, 8)
         assert c.get_option("bfs_alpha") in (7, 14, 28)
+        # Bellman-Ford, or half / once / twice the width chosen from the weights
+        # (10 * 255 / mean row length, to the nearest power of two)
+        want = 10 * 255 / (len(g.col) / g.m)
+        d = 16
+        while d < 4096 and d * 2 ** 0.5 < want:
+            d *= 2
+        d = max(d, 64)
+        assert c.get_option("sssp_delta") in (0, d // 2, d, 2 * d)
+        for s in T.sources(g, 2):
+            assert np.array_equal(c.sssp(G, s), oracle_sssp(g, s))
         c.autotune(G, 32)
         assert c.get_option("batch_blocks_per_cu") in (2, 4, 8)
         assert c.get_option("batch_pull_div") in (2, 4, 8)

@@ -34,15 +34,22 @@ def test_sssp_fixtures(ctx, name):
     G.close()
 
 
+@pytest.mark.parametrize("delta", [-1, 0])
 @pytest.mark.parametrize("key", ["rmat10", "rmat14", "lattice32", "lattice64"])
-def test_sssp_synthetic_vs_oracle(ctx, key):
+def test_sssp_synthetic_vs_oracle(ctx, key, delta):
+    """The default (-1: near-far with the width chosen from the weights) and
+    the Bellman-Ford rounds (0)."""
     g = T.rmat(int(key[4:])) if key.startswith("rmat") else T.lattice(int(key[7:]))
-    G = ctx.upload(g.rowptr32, g.col, g.w)
-    for src in [T.GOLD[key]["source"]] + T.sources(g, 4):
-        want = to_ref_sentinel(O.sssp(g.m, g.rowptr, g.col, g.w, src))
-        got = ctx.sssp(G, src)
-        assert np.array_equal(got, want), (key, src)
-    G.close()
+    ctx.set_option("sssp_delta", delta)
+    try:
+        G = ctx.upload(g.rowptr32, g.col, g.w)
+        for src in [T.GOLD[key]["source"]] + T.sources(g, 4):
+            want = to_ref_sentinel(O.sssp(g.m, g.rowptr, g.col, g.w, src))
+            got = ctx.sssp(G, src)
+            assert np.array_equal(got, want), (key, src)
+        G.close()
+    finally:
+        ctx.set_option("sssp_delta", -1)
 
 
 @pytest.mark.parametrize("delta", [1, 7, 64, 1000, 1 << 20])
@@ -75,7 +82,7 @@ def test_sssp_near_far_vs_oracle(ctx, key, delta):
             assert np.array_equal(got, want), (key, delta, src)
         G.close()
     finally:
-        ctx.set_option("sssp_delta", 0)
+        ctx.set_option("sssp_delta", -1)
 
 
 def wheel(n):
@@ -107,6 +114,7 @@ def test_sssp_pull_rounds_vs_oracle(ctx, key, pull_div):
         if key == "star":
             wt = O.assign_weights(m, rowptr, col)
     ctx.set_option("sssp_pull_div", pull_div)
+    ctx.set_option("sssp_delta", 0)  # pulls apply to the Bellman-Ford rounds
     try:
         G = ctx.upload(rowptr.astype(np.int32), col, wt)
         deg = np.diff(rowptr)
@@ -123,6 +131,7 @@ def test_sssp_pull_rounds_vs_oracle(ctx, key, pull_div):
         G.close()
     finally:
         ctx.set_option("sssp_pull_div", 0)
+        ctx.set_option("sssp_delta", -1)
 
 
 def test_sssp_pull_needs_symmetric_weights(ctx):
@@ -139,6 +148,7 @@ def test_sssp_pull_needs_symmetric_weights(ctx):
     drp, dc, _ = O.coo_to_csr(n, (k // n).astype(np.int32), (k % n).astype(np.int32))
     dw = (1 + rng.integers(0, 255, len(dc))).astype(np.float32)
     ctx.set_option("sssp_pull_div", 1 << 20)
+    ctx.set_option("sssp_delta", 0)
     try:
         for m, rowptr, col, wt in [(g.m, g.rowptr, g.col, w), (n, drp, dc, dw)]:
             G = ctx.upload(rowptr.astype(np.int32), col, wt)
@@ -150,6 +160,7 @@ def test_sssp_pull_needs_symmetric_weights(ctx):
             G.close()
     finally:
         ctx.set_option("sssp_pull_div", 0)
+        ctx.set_option("sssp_delta", -1)
 
 
 def test_sssp_near_far_edge_cases(ctx):
@@ -165,7 +176,7 @@ def test_sssp_near_far_edge_cases(ctx):
         assert got[2] == 0 and np.all(np.delete(got, 2) == A.SSSP_INF)
         G.close()
     finally:
-        ctx.set_option("sssp_delta", 0)
+        ctx.set_option("sssp_delta", -1)
 
 
 def test_sssp_edge_cases(ctx):
