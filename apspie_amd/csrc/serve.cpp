@@ -8,9 +8,11 @@
 // up to max_batch sources go through the batched multi-source traversal, and
 // all queued TC queries on a graph share one count.  Queries that arrive while
 // other batches run are picked up as soon as a worker frees, so their work
-// overlaps on the device with the batches in flight.  A batch that does not
-// fit the worker's share of the budget is split in halves; a single query
-// that does not fit fails with APSPIE_ERR_BUDGET.  Results land in host
+// overlaps on the device with the batches in flight.  Each worker's scratch
+// budget is what the resident graphs leave of the server's budget, split
+// evenly; a batch is sized to fit it (its result rows are held about three
+// times over), and one that still does not fit is split in halves; a single
+// query that does not fit fails with APSPIE_ERR_BUDGET.  Results land in host
 // memory; latency is measured from submit to completion on the host clock.
 // Result buffers are pinned host memory from a pool owned by the server: one
 // direct D2H copy per query (a batch writes its rows to device memory and each
@@ -97,6 +99,10 @@ struct apspie_server {
     apspie_ctx *admin = nullptr;
     std::vector<Resident> graphs;
     int64_t graph_bytes = 0;
+    // each worker's scratch budget: what the resident graphs leave of the
+    // server's budget, split evenly (0 = unlimited); a worker applies it to
+    // its own context before each batch
+    int64_t slot_budget = 0;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<std::shared_ptr<Request>> queue;
@@ -138,22 +144,33 @@ int run_batch(apspie_server *s, apspie_ctx *ctx, std::vector<std::shared_ptr<Req
     }
     std::vector<int32_t> src(n);
     for (size_t i = 0; i < n; ++i) src[i] = batch[i]->source;
+    // the batch's rows land in a device buffer the worker's context keeps
+    // between batches: a hipMalloc / hipFree per batch would wait for the
+    // whole device and stall the other workers' streams
     void *dout = nullptr;
-    int st = apspie_dev_alloc(ctx, (int64_t)(row * n), &dout);
+    int st = apspie::guard(ctx, [&] {
+        apspie::set_device(ctx);
+        dout = apspie::scratch(ctx, "serve_rows", row * n);
+    });
     if (st == APSPIE_OK)
         st = kind == APSPIE_Q_BFS
                  ? apspie_bfs_batch(ctx, R.g, src.data(), (int32_t)n, (int32_t *)dout,
                                     APSPIE_MEM_DEVICE)
                  : apspie_sssp_batch(ctx, R.g, src.data(), (int32_t)n, (float *)dout,
                                      APSPIE_MEM_DEVICE);
-    // each row straight to its query's pinned buffer
+    // each row straight to its query's pinned buffer: the copies queue on the
+    // worker's stream and one wait covers them all
     for (size_t i = 0; i < n && st == APSPIE_OK; ++i) {
         if (!batch[i]->buf) batch[i]->buf = s->pinned.get(row);
-        st = !batch[i]->buf ? APSPIE_ERR_OOM
-                            : apspie_memcpy(ctx, batch[i]->buf, (uint8_t *)dout + row * i,
-                                            (int64_t)row, 1);
+        if (!batch[i]->buf) st = APSPIE_ERR_OOM;
     }
-    if (dout) apspie_dev_free(ctx, dout, (int64_t)(row * n));
+    if (st == APSPIE_OK)
+        st = apspie::guard(ctx, [&] {
+            for (size_t i = 0; i < n; ++i)
+                apspie::copy_from_device(ctx, batch[i]->buf, (uint8_t *)dout + row * i, row,
+                                         APSPIE_MEM_HOST);
+            apspie::sync(ctx);
+        });
     return st;
 }
 
@@ -186,8 +203,10 @@ void execute(apspie_server *s, apspie_ctx *ctx, std::vector<std::shared_ptr<Requ
 
 void worker(apspie_server *s, int slot) {
     apspie_ctx *ctx = s->ctxs[slot];
+    int64_t applied = -1;
     for (;;) {
         std::vector<std::shared_ptr<Request>> batch;
+        int64_t budget = 0;
         {
             std::unique_lock<std::mutex> lk(s->mu);
             s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
@@ -195,11 +214,19 @@ void worker(apspie_server *s, int slot) {
             auto head = s->queue.front();
             s->queue.pop_front();
             batch.push_back(head);
+            budget = s->slot_budget;
             // TC queries on one graph share one count; max_batch == 1 is the
             // plain one-query-per-launch form with no coalescing at all
-            const int cap = s->cfg.max_batch <= 1     ? 1
-                            : head->kind == APSPIE_Q_TC ? INT32_MAX
-                                                        : s->cfg.max_batch;
+            int cap = s->cfg.max_batch <= 1     ? 1
+                      : head->kind == APSPIE_Q_TC ? INT32_MAX
+                                                  : s->cfg.max_batch;
+            // a batch holds its result rows twice (the traversal's matrix and
+            // the rows copied out) plus frontier bits: keep it within the
+            // worker's budget rather than find out by failing
+            if (budget > 0 && head->kind != APSPIE_Q_TC) {
+                const int64_t fit = budget / (3 * result_bytes(s, head->kind, head->graph));
+                cap = (int)std::max<int64_t>(1, std::min<int64_t>(cap, fit));
+            }
             for (auto it = s->queue.begin(); it != s->queue.end() && (int)batch.size() < cap;) {
                 if ((*it)->graph == head->graph && (*it)->kind == head->kind) {
                     batch.push_back(*it);
@@ -209,6 +236,10 @@ void worker(apspie_server *s, int slot) {
                 }
             }
             ++s->busy;
+        }
+        if (budget != applied) {
+            apspie_ctx_set_hbm_budget(ctx, budget);
+            applied = budget;
         }
         execute(s, ctx, batch);
         std::lock_guard<std::mutex> lk(s->mu);
@@ -234,13 +265,12 @@ int apspie_server_create(int device, const apspie_server_config *cfg, apspie_ser
         delete s;
         return st;
     }
-    const int64_t scratch_share =
-        cfg->hbm_budget > 0 ? (int64_t)(0.4 * (double)cfg->hbm_budget) / cfg->slots : 0;
+    s->slot_budget = cfg->hbm_budget / cfg->slots;
     for (int i = 0; i < cfg->slots; ++i) {
         apspie_ctx *c = nullptr;
         st = apspie_ctx_create(device, &c);
         if (st != APSPIE_OK) break;
-        apspie_ctx_set_hbm_budget(c, scratch_share);
+        apspie_ctx_set_hbm_budget(c, s->slot_budget);
         s->ctxs.push_back(c);
     }
     if (st != APSPIE_OK) {
@@ -274,7 +304,8 @@ int apspie_server_add_graph(apspie_server *s, const apspie_csr *host, int32_t *g
     const int64_t bytes = 4 * ((int64_t)host->m + 1) + (host->val ? 8 : 4) * host->nnz;
     {
         std::lock_guard<std::mutex> lk(s->mu);
-        // resident graphs may take 60% of the budget; the rest is worker scratch
+        // resident graphs may take 60% of the budget; the workers split what
+        // the graphs leave
         if (s->cfg.hbm_budget > 0 && s->graph_bytes + bytes > (int64_t)(0.6 * s->cfg.hbm_budget))
             return APSPIE_ERR_BUDGET;
     }
@@ -289,6 +320,9 @@ int apspie_server_add_graph(apspie_server *s, const apspie_csr *host, int32_t *g
     std::lock_guard<std::mutex> lk(s->mu);
     s->graphs.push_back({g, host->m, bytes});
     s->graph_bytes += bytes;
+    if (s->cfg.hbm_budget > 0)
+        s->slot_budget = std::max<int64_t>(1, (s->cfg.hbm_budget - s->graph_bytes) /
+                                                  (int64_t)s->ctxs.size());
     *graph_id = (int32_t)s->graphs.size() - 1;
     return APSPIE_OK;
 }

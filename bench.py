@@ -96,6 +96,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--extras", default="sssp,mis,tc,spgemm,batched,served",
                     help="comma list of the extra workloads to run after the main line")
+    ap.add_argument("--served-queries", type=int, default=240,
+                    help="length of the served query stream (same stream for both forms)")
     ap.add_argument("--batch-graph", type=int, default=1,
                     help="replay batched level chunks from a captured hipGraph (1) or launch "
                          "them one by one (0)")
@@ -175,7 +177,8 @@ def main():
     if not args.no_extra:
         ctx.set_option("batch_graph", args.batch_graph)
         rec["extra"] = extras(ctx, G, rowptr, col, srcs, optr, cpu=not args.no_cpu,
-                              which=set(args.extras.split(",")))
+                              which=set(args.extras.split(",")),
+                              served_queries=args.served_queries)
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(rowptr, col, srcs)
     else:
@@ -185,7 +188,7 @@ def main():
     print(json.dumps(rec), flush=True)
 
 
-def extras(ctx, G, rowptr, col, srcs, optr, cpu=True, which=None):
+def extras(ctx, G, rowptr, col, srcs, optr, cpu=True, which=None, served_queries=240):
     """The other workloads of the path, each with its time, the roofline
     fraction where the per-unit bytes are defined, and a bounded CPU sample of
     the oracle on the same graph; then the batched and served forms."""
@@ -200,7 +203,7 @@ def extras(ctx, G, rowptr, col, srcs, optr, cpu=True, which=None):
         from apspie_amd import servebench as SB
         log("serving pool:")
         pool = SB.build_pool(log=log)
-        ex["served"] = SB.serve_benchmark(pool, n_queries=240, log=log)
+        ex["served"] = SB.serve_benchmark(pool, n_queries=served_queries, log=log)
     return ex
 
 
