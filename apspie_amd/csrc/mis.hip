@@ -13,9 +13,10 @@
 // of rounds, not the set, and is not used.
 //
 // Per round (no host round trip): k_expand marks candidates that lose to an
-// undecided neighbour (pull over the candidate's row, no atomics), k_join
-// admits the winners, k_expand removes candidates adjacent to a winner, and
-// k_build compacts the survivors into the next candidate frontier.
+// undecided neighbour (pull over the candidate's row, no atomics), k_build
+// admits the winners and lists them, k_expand pushes from the winners to
+// remove their undecided neighbours, and k_build compacts the survivors into
+// the next candidate frontier.
 #include <random>
 
 #include "common.hpp"
@@ -47,11 +48,24 @@ struct LoseOp {
     }
 };
 
-// candidate u that did not join leaves if a neighbour joined
+// a candidate that did not lose joins (the join list is built from these)
+struct JoinOp {
+    const int32_t *lose;
+    int32_t *state;
+    int32_t round;
+    __device__ __forceinline__ bool keep(int32_t v) const {
+        if (lose[v] == round) return false;
+        state[v] = 1;
+        return true;
+    }
+};
+
+// the undecided neighbours of a vertex that joined leave (no neighbour of a
+// joiner joined in the same round: keys are a strict total order)
 struct RemoveOp {
     int32_t *state;
-    __device__ __forceinline__ bool edge(int32_t u, int32_t, int32_t v) const {
-        if (v != u && state[u] == -1 && state[v] == 1) state[u] = 0;
+    __device__ __forceinline__ bool edge(int32_t, int32_t, int32_t v) const {
+        if (state[v] == -1) state[v] = 0;
         return false;
     }
 };
@@ -72,20 +86,14 @@ __global__ void k_mis_init(const int32_t *rowptr, int32_t m, int32_t *state, int
     if (blockIdx.x == 0 && threadIdx.x < 16) ctl[threadIdx.x] = 0;
 }
 
-__global__ void k_mis_join(FrontierBufs cur, const uint64_t *cnt, const int32_t *lose,
-                           int32_t round, int32_t *state) {
-    const int32_t F = (int32_t)cnt_entries(*cnt);
-    for (int32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < F; j += gridDim.x * blockDim.x) {
-        const int32_t u = cur.v[j];
-        if (lose[u] != round) state[u] = 1;
-    }
-}
-
 }  // namespace
 
 void mis_run(Ctx *c, const Graph *g, const uint32_t *prio, int32_t *state) {
     if (g->m == 0) return;
     FrontierSet fs = frontier_alloc(c, "mis", g->m, g->nnz, kTile, true);
+    // the vertices that join in a round, expanded to remove their neighbours
+    FrontierSet js = frontier_alloc(c, "mis_join", g->m, g->nnz, kTile, true);
+    uint64_t *jcnt = js.ctl;
     int32_t *lose = (int32_t *)scratch(c, "mis_lose", sizeof(int32_t) * g->m);
     const int grid = fill_grid(c, kBlock, 4);
     k_mis_init<<<grid, kBlock, 0, c->stream>>>(g->rowptr, g->m, state, lose, fs.ctl);
@@ -103,15 +111,19 @@ void mis_run(Ctx *c, const Graph *g, const uint32_t *prio, int32_t *state) {
             const int cur = round & 1;
             uint64_t *cnt = fs.ctl + (round % 3), *nxt = fs.ctl + ((round + 1) % 3);
             kt.begin();
-            // lose marks; this launch also clears the next round's counter
+            // lose marks; this launch also clears the join list's counter
             k_expand<kBlock, kTile, true, false, false><<<grid, kBlock, 0, c->stream>>>(
                 LoseOp{prio, state, lose, round}, g->rowptr, g->col, (int32_t)g->nnz, fs.f[cur],
-                cnt, fs.f[cur ^ 1], nullptr, nxt, fs.ctl + 4);
+                cnt, fs.f[cur ^ 1], nullptr, jcnt, fs.ctl + 4);
             kt.end();
-            k_mis_join<<<grid, kBlock, 0, c->stream>>>(fs.f[cur], cnt, lose, round, state);
+            k_build<kBlock, 8, kTile, true><<<grid, kBlock, 0, c->stream>>>(
+                JoinOp{lose, state, round}, g->rowptr, fs.f[cur].v, 0, cnt, js.f[0], jcnt);
+            // removal pushes from the joiners (few, mostly short rows) instead of
+            // scanning every candidate's row; this launch clears the next
+            // round's counter
             k_expand<kBlock, kTile, true, false, false><<<grid, kBlock, 0, c->stream>>>(
-                RemoveOp{state}, g->rowptr, g->col, (int32_t)g->nnz, fs.f[cur], cnt,
-                fs.f[cur ^ 1], nullptr, nullptr, nullptr);
+                RemoveOp{state}, g->rowptr, g->col, (int32_t)g->nnz, js.f[0], jcnt, js.f[1],
+                nullptr, nxt, nullptr);
             k_build<kBlock, 8, kTile, true><<<grid, kBlock, 0, c->stream>>>(
                 UndecidedOp{state}, g->rowptr, fs.f[cur].v, 0, cnt, fs.f[cur ^ 1], nxt);
             APSPIE_LAUNCH_CHECK();
