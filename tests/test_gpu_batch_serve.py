@@ -248,6 +248,23 @@ def test_server_mixed_stream_equals_alone():
         srv.drain()
 
 
+def test_server_tight_budget_sizes_batches():
+    """A worker budget that holds only part of what a batch of 128 would
+    need: batches are sized to fit (or split after a budget failure), and
+    every result still equals the oracle's."""
+    g = T.lattice(64)
+    with S.Server(slots=1, max_batch=128, hbm_budget=4 << 20) as srv:
+        gi = srv.add_graph(g.rowptr32, g.col, g.w)
+        src = T.sources(g, 90, seed=9)
+        tickets = [(s, A.Q_BFS, srv.submit(gi, A.Q_BFS, s)) for s in src]
+        tickets += [(s, A.Q_SSSP, srv.submit(gi, A.Q_SSSP, s)) for s in src[:30]]
+        for s, kind, t in tickets:
+            res, _ = srv.wait(t, kind, g.m)
+            want = O.bfs(g.m, g.rowptr, g.col, s)[0] if kind == A.Q_BFS else oracle_sssp(g, s)
+            assert np.array_equal(res, want), (kind, s)
+        srv.drain()
+
+
 def test_server_zero_copy_views():
     """wait(copy=False) hands out read-only views of pinned result buffers:
     equal to the copying wait, intact while later requests run, returned to
