@@ -170,8 +170,10 @@ int32_t graph_max_degree(Ctx *c, Graph *g);
 // Smallest and largest value of a weighted graph with entries, computed once
 // (false: no values or no entries).
 bool graph_weight_range(Ctx *c, Graph *g, float *lo, float *hi);
-// Near-far split width for sssp_delta = -1 (0 = Bellman-Ford rounds).
+// Near-far split width for sssp_delta = -1 (0 = Bellman-Ford rounds), and the
+// width the weights alone suggest (what the autotuner centres its candidates on).
 int64_t sssp_auto_delta(Ctx *c, Graph *g);
+int64_t sssp_weight_delta(Ctx *c, Graph *g);
 
 }  // namespace apspie
 

@@ -535,15 +535,14 @@ void sssp_near_far(Ctx *c, const Graph *g, int32_t source, float *dist_out, floa
 
 }  // namespace
 
-// The near-far split width when sssp_delta is left at -1: ten times the
-// largest weight over the mean row length, rounded to the nearest power of
-// two in [16, 4096] -- at weights 1..255 that is 64 on R-MAT (mean row 30:
-// 48-64 measured best at scale 22, 8 % under Bellman-Ford) and 512 on 2D
-// lattices (mean row 3.6: 1.4-2.5x under Bellman-Ford at 2048^2 and 4096^2).
-// Bellman-Ford rounds (0) when the weights are not positive or lie within a
-// factor of two of each other: with near-equal weights a vertex settles in
-// the round that first reaches it and the splits would only add rounds.
-int64_t sssp_auto_delta(Ctx *c, Graph *g) {
+// The near-far split width a graph's weights suggest: ten times the largest
+// weight over the mean row length, rounded to the nearest power of two in
+// [16, 4096] -- 64 on R-MAT-22 at weights 1..255 (mean row 30; 48-64
+// measured best, 8 % under Bellman-Ford) and 512 on 2D lattices (mean row
+// 3.6; 1.35x and 2.5x under Bellman-Ford at 2048^2 and 4096^2).  0 when the
+// weights are not positive or lie within a factor of two of each other (a
+// vertex then settles in the round that first reaches it).
+int64_t sssp_weight_delta(Ctx *c, Graph *g) {
     float lo = 0.0f, hi = 0.0f;
     if (g->m == 0 || !graph_weight_range(c, g, &lo, &hi) || !(lo > 0.0f) || !(hi >= 2.0f * lo))
         return 0;
@@ -551,6 +550,15 @@ int64_t sssp_auto_delta(Ctx *c, Graph *g) {
     int64_t d = 16;
     while (d < 4096 && (double)d * 1.4142135623730951 < want) d *= 2;
     return d;
+}
+
+// The width used when sssp_delta is left at -1: Bellman-Ford rounds (0)
+// below 2^22 vertices, where the rounds' fixed cost outweighs the
+// re-expansions near-far saves (measured at weights 1..255: R-MAT 16-21 and
+// lattices 512^2-1024^2 run as fast or faster without splits), else the
+// width the weights suggest.
+int64_t sssp_auto_delta(Ctx *c, Graph *g) {
+    return g->m < (1 << 22) ? 0 : sssp_weight_delta(c, g);
 }
 
 void sssp_run(Ctx *c, const Graph *g, int32_t source, float *dist) {

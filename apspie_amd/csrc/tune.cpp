@@ -77,7 +77,7 @@ extern "C" int apspie_autotune(apspie_ctx *c, const apspie_graph *g, int32_t bat
                 apspie_ctx_set_option(c, "bfs_alpha", ba);
                 if (g->val) {
                     const int64_t d = std::max<int64_t>(
-                        apspie::sssp_auto_delta(c, const_cast<apspie_graph *>(g)), 64);
+                        apspie::sssp_weight_delta(c, const_cast<apspie_graph *>(g)), 64);
                     int64_t bs = -1;
                     best = 1e300;
                     for (int64_t delta : {(int64_t)0, d / 2, d, 2 * d}) {
