@@ -35,15 +35,18 @@ __device__ __forceinline__ uint64_t mis_key(const uint32_t *prio, int32_t v) {
     return (uint64_t(prio[v]) << 32) | uint64_t(0x7fffffffu - (uint32_t)v);
 }
 
+// live[v]: v's key while v is undecided, 0 once decided (a key is never 0:
+// its low word is 0x7fffffff - v and v < 2^31 - 1), so one 8-byte load per
+// edge answers "undecided with a larger key"
+
 // candidate u loses if some undecided neighbour has a larger key
 struct LoseOp {
-    const uint32_t *prio;
-    const int32_t *state;
+    const uint64_t *live;
     int32_t *lose;
     int32_t round;
     __device__ __forceinline__ bool edge(int32_t u, int32_t, int32_t v) const {
         if (v == u || lose[u] == round) return false;
-        if (state[v] != 0 && mis_key(prio, v) > mis_key(prio, u)) lose[u] = round;
+        if (live[v] > live[u]) lose[u] = round;
         return false;
     }
 };
@@ -52,10 +55,12 @@ struct LoseOp {
 struct JoinOp {
     const int32_t *lose;
     int32_t *state;
+    uint64_t *live;
     int32_t round;
     __device__ __forceinline__ bool keep(int32_t v) const {
         if (lose[v] == round) return false;
         state[v] = 1;
+        live[v] = 0;
         return true;
     }
 };
@@ -64,8 +69,12 @@ struct JoinOp {
 // joiner joined in the same round: keys are a strict total order)
 struct RemoveOp {
     int32_t *state;
+    uint64_t *live;
     __device__ __forceinline__ bool edge(int32_t, int32_t, int32_t v) const {
-        if (state[v] == -1) state[v] = 0;
+        if (state[v] == -1) {
+            state[v] = 0;
+            live[v] = 0;
+        }
         return false;
     }
 };
@@ -75,12 +84,14 @@ struct UndecidedOp {
     __device__ __forceinline__ bool keep(int32_t v) const { return state[v] == -1; }
 };
 
-__global__ void k_mis_init(const int32_t *rowptr, int32_t m, int32_t *state, int32_t *lose,
-                           uint64_t *ctl) {
+__global__ void k_mis_init(const int32_t *rowptr, const uint32_t *prio, int32_t m,
+                           int32_t *state, uint64_t *live, int32_t *lose, uint64_t *ctl) {
     for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < m;
          v += (int64_t)gridDim.x * blockDim.x) {
         // a vertex without neighbours is in every maximal independent set
-        state[v] = rowptr[v + 1] == rowptr[v] ? 1 : -1;
+        const bool iso = rowptr[v + 1] == rowptr[v];
+        state[v] = iso ? 1 : -1;
+        live[v] = iso ? 0 : mis_key(prio, (int32_t)v);
         lose[v] = -1;
     }
     if (blockIdx.x == 0 && threadIdx.x < 16) ctl[threadIdx.x] = 0;
@@ -95,8 +106,9 @@ void mis_run(Ctx *c, const Graph *g, const uint32_t *prio, int32_t *state) {
     FrontierSet js = frontier_alloc(c, "mis_join", g->m, g->nnz, kTile, true);
     uint64_t *jcnt = js.ctl;
     int32_t *lose = (int32_t *)scratch(c, "mis_lose", sizeof(int32_t) * g->m);
+    uint64_t *live = (uint64_t *)scratch(c, "mis_live", sizeof(uint64_t) * g->m);
     const int grid = fill_grid(c, kBlock, 4);
-    k_mis_init<<<grid, kBlock, 0, c->stream>>>(g->rowptr, g->m, state, lose, fs.ctl);
+    k_mis_init<<<grid, kBlock, 0, c->stream>>>(g->rowptr, prio, g->m, state, live, lose, fs.ctl);
     APSPIE_LAUNCH_CHECK();
     // round 0 candidates: every undecided vertex (degree > 0)
     k_build<kBlock, 8, kTile, true><<<grid, kBlock, 0, c->stream>>>(
@@ -113,16 +125,16 @@ void mis_run(Ctx *c, const Graph *g, const uint32_t *prio, int32_t *state) {
             kt.begin();
             // lose marks; this launch also clears the join list's counter
             k_expand<kBlock, kTile, true, false, false><<<grid, kBlock, 0, c->stream>>>(
-                LoseOp{prio, state, lose, round}, g->rowptr, g->col, (int32_t)g->nnz, fs.f[cur],
+                LoseOp{live, lose, round}, g->rowptr, g->col, (int32_t)g->nnz, fs.f[cur],
                 cnt, fs.f[cur ^ 1], nullptr, jcnt, fs.ctl + 4);
             kt.end();
             k_build<kBlock, 8, kTile, true><<<grid, kBlock, 0, c->stream>>>(
-                JoinOp{lose, state, round}, g->rowptr, fs.f[cur].v, 0, cnt, js.f[0], jcnt);
+                JoinOp{lose, state, live, round}, g->rowptr, fs.f[cur].v, 0, cnt, js.f[0], jcnt);
             // removal pushes from the joiners (few, mostly short rows) instead of
             // scanning every candidate's row; this launch clears the next
             // round's counter
             k_expand<kBlock, kTile, true, false, false><<<grid, kBlock, 0, c->stream>>>(
-                RemoveOp{state}, g->rowptr, g->col, (int32_t)g->nnz, js.f[0], jcnt, js.f[1],
+                RemoveOp{state, live}, g->rowptr, g->col, (int32_t)g->nnz, js.f[0], jcnt, js.f[1],
                 nullptr, nxt, nullptr);
             k_build<kBlock, 8, kTile, true><<<grid, kBlock, 0, c->stream>>>(
                 UndecidedOp{state}, g->rowptr, fs.f[cur].v, 0, cnt, fs.f[cur ^ 1], nxt);
