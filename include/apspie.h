@@ -180,7 +180,9 @@ int apspie_bfs(apspie_ctx *ctx, const apspie_graph *g, int32_t source, int32_t *
  * spmspvSssp() (spmspvSssp.cuh:91-291), with real weights in every round. */
 int apspie_sssp(apspie_ctx *ctx, const apspie_graph *g, int32_t source, float *dist,
                 int mem);
-/* Maximal independent set by Luby rounds under the max-priority semiring;
+/* Maximal independent set by Luby rounds under the max-priority semiring, on
+ * an undirected graph (a CSR equal to its transpose, as the reference's
+ * drivers build it; a directed CSR gives a set that is not specified);
  * in_set[v] = 1 in the set, 0 not.  prio (m uint32, host or device per
  * prio_mem) may be NULL: then prio[v] = v-th draw of mt19937 seeded 5489.
  * Ties break to the lower id.  Replaces spmspvMis() (spmspvMis.cuh:149-344). */
