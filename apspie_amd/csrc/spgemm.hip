@@ -290,7 +290,8 @@ __global__ __launch_bounds__(BLOCK) void k_rows_table(const int32_t *rows, const
                                                       int32_t *row_nnz, const int32_t *crp,
                                                       int32_t *ccol, float *cval,
                                                       const int64_t *ub, int32_t *g_keys,
-                                                      float *g_vals, int64_t g_cap) {
+                                                      float *g_vals, int64_t g_cap,
+                                                      int32_t ncols) {
     constexpr bool LDS = TABLE > 0;
     __shared__ int32_t s_keys[LDS ? TABLE : 1];
     __shared__ float s_vals[(LDS && NUMERIC) ? TABLE : 1];
@@ -326,6 +327,42 @@ __global__ __launch_bounds__(BLOCK) void k_rows_table(const int32_t *rows, const
             if (created) atomicAdd(&s_count, created);
             __syncthreads();
             if (threadIdx.x == 0) row_nnz[r] = (int32_t)s_count;
+            __syncthreads();
+            continue;
+        }
+        if (LDS && ncols <= 32 * kChunk) {
+            // order by rank instead of sorting: a bit per column (the chunk
+            // staging arrays, free once the products are done), the words'
+            // popcount prefix, and each entry's rank from its own word
+            uint32_t *bits = (uint32_t *)pre, *wpre = (uint32_t *)rs;
+            const int32_t nw = (ncols + 31) / 32;
+            __syncthreads();  // every thread is past the last chunk
+            for (int32_t q = threadIdx.x; q < nw; q += BLOCK) bits[q] = 0;
+            __syncthreads();
+            for (uint32_t t = threadIdx.x; t < tsize; t += BLOCK) {
+                const int32_t k = keys[t];
+                if (k != kEmpty) atomicOr(bits + (k >> 5), 1u << (k & 31));
+            }
+            __syncthreads();
+            uint32_t carry = 0;
+            for (int32_t q0 = 0; q0 < nw; q0 += BLOCK) {
+                const int32_t q = q0 + threadIdx.x;
+                uint32_t tot;
+                const uint32_t ex = dev::block_exclusive_add32<BLOCK>(
+                    q < nw ? (uint32_t)__popc(bits[q]) : 0u, s_scan, tot);
+                if (q < nw) wpre[q] = carry + ex;
+                carry += tot;
+            }
+            __syncthreads();
+            const int32_t c0 = crp[r];
+            for (uint32_t t = threadIdx.x; t < tsize; t += BLOCK) {
+                const int32_t k = keys[t];
+                if (k == kEmpty) continue;
+                const uint32_t below = bits[k >> 5] & ((1u << (k & 31)) - 1u);
+                const int32_t pos = (int32_t)(wpre[k >> 5] + (uint32_t)__popc(below));
+                ccol[c0 + pos] = k;
+                cval[c0 + pos] = vals[t];
+            }
             __syncthreads();
             continue;
         }
@@ -438,7 +475,7 @@ Graph *spgemm_run(Ctx *c, const Graph *a, const Graph *b) {
             if (h[3])  // longest rows first
                 k_rows_table<NUM, kBig, 0><<<gblocks, kBig, 0, c->stream>>>(
                     lists + 3 * (size_t)m, ctl + 3, a->rowptr, a->col, a->val, b->rowptr, b->col,
-                    b->val, row_nnz, crp, ccol, cval, ub, gkeys, gvals, gcap);
+                    b->val, row_nnz, crp, ccol, cval, ub, gkeys, gvals, gcap, b->n);
             if (h[2])
                 k_rows_dense<NUM><<<(int)std::min<int64_t>(h[2], c->cus), kBig, 0, c->stream>>>(
                     lists + 2 * (size_t)m, ctl + 2, b->n, a->rowptr, a->col, a->val, b->rowptr,
@@ -447,7 +484,7 @@ Graph *spgemm_run(Ctx *c, const Graph *a, const Graph *b) {
                 k_rows_table<NUM, kBlock, kBlockTable>
                     <<<(int)std::min<int64_t>(h[1], 4 * c->cus), kBlock, 0, c->stream>>>(
                         lists + m, ctl + 1, a->rowptr, a->col, a->val, b->rowptr, b->col, b->val,
-                        row_nnz, crp, ccol, cval, ub, nullptr, nullptr, 0);
+                        row_nnz, crp, ccol, cval, ub, nullptr, nullptr, 0, b->n);
             if (h[0])
                 k_rows_wave<NUM><<<(int)std::min<int64_t>(ceil_div(h[0], kBlock / 64), 8 * c->cus),
                                    kBlock, 0, c->stream>>>(lists, ctl + 0, a->rowptr, a->col,
