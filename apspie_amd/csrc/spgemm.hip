@@ -15,8 +15,10 @@
 //   2. symbolic pass: distinct columns per row
 //   3. exclusive scan -> C row offsets
 //   4. numeric pass: accumulate, then write the row in column order (the
-//      dense accumulator is in order already; a table's used slots are
-//      compacted and only those are sorted)
+//      dense accumulator is in order already; a block table on a matrix of
+//      at most 32768 columns writes each entry at its rank among the row's
+//      columns, from a column bitmap; other tables compact their used slots
+//      and sort only those)
 // The block tiers walk a row's products load-balanced: an exclusive scan of
 // |B_k| over a chunk of A_i's entries, then each lane takes 8 consecutive
 // products (one binary search).  Values are exact for integer-valued inputs;
