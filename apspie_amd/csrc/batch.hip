@@ -700,8 +700,11 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
     APSPIE_HIP(hipMemcpyAsync(ulist, uniq.data(), sizeof(int32_t) * uniq.size(),
                               hipMemcpyHostToDevice, c->stream));
     const int grid = fill_grid(c, kBlock, (int)opt(c, "batch_blocks_per_cu", 4));
-    // near-far rounds for SSSP (threshold step batch_sssp_delta, 0 = off)
-    const float delta = SSSP ? (float)opt(c, "batch_sssp_delta", 256) : 0.0f;
+    // near-far rounds for SSSP (threshold step batch_sssp_delta, 0 = off,
+    // -1 = chosen per graph)
+    int64_t bdelta = SSSP ? opt(c, "batch_sssp_delta", -1) : 0;
+    if (bdelta < 0) bdelta = sssp_batch_auto_delta(c, const_cast<Graph *>(g));
+    const float delta = (float)bdelta;
     const bool near_far = SSSP && delta > 0;
     uint32_t *pend =
         near_far ? (uint32_t *)scratch(c, "batch_pend", sizeof(uint32_t) * words) : nullptr;

@@ -26,7 +26,8 @@ static const char *kOptionNames[] = {
     "batch_pull_div",    // batched BFS pulls a level when union-frontier edges * this > nnz
                          // (times 4 for batches of up to 128 sources on graphs whose rows
                          // are at most 64 long; 0 = never)
-    "batch_sssp_delta",  // batched SSSP near-far threshold step (0 = Bellman-Ford frontier)
+    "batch_sssp_delta",  // batched SSSP near-far threshold step (0 = Bellman-Ford frontier,
+                         // -1 = chosen per graph from its weights and mean row length)
     "batch_snapshot_mb", // HBM for the bit rows of pulled batched BFS levels (0 = none)
     "profiling",         // 1 = time the dominant kernel with hipEvents
     nullptr};
@@ -46,7 +47,7 @@ static bool option_default(const std::string &n, int64_t *v) {
         {"batch_blocks_per_cu", 4},
         {"batch_graph", 1},
         {"batch_pull_div", 4},
-        {"batch_sssp_delta", 256},
+        {"batch_sssp_delta", -1},
         {"batch_snapshot_mb", 24576},
         {"profiling", 0}};
     for (const auto &d : kDefaults)
@@ -274,7 +275,7 @@ int apspie_ctx_set_option(apspie_ctx *c, const char *name, int64_t v) {
     if (n == "bfs_levels_per_sync" && (v < 1 || v > 4096)) return APSPIE_ERR_INVALID;
     if (n == "batch_pull_div" && (v < 0 || v > (1 << 20))) return APSPIE_ERR_INVALID;
     if (n == "batch_snapshot_mb" && (v < 0 || v > (1 << 22))) return APSPIE_ERR_INVALID;
-    if (n == "batch_sssp_delta" && (v < 0 || v > (1 << 30))) return APSPIE_ERR_INVALID;
+    if (n == "batch_sssp_delta" && (v < -1 || v > (1 << 30))) return APSPIE_ERR_INVALID;
     if (n == "sssp_delta" && (v < -1 || v > (1 << 30))) return APSPIE_ERR_INVALID;
     if (n == "sssp_pull_div" && (v < 0 || v > (1 << 20))) return APSPIE_ERR_INVALID;
     if (n == "profiling") c->profiling = v != 0;

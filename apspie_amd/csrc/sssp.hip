@@ -561,6 +561,18 @@ int64_t sssp_auto_delta(Ctx *c, Graph *g) {
     return g->m < (1 << 22) ? 0 : sssp_weight_delta(c, g);
 }
 
+// The batched SSSP width when batch_sssp_delta is left at -1: the weights'
+// width where it is at least 256 (sparse rows, long paths: 512 on the
+// lattices, marginally ahead of 256 at 128 and 512 sources), Bellman-Ford
+// rounds otherwise (R-MAT, width 64: with many sources in flight the splits
+// cost more than the re-expansions they save -- 32 / 128 sources of R-MAT-22
+// measured 2.64 / 2.09 ms per query without splits, 2.68 / 2.24 at width
+// 256, 4.57 / 3.05 at 64).
+int64_t sssp_batch_auto_delta(Ctx *c, Graph *g) {
+    const int64_t d = sssp_weight_delta(c, g);
+    return d >= 256 ? d : 0;
+}
+
 void sssp_run(Ctx *c, const Graph *g, int32_t source, float *dist) {
     int64_t delta = opt(c, "sssp_delta", -1);
     if (delta < 0) delta = sssp_auto_delta(c, const_cast<Graph *>(g));

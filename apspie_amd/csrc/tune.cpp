@@ -11,7 +11,8 @@
 //               and half, once and twice the width chosen from the weights);
 //   batch  > 0: batched BFS over blocks per CU x levels per hipGraph chunk,
 //               then the pull threshold batch_pull_div, and for a weighted
-//               graph the batched SSSP threshold step batch_sssp_delta.
+//               graph the batched SSSP threshold step batch_sssp_delta
+//               (Bellman-Ford, 256, 1024).
 #include <algorithm>
 #include <random>
 #include <vector>
@@ -118,9 +119,9 @@ extern "C" int apspie_autotune(apspie_ctx *c, const apspie_graph *g, int32_t bat
                 }
                 apspie_ctx_set_option(c, "batch_pull_div", bd);
                 if (g->val) {
-                    int64_t bs = 256;
+                    int64_t bs = -1;
                     best = 1e300;
-                    for (int64_t delta : {64, 256, 1024}) {
+                    for (int64_t delta : {0, 256, 1024}) {
                         apspie_ctx_set_option(c, "batch_sssp_delta", delta);
                         const double t = time_of([&] {
                             return apspie_sssp_batch(c, g, src.data(), batch, (float *)out,

@@ -132,7 +132,7 @@ def test_sssp_batch_vs_oracle(ctx, key, nsrc):
     G.close()
 
 
-@pytest.mark.parametrize("delta", [0, 1, 16, 256, 1 << 20])
+@pytest.mark.parametrize("delta", [-1, 0, 1, 16, 256, 1 << 20])
 @pytest.mark.parametrize("nsrc", [7, 64, 150])
 @pytest.mark.parametrize("key", ["rmat11", "lattice32", "path"])
 def test_sssp_batch_near_far(ctx, key, nsrc, delta):
@@ -156,7 +156,7 @@ def test_sssp_batch_near_far(ctx, key, nsrc, delta):
             assert np.array_equal(got[i], want), (key, nsrc, delta, i, s)
         G.close()
     finally:
-        ctx.set_option("batch_sssp_delta", 256)
+        ctx.set_option("batch_sssp_delta", -1)
 
 
 @pytest.mark.parametrize("graph_on,lps", [(0, 6), (1, 6), (1, 48)])
@@ -211,7 +211,7 @@ def test_autotune_sets_valid_options():
         c.autotune(G, 32)
         assert c.get_option("batch_blocks_per_cu") in (2, 4, 8)
         assert c.get_option("batch_pull_div") in (2, 4, 8)
-        assert c.get_option("batch_sssp_delta") in (64, 256, 1024)
+        assert c.get_option("batch_sssp_delta") in (0, 256, 1024)
         src = T.sources(g, 4)
         for s in src:
             assert np.array_equal(c.bfs(G, s), O.bfs(g.m, g.rowptr, g.col, s)[0])
