@@ -326,7 +326,10 @@ void sssp_do(Ctx *c, const Graph *g0, int32_t source, float *dist_out, int32_t p
     const FrontierBufs heavy{g->heavy_v, g->heavy_start, g->heavy_off, g->heavy_tfirst};
     const int grid = fill_grid(
         c, kDoBlock, (int)std::max<int64_t>(1, opt(c, "bfs_blocks_per_cu", 8) * 256 / kDoBlock));
+    // rounds between host checks: the option first, then 4, 8, 16, ... (the
+    // idle tail of the last chunk is a launch per round)
     int chunk = (int)std::max<int64_t>(1, opt(c, "bfs_levels_per_sync", 8));
+    int grow = 4;
     KernelTimer kt(c);
     uint64_t *h = (uint64_t *)c->pinned;
     int r = 0;
@@ -346,7 +349,8 @@ void sssp_do(Ctx *c, const Graph *g0, int32_t source, float *dist_out, int32_t p
         c->stats.host_syncs++;
         if (cnt_entries(h[r % 3]) == 0) break;
         APSPIE_CHECK(r <= 4 * g->m + 64, APSPIE_ERR_DEVICE, "SSSP did not converge");
-        chunk = std::min(chunk * 2, 256);
+        chunk = grow;
+        grow = std::min(grow * 2, 256);
     }
     c->stats.edges_traversed = (int64_t)h[4];
     c->stats.vertices_reached = (int64_t)h[5];
@@ -500,7 +504,10 @@ void sssp_near_far(Ctx *c, const Graph *g, int32_t source, float *dist_out, floa
     APSPIE_LAUNCH_CHECK();
     const int grid = fill_grid(
         c, kNfBlock, (int)std::max<int64_t>(1, opt(c, "bfs_blocks_per_cu", 8) * 256 / kNfBlock));
+    // rounds between host checks: the option first, then 4, 8, 16, ... (the
+    // idle tail of the last chunk is a launch per round)
     int chunk = (int)std::max<int64_t>(1, opt(c, "bfs_levels_per_sync", 8));
+    int grow = 4;
     KernelTimer kt(c);
     uint64_t *h = (uint64_t *)c->pinned;
     int r = 0;
@@ -524,7 +531,8 @@ void sssp_near_far(Ctx *c, const Graph *g, int32_t source, float *dist_out, floa
         for (int i = 0; i < kSpread; ++i) left += h[16 + i * 16 + (r + 2) % 3];
         if (cnt_entries(h[r % 3]) == 0 && h[14 + (r & 1)] != 0 && left == 0) break;
         APSPIE_CHECK(r <= 8 * g->m + 1024, APSPIE_ERR_DEVICE, "SSSP did not converge");
-        chunk = std::min(chunk * 2, 256);
+        chunk = grow;
+        grow = std::min(grow * 2, 256);
     }
     c->stats.edges_traversed = (int64_t)h[4];
     c->stats.vertices_reached = (int64_t)h[5];
