@@ -89,6 +89,7 @@ struct Ctx {
     bool profiling = false;
     int cus = 256;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool ev1_recorded = false;  // the call recorded ev1 after its last work itself
     // pinned host word for counters read back by the host
     void *pinned = nullptr;
     // the default MIS priorities (mt19937 seeded 5489, one draw per vertex)
