@@ -20,7 +20,7 @@ import numpy as np
 __all__ = ["Context", "Graph", "ApspieError", "lib", "HOST", "DEVICE", "SSSP_INF",
            "SR_PLUS_TIMES", "SR_BOOL", "SR_MIN_PLUS", "SR_MAX_FIRST",
            "PATH_AUTO", "PATH_SORT", "PATH_ATOMIC",
-           "gen_rmat", "gen_lattice", "gen_weights", "read_mtx"]
+           "gen_rmat", "gen_lattice", "gen_weights", "read_mtx", "csr_write", "csr_read"]
 
 LIB_PATH = Path(__file__).resolve().parent / "libapspie.so"
 
@@ -127,7 +127,18 @@ def _declare(L):
         "apspie_gen_weights": ([i32, vp, vp, u32, vp], C.c_int),
         "apspie_read_mtx": ([C.c_char_p, C.POINTER(i32), C.POINTER(i32), C.POINTER(i64), vp, vp,
                              vp], C.c_int),
+        "apspie_csr_write": ([C.c_char_p, C.POINTER(_Csr)], C.c_int),
+        "apspie_csr_read": ([C.c_char_p, C.POINTER(i32), C.POINTER(i32), C.POINTER(i64),
+                             C.POINTER(i32), vp, vp, vp], C.c_int),
+        "apspie_graph_save": ([vp, vp, C.c_char_p], C.c_int),
+        "apspie_graph_load": ([vp, C.c_char_p, pp], C.c_int),
         "apspie_server_create": ([C.c_int, C.POINTER(_ServerCfg), pp], C.c_int),
+        "apspie_server_add_graph_coo": ([vp, C.POINTER(_Coo), C.c_int, C.c_int, C.POINTER(i32)],
+                                        C.c_int),
+        "apspie_server_add_graph_file": ([vp, C.c_char_p, C.POINTER(i32)], C.c_int),
+        "apspie_server_last_error": ([vp], C.c_char_p),
+        "apspie_server_busy": ([vp, i32, C.c_int, C.POINTER(i64), C.POINTER(i64),
+                                C.POINTER(f64), C.c_int], C.c_int),
         "apspie_server_destroy": ([vp], C.c_int),
         "apspie_server_add_graph": ([vp, C.POINTER(_Csr), C.POINTER(i32)], C.c_int),
         "apspie_server_submit": ([vp, i32, C.c_int, i32, C.POINTER(i64)], C.c_int),
@@ -210,6 +221,33 @@ def read_mtx(path):
     return m.value, n.value, row[:k], col[:k], val[:k]
 
 
+def csr_write(path, rowptr, col, val=None, n=None):
+    """Write host CSR arrays as a binary CSR cache file."""
+    rowptr = np.ascontiguousarray(rowptr, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    val = None if val is None else np.ascontiguousarray(val, np.float32)
+    m = len(rowptr) - 1
+    csr = _Csr(m, m if n is None else n, len(col), _p(rowptr).value,
+               _p(col).value if len(col) else None, None if val is None else _p(val).value)
+    _check(lib().apspie_csr_write(str(path).encode(), C.byref(csr)))
+
+
+def csr_read(path):
+    """(rowptr, col, val or None, n) of a binary CSR cache file."""
+    L = lib()
+    bp = str(path).encode()
+    m, n, z, hv = C.c_int32(), C.c_int32(), C.c_int64(), C.c_int32()
+    _check(L.apspie_csr_read(bp, C.byref(m), C.byref(n), C.byref(z), C.byref(hv), None, None,
+                             None))
+    rowptr = np.empty(m.value + 1, np.int32)
+    col = np.empty(max(z.value, 1), np.int32)
+    val = np.empty(max(z.value, 1), np.float32) if hv.value else None
+    _check(L.apspie_csr_read(bp, C.byref(m), C.byref(n), C.byref(z), C.byref(hv), _p(rowptr),
+                             _p(col), _p(val)))
+    k = z.value
+    return rowptr, col[:k], (None if val is None else val[:k]), n.value
+
+
 # ------------------------------------------------------------------- context
 class Context:
     """One HIP stream, its scratch memory and tuning table on one device."""
@@ -283,6 +321,12 @@ class Context:
 
     def from_coo(self, m, row, col, val=None, n=None, symmetrize=True):
         return Graph.from_coo(self, m, row, col, val, n, symmetrize)
+
+    def load(self, path):
+        """A graph read from a binary CSR cache file into HBM."""
+        h = C.c_void_p()
+        _check(self._L.apspie_graph_load(self.h, str(path).encode(), C.byref(h)), self.h)
+        return Graph(self, h)
 
     # -- algorithms (host numpy in/out unless mem=DEVICE with raw pointers)
     def bfs(self, g, source, out=None, mem=HOST):
@@ -466,6 +510,10 @@ class Graph:
         n = L.value
         return (pp, ci[:n], co[:n + 1], ri[:self.nnz],
                 None if vv is None else vv[:self.nnz])
+
+    def save(self, path):
+        """Write the graph as a binary CSR cache file."""
+        _check(lib().apspie_graph_save(self.ctx.h, self.h, str(path).encode()), self.ctx.h)
 
     def transpose(self):
         h = C.c_void_p()

@@ -432,21 +432,23 @@ __global__ __launch_bounds__(kBlock) void k_batch_sssp_nf(
     constexpr int B = 32 * W;
     __shared__ BatchLds lds;
     __shared__ uint64_t s_part[kBlock / 64];
+    __shared__ uint32_t s_min[kBlock / 64];
     const uint64_t cc = ctl[r % 3];
     const int32_t F = (int32_t)cnt_entries(cc), E = (int32_t)cnt_edges(cc);
     const int32_t tbits = (int32_t)ctl[12 + (r & 1)];
     const bool split_last = ctl[14 + (r & 1)] != 0;
-    uint64_t left_before = 0;
-    for (int i = 0; i < kSpread; ++i) left_before += ctl[16 + i * 16 + (r + 2) % 3];
+    const uint64_t left_before = nf_left(ctl, r + 2);
     const bool split = F == 0;
     const bool done = split && split_last && left_before == 0;
-    const int32_t tnext = split && !done ? dev::fbits(dev::bitsf(tbits) + delta) : tbits;
+    const int32_t tnext =
+        split && !done ? nf_next_threshold(tbits, delta, split_last, nf_min_left(ctl, r + 2))
+                       : tbits;
     __syncthreads();  // every thread has read the control words
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl[(r + 2) % 3] = 0;
         ctl[12 + ((r + 1) & 1)] = (uint64_t)(uint32_t)tnext;
         ctl[14 + ((r + 1) & 1)] = split ? 1 : 0;
-        for (int i = 0; i < kSpread; ++i) ctl[16 + i * 16 + (r + 1) % 3] = 0;
+        nf_reset(ctl, r + 1);
         if (F > 0) {
             ctl[4] += (uint64_t)E;
             ctl[5] += (uint64_t)F;
@@ -471,6 +473,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_sssp_nf(
     const uint64_t gmask = W == 64 ? ~0ull : (((1ull << W) - 1) << ((lane / W) * W));
     const int32_t round = *op.lvl;
     uint64_t left = 0;
+    uint32_t min_left = ~0u;  // smallest distance left pending (bits)
     for (int64_t pbase = (int64_t)blockIdx.x * VPB * KP; pbase < m;
          pbase += (int64_t)gridDim.x * VPB * KP) {
         int32_t cv[KP], crs[KP], cd[KP];
@@ -491,8 +494,12 @@ __global__ __launch_bounds__(kBlock) void k_batch_sssp_nf(
                         x &= x - 1;
                         const int32_t d = op.dist[v * B + w * 32 + b];
                         if (d < tbits) continue;  // lowered below T since: expanded
-                        if (d < tnext) moved |= 1u << b;
-                        else keep |= 1u << b;
+                        if (d < tnext) {
+                            moved |= 1u << b;
+                        } else {
+                            keep |= 1u << b;
+                            min_left = min(min_left, (uint32_t)d);
+                        }
                     }
                     op.pend[i] = keep;
                     left += (uint64_t)__popc(keep);
@@ -528,12 +535,20 @@ __global__ __launch_bounds__(kBlock) void k_batch_sssp_nf(
     }
     if (q.n > 0) queue_flush(q, nxt, ctl + (r + 1) % 3);
     const uint64_t ws = dev::wave_sum_u64(left);
-    if (lane == 0) s_part[threadIdx.x / 64] = ws;
+    const uint32_t wm = dev::wave_min_u32(min_left);
+    if (lane == 0) {
+        s_part[threadIdx.x / 64] = ws;
+        s_min[threadIdx.x / 64] = wm;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint64_t t = 0;
-        for (int qq = 0; qq < kBlock / 64; ++qq) t += s_part[qq];
-        if (t) dev::atomic_add_dev(ctl + 16 + (blockIdx.x % kSpread) * 16 + r % 3, t);
+        uint32_t mn = ~0u;
+        for (int qq = 0; qq < kBlock / 64; ++qq) {
+            t += s_part[qq];
+            mn = min(mn, s_min[qq]);
+        }
+        nf_report(ctl, r, t, mn);
     }
 }
 
@@ -702,7 +717,7 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
     const int grid = fill_grid(c, kBlock, (int)opt(c, "batch_blocks_per_cu", 4));
     // near-far rounds for SSSP (threshold step batch_sssp_delta, 0 = off,
     // -1 = chosen per graph)
-    int64_t bdelta = SSSP ? opt(c, "batch_sssp_delta", -1) : 0;
+    double bdelta = SSSP ? (double)opt(c, "batch_sssp_delta", -1) : 0;
     if (bdelta < 0) bdelta = sssp_batch_auto_delta(c, const_cast<Graph *>(g));
     const float delta = (float)bdelta;
     const bool near_far = SSSP && delta > 0;
@@ -807,9 +822,8 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
         c->stats.host_syncs++;
         if (!near_far && cnt_entries(h[l % 3]) == 0) break;
         if (near_far) {  // the last round was a split that moved and left nothing
-            uint64_t left = 0;
-            for (int i = 0; i < kSpread; ++i) left += h[16 + i * 16 + (l + 2) % 3];
-            if (cnt_entries(h[l % 3]) == 0 && h[14 + (l & 1)] != 0 && left == 0) break;
+            if (cnt_entries(h[l % 3]) == 0 && h[14 + (l & 1)] != 0 && nf_left(h, l + 2) == 0)
+                break;
         }
         if (l > 8 * (int64_t)m + 4096) {
             if (exec) (void)hipGraphExecDestroy(exec);

@@ -173,10 +173,10 @@ int32_t graph_max_degree(Ctx *c, Graph *g);
 bool graph_weight_range(Ctx *c, Graph *g, float *lo, float *hi);
 // Near-far split width for sssp_delta = -1 (0 = Bellman-Ford rounds), and the
 // width the weights alone suggest (what the autotuner centres its candidates on).
-int64_t sssp_auto_delta(Ctx *c, Graph *g);
-int64_t sssp_weight_delta(Ctx *c, Graph *g);
+double sssp_auto_delta(Ctx *c, Graph *g);
+double sssp_weight_delta(Ctx *c, Graph *g);
 // The batched SSSP width for batch_sssp_delta = -1 (0 = Bellman-Ford rounds).
-int64_t sssp_batch_auto_delta(Ctx *c, Graph *g);
+double sssp_batch_auto_delta(Ctx *c, Graph *g);
 
 }  // namespace apspie
 

@@ -166,6 +166,20 @@ static void check_graph(Ctx *c, const Graph *g) {
     APSPIE_CHECK(g->device == c->device, APSPIE_ERR_INVALID, "graph lives on another device");
 }
 
+// SSSP orders distances by their bit patterns (an int32 atomic min), which
+// orders finite non-negative floats only; the reference's CPU SSSP
+// (testSssp.cpp:86-100) would accept negative edges, which this path cannot
+// order, so they are refused instead of answered wrongly.  The range is
+// reduced on the device once per graph.
+static void check_sssp_weights(Ctx *c, const Graph *g) {
+    APSPIE_CHECK(g->val, APSPIE_ERR_INVALID, "SSSP needs edge weights");
+    float lo = 0.0f, hi = 0.0f;
+    if (!graph_weight_range(c, const_cast<Graph *>(g), &lo, &hi)) return;  // no entries
+    APSPIE_CHECK(lo >= 0.0f && hi <= 3.402823466e+38f, APSPIE_ERR_UNSUPPORTED,
+                 "SSSP needs finite non-negative weights (smallest " + std::to_string(lo) +
+                     ", largest " + std::to_string(hi) + ")");
+}
+
 // device time of one algorithm call, bracketed by events on the context stream
 template <class F>
 static void timed(Ctx *c, F &&f) {
@@ -594,9 +608,9 @@ int apspie_sssp(apspie_ctx *c, const apspie_graph *g, int32_t source, float *dis
     return guard(c, [&] {
         check_graph(c, g);
         APSPIE_CHECK(source >= 0 && source < g->m, APSPIE_ERR_INVALID, "source out of range");
-        APSPIE_CHECK(g->val, APSPIE_ERR_INVALID, "SSSP needs edge weights");
         APSPIE_CHECK(g->m == g->n, APSPIE_ERR_INVALID, "SSSP needs a square matrix");
         set_device(c);
+        check_sssp_weights(c, g);
         float *out = mem == APSPIE_MEM_DEVICE ? dist
                                               : (float *)scratch(c, "sssp_out", sizeof(float) * g->m);
         timed(c, [&] { sssp_run(c, g, source, out); });
@@ -839,12 +853,12 @@ int apspie_sssp_batch(apspie_ctx *c, const apspie_graph *g, const int32_t *sourc
     if (!c || !sources || !dist || nsrc <= 0) return APSPIE_ERR_INVALID;
     return guard(c, [&] {
         check_graph(c, g);
-        APSPIE_CHECK(g->val, APSPIE_ERR_INVALID, "SSSP needs edge weights");
         APSPIE_CHECK(g->m == g->n, APSPIE_ERR_INVALID, "SSSP needs a square matrix");
         for (int32_t i = 0; i < nsrc; ++i)
             APSPIE_CHECK(sources[i] >= 0 && sources[i] < g->m, APSPIE_ERR_INVALID,
                          "source out of range");
         set_device(c);
+        check_sssp_weights(c, g);
         size_t bytes = sizeof(float) * (size_t)nsrc * g->m;
         float *out = mem == APSPIE_MEM_DEVICE ? dist : (float *)scratch(c, "ssspb_out", bytes);
         timed(c, [&] { sssp_batch_run(c, g, sources, nsrc, out); });

@@ -71,6 +71,15 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) {
     return x;
 }
 
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        uint32_t y = __shfl_xor(x, d, 64);
+        x = x < y ? x : y;
+    }
+    return x;
+}
+
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);

@@ -255,6 +255,49 @@ inline uint64_t spread_sum(const uint64_t *h) {
     return t;
 }
 
+// Near-far split bookkeeping in the spread slots: the split of round r adds
+// the count of pending entries it leaves to slot r % 3 and lowers slot
+// 3 + r % 3 to the smallest of their distances (non-negative float bits;
+// ~0 = none).  Round r reads the slots of round r - 1 and resets those of
+// round r + 1, so no round reads a word it writes.
+__host__ __device__ inline uint64_t nf_left(const uint64_t *ctl, int r) {
+    uint64_t t = 0;
+    for (int i = 0; i < kSpread; ++i) t += ctl[16 + i * 16 + r % 3];
+    return t;
+}
+__device__ __forceinline__ uint64_t nf_min_left(const uint64_t *ctl, int r) {
+    uint64_t t = ~0ull;
+    for (int i = 0; i < kSpread; ++i) t = min(t, ctl[16 + i * 16 + 3 + r % 3]);
+    return t;
+}
+__device__ __forceinline__ void nf_reset(uint64_t *ctl, int r) {
+    for (int i = 0; i < kSpread; ++i) {
+        ctl[16 + i * 16 + r % 3] = 0;
+        ctl[16 + i * 16 + 3 + r % 3] = ~0ull;
+    }
+}
+// one block's share of a split's result (called by one thread)
+__device__ __forceinline__ void nf_report(uint64_t *ctl, int r, uint64_t left, uint32_t min_bits) {
+    uint64_t *row = ctl + 16 + (blockIdx.x % kSpread) * 16;
+    if (left) {
+        dev::atomic_add_dev(row + r % 3, left);
+        __hip_atomic_fetch_min(row + 3 + r % 3, (uint64_t)min_bits, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+// The threshold a split round raises T (float bits, non-negative) to: T +
+// delta, at least the next float above T, and -- when the previous round was
+// a split that expanded nothing -- above the smallest distance that split
+// left pending, so that this split moves it.
+__device__ __forceinline__ int32_t nf_next_threshold(int32_t tbits, float delta, bool jump,
+                                                     uint64_t min_left) {
+    int32_t t = dev::fbits(dev::bitsf(tbits) + delta);
+    if (t <= tbits) t = tbits + 1;
+    if (jump && min_left != ~0ull && (int64_t)min_left + 1 > (int64_t)t)
+        t = (int32_t)min_left + 1;
+    return t;
+}
+
 // Frontier storage sized for a graph: two ping-pong frontiers, a ring of
 // three counters and a stats block.
 struct FrontierSet {

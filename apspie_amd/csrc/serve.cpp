@@ -78,6 +78,9 @@ struct Request {
     int32_t graph = 0;
     int kind = 0;
     int32_t source = 0;
+    int64_t bytes = 0;  // result size, fixed at submit
+    const apspie_graph *g = nullptr;  // resident graph and its vertex count, fixed at submit
+    int32_t m = 0;
     Clock::time_point t_submit, t_done;
     void *buf = nullptr;    // pinned result (BFS depths / SSSP distances)
     uint64_t triangles = 0;  // TC result
@@ -85,10 +88,18 @@ struct Request {
     bool done = false;
 };
 
+// Work done per (graph, kind): batches launched, queries answered and the
+// wall time workers spent on them (launch to results in host memory).
+struct Busy {
+    int64_t batches = 0, queries = 0;
+    double busy_ms = 0;
+};
+
 struct Resident {
     apspie_graph *g = nullptr;
     int32_t m = 0;
     int64_t bytes = 0;
+    Busy busy[3];  // by query kind
 };
 
 }  // namespace
@@ -97,8 +108,11 @@ struct apspie_server {
     int device = 0;
     apspie_server_config cfg{};
     apspie_ctx *admin = nullptr;
-    std::vector<Resident> graphs;
+    // a deque: admitting a graph never moves the residents that queued
+    // requests and running batches refer to
+    std::deque<Resident> graphs;
     int64_t graph_bytes = 0;
+    std::mutex admin_mu;  // admissions share the admin context, one at a time
     // each worker's scratch budget: what the resident graphs leave of the
     // server's budget, split evenly (0 = unlimited); a worker applies it to
     // its own context before each batch
@@ -119,12 +133,18 @@ struct apspie_server {
 
 namespace {
 
-int64_t result_bytes(const apspie_server *s, int kind, int32_t graph) {
-    return kind == APSPIE_Q_TC ? (int64_t)sizeof(uint64_t) : 4 * (int64_t)s->graphs[graph].m;
+int64_t result_bytes(int kind, int32_t m) {
+    return kind == APSPIE_Q_TC ? (int64_t)sizeof(uint64_t) : 4 * (int64_t)m;
 }
 
+// The batch's graph and size come from its requests (fixed at submit under
+// the server lock), so a batch reads nothing of the resident pool while
+// graphs are being admitted.
 int run_batch(apspie_server *s, apspie_ctx *ctx, std::vector<std::shared_ptr<Request>> &batch) {
-    const Resident &R = s->graphs[batch[0]->graph];
+    struct {
+        const apspie_graph *g;
+        int32_t m;
+    } R{batch[0]->g, batch[0]->m};
     const int kind = batch[0]->kind;
     const size_t n = batch.size();
     const size_t row = 4 * (size_t)R.m;
@@ -178,6 +198,7 @@ int run_batch(apspie_server *s, apspie_ctx *ctx, std::vector<std::shared_ptr<Req
 // drops its cached scratch (buffers sized for earlier, larger batches) and
 // tries again; a batch that still does not fit is split in halves.
 void execute(apspie_server *s, apspie_ctx *ctx, std::vector<std::shared_ptr<Request>> &batch) {
+    const auto t0 = Clock::now();
     int st = run_batch(s, ctx, batch);
     if (st == APSPIE_ERR_BUDGET || st == APSPIE_ERR_OOM) {
         apspie_ctx_trim(ctx);
@@ -198,6 +219,10 @@ void execute(apspie_server *s, apspie_ctx *ctx, std::vector<std::shared_ptr<Requ
         r->t_done = now;
         r->done = true;
     }
+    Busy &b = s->graphs[batch[0]->graph].busy[batch[0]->kind];
+    b.batches += 1;
+    b.queries += (int64_t)batch.size();
+    b.busy_ms += std::chrono::duration<double, std::milli>(now - t0).count();
     s->cv_done.notify_all();
 }
 
@@ -224,7 +249,7 @@ void worker(apspie_server *s, int slot) {
             // the rows copied out) plus frontier bits: keep it within the
             // worker's budget rather than find out by failing
             if (budget > 0 && head->kind != APSPIE_Q_TC) {
-                const int64_t fit = budget / (3 * result_bytes(s, head->kind, head->graph));
+                const int64_t fit = budget / (3 * head->bytes);
                 cap = (int)std::max<int64_t>(1, std::min<int64_t>(cap, fit));
             }
             for (auto it = s->queue.begin(); it != s->queue.end() && (int)batch.size() < cap;) {
@@ -299,32 +324,85 @@ int apspie_server_destroy(apspie_server *s) {
     return APSPIE_OK;
 }
 
-int apspie_server_add_graph(apspie_server *s, const apspie_csr *host, int32_t *graph_id) {
-    if (!s || !host || !graph_id) return APSPIE_ERR_INVALID;
-    const int64_t bytes = 4 * ((int64_t)host->m + 1) + (host->val ? 8 : 4) * host->nnz;
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        // resident graphs may take 60% of the budget; the workers split what
-        // the graphs leave
-        if (s->cfg.hbm_budget > 0 && s->graph_bytes + bytes > (int64_t)(0.6 * s->cfg.hbm_budget))
-            return APSPIE_ERR_BUDGET;
+}  // extern "C"
+
+namespace {
+
+// Resident graphs may take 60% of the budget; the workers split what the
+// graphs leave.
+bool fits(apspie_server *s, int64_t bytes) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    return s->cfg.hbm_budget <= 0 || s->graph_bytes + bytes <= (int64_t)(0.6 * s->cfg.hbm_budget);
+}
+
+// Admit a graph the admin context has just built: check the budget with its
+// actual size, prepare what TC needs, and publish it to the pool.
+int admit(apspie_server *s, apspie_graph *g, int32_t *graph_id) {
+    int32_t m = 0;
+    int64_t bytes = 0;
+    apspie_graph_info(g, &m, nullptr, nullptr, &bytes);
+    if (!fits(s, bytes)) {
+        apspie_graph_destroy(g);
+        return APSPIE_ERR_BUDGET;
     }
-    apspie_graph *g = nullptr;
-    int st = apspie_graph_upload(s->admin, host, &g);
-    if (st != APSPIE_OK) return st;
-    st = apspie::guard(s->admin, [&] { apspie::graph_prepare_tc(s->admin, g); });
+    int st = apspie::guard(s->admin, [&] { apspie::graph_prepare_tc(s->admin, g); });
     if (st != APSPIE_OK && st != APSPIE_ERR_INVALID) {  // unsorted rows: TC fails later
         apspie_graph_destroy(g);
         return st;
     }
     std::lock_guard<std::mutex> lk(s->mu);
-    s->graphs.push_back({g, host->m, bytes});
+    if (s->cfg.hbm_budget > 0 && s->graph_bytes + bytes > (int64_t)(0.6 * s->cfg.hbm_budget)) {
+        apspie_graph_destroy(g);  // another admission took the room meanwhile
+        return APSPIE_ERR_BUDGET;
+    }
+    s->graphs.push_back({g, m, bytes});
     s->graph_bytes += bytes;
     if (s->cfg.hbm_budget > 0)
         s->slot_budget = std::max<int64_t>(1, (s->cfg.hbm_budget - s->graph_bytes) /
                                                   (int64_t)s->ctxs.size());
     *graph_id = (int32_t)s->graphs.size() - 1;
     return APSPIE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int apspie_server_add_graph(apspie_server *s, const apspie_csr *host, int32_t *graph_id) {
+    if (!s || !host || !graph_id) return APSPIE_ERR_INVALID;
+    // refuse before uploading when the arrays alone do not fit
+    if (!fits(s, 4 * ((int64_t)host->m + 1) + (host->val ? 8 : 4) * host->nnz))
+        return APSPIE_ERR_BUDGET;
+    std::lock_guard<std::mutex> ak(s->admin_mu);
+    apspie_graph *g = nullptr;
+    int st = apspie_graph_upload(s->admin, host, &g);
+    if (st != APSPIE_OK) return st;
+    return admit(s, g, graph_id);
+}
+
+int apspie_server_add_graph_coo(apspie_server *s, const apspie_coo *coo, int mem, int symmetrize,
+                                int32_t *graph_id) {
+    if (!s || !coo || !graph_id) return APSPIE_ERR_INVALID;
+    std::lock_guard<std::mutex> ak(s->admin_mu);
+    apspie_graph *g = nullptr;
+    int st = apspie_graph_from_coo(s->admin, coo, mem, symmetrize, &g);
+    if (st != APSPIE_OK) return st;
+    // the build's COO staging and sort buffers are not the graph's: release them
+    apspie_ctx_trim(s->admin);
+    return admit(s, g, graph_id);
+}
+
+int apspie_server_add_graph_file(apspie_server *s, const char *path, int32_t *graph_id) {
+    if (!s || !path || !graph_id) return APSPIE_ERR_INVALID;
+    std::lock_guard<std::mutex> ak(s->admin_mu);
+    apspie_graph *g = nullptr;
+    int st = apspie_graph_load(s->admin, path, &g);
+    if (st != APSPIE_OK) return st;
+    return admit(s, g, graph_id);
+}
+
+const char *apspie_server_last_error(apspie_server *s) {
+    return s ? apspie_last_error(s->admin) : "no server";
 }
 
 int apspie_server_submit(apspie_server *s, int32_t graph_id, int kind, int32_t source,
@@ -334,14 +412,17 @@ int apspie_server_submit(apspie_server *s, int32_t graph_id, int kind, int32_t s
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (graph_id < 0 || graph_id >= (int32_t)s->graphs.size()) return APSPIE_ERR_INVALID;
-        if (kind != APSPIE_Q_TC && (source < 0 || source >= s->graphs[graph_id].m))
-            return APSPIE_ERR_INVALID;
+        const Resident &R = s->graphs[graph_id];
+        if (kind != APSPIE_Q_TC && (source < 0 || source >= R.m)) return APSPIE_ERR_INVALID;
         if (s->cfg.queue_capacity > 0 && (int)s->queue.size() >= s->cfg.queue_capacity)
             return APSPIE_ERR_BUSY;
         r->ticket = s->next_ticket++;
         r->graph = graph_id;
         r->kind = kind;
         r->source = source;
+        r->g = R.g;
+        r->m = R.m;
+        r->bytes = result_bytes(kind, R.m);
         r->t_submit = Clock::now();
         s->queue.push_back(r);
         s->live[r->ticket] = r;
@@ -364,7 +445,7 @@ int apspie_server_wait(apspie_server *s, int64_t ticket, void *out, int64_t out_
         *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
     lk.unlock();
     int st = r->status;
-    const int64_t need = result_bytes(s, r->kind, r->graph);
+    const int64_t need = r->bytes;
     if (st == APSPIE_OK && out) {
         if (out_bytes < need) st = APSPIE_ERR_INVALID;
         else if (r->kind == APSPIE_Q_TC) std::memcpy(out, &r->triangles, sizeof r->triangles);
@@ -394,7 +475,7 @@ int apspie_server_wait_view(apspie_server *s, int64_t ticket, const void **data,
         r->buf = nullptr;
         return r->status;
     }
-    *bytes = result_bytes(s, r->kind, r->graph);
+    *bytes = r->bytes;
     *data = r->kind == APSPIE_Q_TC ? (const void *)&r->triangles : r->buf;
     s->viewed[ticket] = r;
     return APSPIE_OK;
@@ -412,6 +493,19 @@ int apspie_server_release(apspie_server *s, int64_t ticket) {
     }
     s->pinned.put(r->buf);
     r->buf = nullptr;
+    return APSPIE_OK;
+}
+
+int apspie_server_busy(apspie_server *s, int32_t graph_id, int kind, int64_t *batches,
+                       int64_t *queries, double *busy_ms, int reset) {
+    if (!s || kind < APSPIE_Q_BFS || kind > APSPIE_Q_TC) return APSPIE_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (graph_id < 0 || graph_id >= (int32_t)s->graphs.size()) return APSPIE_ERR_INVALID;
+    Busy &b = s->graphs[graph_id].busy[kind];
+    if (batches) *batches = b.batches;
+    if (queries) *queries = b.queries;
+    if (busy_ms) *busy_ms = b.busy_ms;
+    if (reset) b = Busy{};
     return APSPIE_OK;
 }
 

@@ -377,8 +377,17 @@ void sssp_do(Ctx *c, const Graph *g0, int32_t source, float *dist_out, int32_t p
 // ctl words (every word a round reads was written by the previous round, and
 // no word a round reads is written in that round): [0..2] frontier counter
 // ring, [4..7] stats, [12 + (r & 1)] T of round r (float bits), [14 + (r & 1)]
-// 1 if round r - 1 was a split, and the pending-vertex count left by the
-// split of round r in the spread slots [16 + 16 i + r % 3].
+// 1 if round r - 1 was a split, and, in the spread slots, the pending-vertex
+// count left by the split of round r ([16 + 16 i + r % 3]) and the smallest
+// distance among them ([16 + 16 i + 3 + r % 3], float bits, ~0 if none).
+//
+// The threshold of a split is T + delta, at least the next float above T (T +
+// delta rounds to T once T exceeds delta * 2^24).  A split that follows a
+// split that expanded nothing jumps further, to just above the smallest
+// pending distance the earlier sweep left: with weights much larger than
+// delta, T would otherwise climb in maxdist / delta empty splits.  Any
+// threshold above T gives the same fixed point; the jump only bounds the
+// rounds (every second consecutive split moves a vertex).
 constexpr int kNfBlock = 1024;
 constexpr int kNfPer = 8;  // vertices per thread per sweep pass
 
@@ -401,10 +410,6 @@ struct SsspNearOp {
     }
 };
 
-__device__ __forceinline__ uint64_t *pend_slot(uint64_t *ctl, int r) {
-    return ctl + 16 + (blockIdx.x % kSpread) * 16 + r % 3;
-}
-
 // two 1024-thread blocks per CU resident: at most 64 VGPRs
 template <int TILE>
 __global__ __launch_bounds__(kNfBlock) __attribute__((amdgpu_waves_per_eu(8))) void
@@ -414,26 +419,30 @@ k_sssp_nf_round(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ 
                 FrontierBufs nxt, uint64_t *ctl, int32_t r) {
     __shared__ ExpandLds<kNfBlock, TILE, true> lds;
     __shared__ uint64_t s_part[kNfBlock / 64];
+    __shared__ uint32_t s_min[kNfBlock / 64];
     const uint64_t cc = ctl[r % 3];
     const int32_t F = (int32_t)cnt_entries(cc), E = (int32_t)cnt_edges(cc);
     const int32_t tbits = (int32_t)ctl[12 + (r & 1)];
     const bool split_last = ctl[14 + (r & 1)] != 0;
-    uint64_t left_before = 0;  // pending count left by the previous round's split
-    for (int i = 0; i < kSpread; ++i) left_before += ctl[16 + i * 16 + (r + 2) % 3];
+    const uint64_t left_before = nf_left(ctl, r + 2);  // left by the previous round's split
     const bool split = F == 0;
     const bool done = split && split_last && left_before == 0;
-    const int32_t tnext = split && !done ? dev::fbits(dev::bitsf(tbits) + delta) : tbits;
+    const int32_t tnext =
+        split && !done ? nf_next_threshold(tbits, delta, split_last, nf_min_left(ctl, r + 2))
+                       : tbits;
     __syncthreads();  // every thread has read the control words
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl[(r + 2) % 3] = 0;
         ctl[12 + ((r + 1) & 1)] = (uint64_t)(uint32_t)tnext;
         ctl[14 + ((r + 1) & 1)] = split ? 1 : 0;
-        for (int i = 0; i < kSpread; ++i) ctl[16 + i * 16 + (r + 1) % 3] = 0;
+        nf_reset(ctl, r + 1);
         if (F > 0) {
             ctl[4] += (uint64_t)E;
             ctl[5] += (uint64_t)F;
             ctl[6] += 1;
             ctl[7] += (uint64_t)E;  // CSR entries read
+        } else if (!done) {
+            ctl[6] += 1;  // split rounds count as rounds
         }
     }
     if (done) return;
@@ -449,6 +458,7 @@ k_sssp_nf_round(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ 
     }
     // split: move the pending vertices below the new threshold into the frontier
     uint64_t left = 0;
+    uint32_t min_left = ~0u;  // smallest distance left pending (bits)
     for (int64_t pbase = (int64_t)blockIdx.x * kNfBlock * kNfPer; pbase < m;
          pbase += (int64_t)gridDim.x * kNfBlock * kNfPer) {
         int32_t cv[kNfPer], crs[kNfPer], cd[kNfPer];
@@ -464,7 +474,10 @@ k_sssp_nf_round(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ 
                 const int32_t dv = dist[v];
                 if (dv < tbits) drop[k] = true;  // lowered below T since: already expanded
                 else if (dv < tnext) take[k] = true;
-                else ++left;
+                else {
+                    ++left;
+                    min_left = min(min_left, (uint32_t)dv);
+                }
             }
         }
 #pragma unroll
@@ -479,12 +492,20 @@ k_sssp_nf_round(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ 
                                               &lds.base);
     }
     const uint64_t ws = dev::wave_sum_u64(left);
-    if (dev::lane_id() == 0) s_part[threadIdx.x / 64] = ws;
+    const uint32_t wm = dev::wave_min_u32(min_left);
+    if (dev::lane_id() == 0) {
+        s_part[threadIdx.x / 64] = ws;
+        s_min[threadIdx.x / 64] = wm;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint64_t t = 0;
-        for (int q = 0; q < kNfBlock / 64; ++q) t += s_part[q];
-        if (t) dev::atomic_add_dev(pend_slot(ctl, r), t);
+        uint32_t mn = ~0u;
+        for (int q = 0; q < kNfBlock / 64; ++q) {
+            t += s_part[q];
+            mn = min(mn, s_min[q]);
+        }
+        nf_report(ctl, r, t, mn);
     }
 }
 
@@ -527,9 +548,7 @@ void sssp_near_far(Ctx *c, const Graph *g, int32_t source, float *dist_out, floa
         c->stats.host_syncs++;
         // finished when the last round was a split that left nothing pending
         // and found nothing below the new threshold
-        uint64_t left = 0;
-        for (int i = 0; i < kSpread; ++i) left += h[16 + i * 16 + (r + 2) % 3];
-        if (cnt_entries(h[r % 3]) == 0 && h[14 + (r & 1)] != 0 && left == 0) break;
+        if (cnt_entries(h[r % 3]) == 0 && h[14 + (r & 1)] != 0 && nf_left(h, r + 2) == 0) break;
         APSPIE_CHECK(r <= 8 * g->m + 1024, APSPIE_ERR_DEVICE, "SSSP did not converge");
         chunk = grow;
         grow = std::min(grow * 2, 256);
@@ -544,19 +563,21 @@ void sssp_near_far(Ctx *c, const Graph *g, int32_t source, float *dist_out, floa
 }  // namespace
 
 // The near-far split width a graph's weights suggest: ten times the largest
-// weight over the mean row length, rounded to the nearest power of two in
-// [16, 4096] -- 64 on R-MAT-22 at weights 1..255 (mean row 30; 48-64
-// measured best, 8 % under Bellman-Ford) and 512 on 2D lattices (mean row
-// 3.6; 1.35x and 2.5x under Bellman-Ford at 2048^2 and 4096^2).  0 when the
-// weights are not positive or lie within a factor of two of each other (a
-// vertex then settles in the round that first reaches it).
-int64_t sssp_weight_delta(Ctx *c, Graph *g) {
+// weight over the mean row length, rounded to the nearest power of two of at
+// least 16 -- 64 on R-MAT-22 at weights 1..255 (mean row 30; 48-64 measured
+// best, 8 % under Bellman-Ford) and 512 on 2D lattices (mean row 3.6; 1.35x
+// and 2.5x under Bellman-Ford at 2048^2 and 4096^2).  No upper cap: the width
+// follows the weights' scale (2^32 at weights up to 1e10 on R-MAT), so the
+// number of splits does not grow with the weights.  0 when the weights are
+// not positive or lie within a factor of two of each other (a vertex then
+// settles in the round that first reaches it).
+double sssp_weight_delta(Ctx *c, Graph *g) {
     float lo = 0.0f, hi = 0.0f;
     if (g->m == 0 || !graph_weight_range(c, g, &lo, &hi) || !(lo > 0.0f) || !(hi >= 2.0f * lo))
         return 0;
     const double want = 10.0 * hi / std::max(1.0, (double)g->nnz / g->m);
-    int64_t d = 16;
-    while (d < 4096 && (double)d * 1.4142135623730951 < want) d *= 2;
+    double d = 16;
+    while (d < 0x1p100 && d * 1.4142135623730951 < want) d *= 2;
     return d;
 }
 
@@ -565,7 +586,7 @@ int64_t sssp_weight_delta(Ctx *c, Graph *g) {
 // re-expansions near-far saves (measured at weights 1..255: R-MAT 16-21 and
 // lattices 512^2-1024^2 run as fast or faster without splits), else the
 // width the weights suggest.
-int64_t sssp_auto_delta(Ctx *c, Graph *g) {
+double sssp_auto_delta(Ctx *c, Graph *g) {
     return g->m < (1 << 22) ? 0 : sssp_weight_delta(c, g);
 }
 
@@ -576,13 +597,13 @@ int64_t sssp_auto_delta(Ctx *c, Graph *g) {
 // cost more than the re-expansions they save -- 32 / 128 sources of R-MAT-22
 // measured 2.64 / 2.09 ms per query without splits, 2.68 / 2.24 at width
 // 256, 4.57 / 3.05 at 64).
-int64_t sssp_batch_auto_delta(Ctx *c, Graph *g) {
-    const int64_t d = sssp_weight_delta(c, g);
+double sssp_batch_auto_delta(Ctx *c, Graph *g) {
+    const double d = sssp_weight_delta(c, g);
     return d >= 256 ? d : 0;
 }
 
 void sssp_run(Ctx *c, const Graph *g, int32_t source, float *dist) {
-    int64_t delta = opt(c, "sssp_delta", -1);
+    double delta = (double)opt(c, "sssp_delta", -1);
     if (delta < 0) delta = sssp_auto_delta(c, const_cast<Graph *>(g));
     if (delta > 0) {
         switch (opt(c, "bfs_tile", 2048)) {

@@ -77,8 +77,10 @@ extern "C" int apspie_autotune(apspie_ctx *c, const apspie_graph *g, int32_t bat
                 }
                 apspie_ctx_set_option(c, "bfs_alpha", ba);
                 if (g->val) {
-                    const int64_t d = std::max<int64_t>(
-                        apspie::sssp_weight_delta(c, const_cast<apspie_graph *>(g)), 64);
+                    const int64_t d = (int64_t)std::min<double>(
+                        std::max<double>(
+                            apspie::sssp_weight_delta(c, const_cast<apspie_graph *>(g)), 64),
+                        1 << 29);
                     int64_t bs = -1;
                     best = 1e300;
                     for (int64_t delta : {(int64_t)0, d / 2, d, 2 * d}) {

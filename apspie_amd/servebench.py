@@ -4,8 +4,12 @@ uniformly random graph and source, submitted to the continuous-batching
 server; reports queries/s and p50/p99 latency (submit -> result in pinned host
 memory, handed out without a copy).
 
-The same stream is replayed against the plain form (one worker, one query per
-launch, no coalescing) for comparison.
+The same stream is replayed against two plain forms (one query per launch,
+no coalescing): with one worker (the drop-in loop) and with as many workers
+as the served form, so that the gain from concurrency and the gain from
+batching show separately.  Each form's busy time is broken down per (graph,
+kind).  An untimed pass afterwards checks the served results against the
+plain ones query by query (xxh3 digests of the result bytes).
 """
 from __future__ import annotations
 
@@ -13,12 +17,14 @@ import threading
 import time
 
 import numpy as np
+import xxhash
 
 from . import Q_BFS, Q_SSSP, Q_TC, gen_lattice, gen_rmat, gen_weights
 from .server import Server
 
 POOL = [("rmat", 16), ("rmat", 18), ("rmat", 20), ("rmat", 22),
         ("lattice", 512), ("lattice", 1024), ("lattice", 2048), ("lattice", 4096)]
+KIND_NAMES = {Q_BFS: "bfs", Q_SSSP: "sssp", Q_TC: "tc"}
 
 
 def build_pool(pool=POOL, log=None):
@@ -47,9 +53,14 @@ def make_stream(graphs, n, seed=7):
     return out
 
 
-def run_stream(server, ids, graphs, stream, rate, seed=11):
+def digest(res):
+    return int(res) if isinstance(res, int) else xxhash.xxh3_64_intdigest(res)
+
+
+def run_stream(server, ids, graphs, stream, rate, seed=11, digests=False):
     """Submit `stream` with exponential inter-arrival times at `rate` q/s
-    from a separate thread; wait for all; returns qps and latency stats."""
+    (0: all at once) from a separate thread; wait for all; returns qps and
+    latency stats (and the results' digests when asked, outside timing)."""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate, len(stream)) if rate > 0 else np.zeros(len(stream))
     tickets = [None] * len(stream)
@@ -67,34 +78,65 @@ def run_stream(server, ids, graphs, stream, rate, seed=11):
     th = threading.Thread(target=submitter)
     th.start()
     lat = []
+    dig = []
     for i, (gi, kind, src) in enumerate(stream):
         while tickets[i] is None:
             time.sleep(0.0002)
         # results as views of the server's pinned buffers (no host copy)
         res, ms = server.wait(tickets[i], kind, len(graphs[gi][1]) - 1, copy=False)
+        if digests:
+            dig.append(digest(res))
         del res
         lat.append(ms)
     th.join()
     wall = time.perf_counter() - t0
     lat = np.array(lat)
-    return {"offered_qps": rate, "queries": len(stream), "qps": len(stream) / wall,
-            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "wall_s": wall}
+    out = {"offered_qps": rate, "queries": len(stream), "qps": len(stream) / wall,
+           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "wall_s": wall}
+    return (out, dig) if digests else out
 
 
-def serve_benchmark(graphs, n_queries=300, rates=None, slots=4, max_batch=128,
-                    hbm_budget=64 << 30, log=None):
-    """Plain capacity first (the whole stream, one query per launch), then the
-    served form on the same stream at three offered rates (x1, x4, x16 the
-    plain capacity) and saturated."""
+def busy_breakdown(server, ids, graphs):
+    """Busy time per (graph, kind) since the last reset, largest first."""
+    rows = []
+    for gi, g in enumerate(graphs):
+        for kind in (Q_BFS, Q_SSSP, Q_TC):
+            b, q, ms = server.busy(ids[gi], kind, reset=True)
+            if q:
+                rows.append({"graph": g[0], "kind": KIND_NAMES[kind], "batches": b,
+                             "queries": q, "busy_ms": ms, "ms_per_query": ms / q})
+    total = sum(r["busy_ms"] for r in rows) or 1.0
+    for r in rows:
+        r["share"] = r["busy_ms"] / total
+    return sorted(rows, key=lambda r: -r["busy_ms"])
+
+
+def serve_benchmark(graphs, n_queries=2000, rates=None, slots=4, max_batch=128,
+                    hbm_budget=64 << 30, n_verify=400, log=None):
+    """Plain capacity first (one worker, one query per launch), then the plain
+    form with `slots` workers, then the served form on the same stream at
+    three offered rates (x1, x4, x16 the one-worker plain capacity) and
+    saturated; finally an untimed check of served against plain results."""
     res = {}
     stream = make_stream(graphs, n_queries)
+
+    def say(tag, r):
+        if log:
+            log(f"  {tag}: {r['qps']:.1f} q/s p50 {r['p50_ms']:.1f} ms p99 {r['p99_ms']:.1f} ms")
+
     with Server(slots=1, max_batch=1, hbm_budget=hbm_budget) as plain:
         ids = [plain.add_graph(g[1], g[2], g[3]) for g in graphs]
         r = run_stream(plain, ids, graphs, stream, rate=0)
         res["plain_saturated"] = r
-        if log:
-            log(f"  plain: {r['qps']:.1f} q/s p50 {r['p50_ms']:.1f} ms p99 {r['p99_ms']:.1f} ms")
+        res["plain_busy"] = busy_breakdown(plain, ids, graphs)
+        say("plain, 1 worker", r)
+    with Server(slots=slots, max_batch=1, hbm_budget=hbm_budget) as plainn:
+        ids = [plainn.add_graph(g[1], g[2], g[3]) for g in graphs]
+        r = run_stream(plainn, ids, graphs, stream, rate=0)
+        res["plain_slots_saturated"] = r
+        busy_breakdown(plainn, ids, graphs)
+        say(f"plain, {slots} workers", r)
     cap = res["plain_saturated"]["qps"]
     rates = rates or [cap, 4 * cap, 16 * cap]
     with Server(slots=slots, max_batch=max_batch, hbm_budget=hbm_budget) as srv:
@@ -103,15 +145,27 @@ def serve_benchmark(graphs, n_queries=300, rates=None, slots=4, max_batch=128,
         for rate in rates:
             r = run_stream(srv, ids, graphs, stream, rate=rate)
             res["served"].append(r)
-            if log:
-                log(f"  served @ {rate:.1f} q/s offered: {r['qps']:.1f} q/s p50 {r['p50_ms']:.1f} "
-                    f"ms p99 {r['p99_ms']:.1f} ms")
+            say(f"served @ {rate:.1f} q/s offered", r)
+        busy_breakdown(srv, ids, graphs)
         r = run_stream(srv, ids, graphs, stream, rate=0)
         res["served_saturated"] = r
-        if log:
-            log(f"  served saturated: {r['qps']:.1f} q/s p50 {r['p50_ms']:.1f} ms "
-                f"p99 {r['p99_ms']:.1f} ms")
+        res["served_busy"] = busy_breakdown(srv, ids, graphs)
+        say("served saturated", r)
+        # untimed: the served results of the stream's first n_verify queries
+        part = stream[:n_verify]
+        _, served_dig = run_stream(srv, ids, graphs, part, rate=0, digests=True)
+    with Server(slots=1, max_batch=1, hbm_budget=hbm_budget) as plain:
+        ids = [plain.add_graph(g[1], g[2], g[3]) for g in graphs]
+        _, plain_dig = run_stream(plain, ids, graphs, part, rate=0, digests=True)
+    same = sum(int(a == b) for a, b in zip(served_dig, plain_dig))
+    res["verified"] = {"queries": len(part), "equal_to_plain": same,
+                       "how": "xxh3-64 of each result's bytes, served vs one query per launch"}
+    if log:
+        log(f"  verified: {same}/{len(part)} served results equal the plain ones")
     res["speedup_saturated"] = res["served_saturated"]["qps"] / cap
+    res["speedup_vs_plain_same_slots"] = (res["served_saturated"]["qps"]
+                                          / res["plain_slots_saturated"]["qps"])
     res["config"] = {"slots": slots, "max_batch": max_batch, "hbm_budget": hbm_budget,
+                     "queries_per_run": n_queries,
                      "mix": "60% BFS / 30% SSSP / 10% TC", "pool": [g[0] for g in graphs]}
     return res

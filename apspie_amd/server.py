@@ -7,7 +7,7 @@ import weakref
 
 import numpy as np
 
-from . import Q_BFS, Q_SSSP, Q_TC, _Csr, _ServerCfg, _check, _p, lib
+from . import HOST, Q_BFS, Q_SSSP, Q_TC, ApspieError, _Coo, _Csr, _ServerCfg, _check, _p, lib
 
 __all__ = ["Server"]
 
@@ -51,6 +51,31 @@ class Server:
         _check(self._L.apspie_server_add_graph(self.h, C.byref(csr), C.byref(gid)))
         return gid.value
 
+    def _admit_check(self, st):
+        if st != 0:
+            msg = self._L.apspie_status_str(st).decode()
+            detail = self._L.apspie_server_last_error(self.h).decode()
+            raise ApspieError(st, f"{msg}: {detail}" if detail else msg)
+
+    def add_graph_coo(self, m, row, col, val=None, n=None, symmetrize=True):
+        """Admit a graph built on the device from host COO triples."""
+        row = np.ascontiguousarray(row, np.int32)
+        col = np.ascontiguousarray(col, np.int32)
+        val = None if val is None else np.ascontiguousarray(val, np.float32)
+        coo = _Coo(m, m if n is None else n, len(row), _p(row).value if len(row) else None,
+                   _p(col).value if len(col) else None, None if val is None else _p(val).value)
+        gid = C.c_int32()
+        self._admit_check(self._L.apspie_server_add_graph_coo(self.h, C.byref(coo), HOST,
+                                                              int(symmetrize), C.byref(gid)))
+        return gid.value
+
+    def add_graph_file(self, path):
+        """Admit a graph from a binary CSR cache file."""
+        gid = C.c_int32()
+        self._admit_check(self._L.apspie_server_add_graph_file(self.h, str(path).encode(),
+                                                               C.byref(gid)))
+        return gid.value
+
     def submit(self, graph_id, kind, source=0):
         t = C.c_int64()
         _check(self._L.apspie_server_submit(self.h, graph_id, kind, int(source), C.byref(t)))
@@ -86,6 +111,13 @@ class Server:
         srv = server_ref()
         if srv is not None and srv.h:  # a closed server already freed its buffers
             srv._L.apspie_server_release(srv.h, ticket)
+
+    def busy(self, graph_id, kind, reset=False):
+        """(batches, queries, busy_ms) on one graph for one kind since the last reset."""
+        b, q, ms = C.c_int64(), C.c_int64(), C.c_double()
+        _check(self._L.apspie_server_busy(self.h, graph_id, kind, C.byref(b), C.byref(q),
+                                          C.byref(ms), int(reset)))
+        return b.value, q.value, ms.value
 
     def drain(self):
         _check(self._L.apspie_server_drain(self.h))

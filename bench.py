@@ -65,25 +65,107 @@ def read_traffic(path: Path):
     return float(m.group(1)) if m else None
 
 
+def host_cpu():
+    """The host CPU's model name and the threads this process may use."""
+    model = "unknown"
+    try:
+        for ln in Path("/proc/cpuinfo").read_text().splitlines():
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "nproc": len(os.sched_getaffinity(0)),
+            "online": os.cpu_count()}
+
+
+class one_core:
+    """Pins the calling thread to one core (the last one this process may
+    use, away from the runtime's helper threads on the first) for the
+    single-threaded CPU timings, and restores the affinity afterwards."""
+
+    def __enter__(self):
+        self.saved = os.sched_getaffinity(0)
+        self.core = max(self.saved)
+        os.sched_setaffinity(0, {self.core})
+        return self
+
+    def __exit__(self, *a):
+        os.sched_setaffinity(0, self.saved)
+
+
 def cpu_baseline(rowptr, col, srcs, budget_s=15.0):
     """The oracle's restatement of SimpleReferenceBfs (testBfs.cpp:6-72), one
-    core, on a bounded sample of the same sources."""
+    pinned core, on a bounded sample of the same sources."""
     from oracle import oracle as O
     rp64 = rowptr.astype(np.int64)
     m = len(rowptr) - 1
     deg = np.diff(rowptr)
     edges, t_tot, n = 0, 0.0, 0
-    for s in srcs:
-        t0 = time.perf_counter()
-        d, _ = O.bfs(m, rp64, col, s)
-        t_tot += time.perf_counter() - t0
-        edges += int(deg[d >= 0].sum())
-        n += 1
-        if t_tot > budget_s:
-            break
+    with one_core() as pin:
+        for s in srcs:
+            t0 = time.perf_counter()
+            d, _ = O.bfs(m, rp64, col, s)
+            t_tot += time.perf_counter() - t0
+            edges += int(deg[d >= 0].sum())
+            n += 1
+            if t_tot > budget_s:
+                break
     return {"value": edges / t_tot / 1e9, "unit": "GTEPS", "cores": 1, "kind": "port",
+            "pinned_core": pin.core, "host_cpu": host_cpu(),
             "sample": f"oracle deque BFS (SimpleReferenceBfs restated) from the first {n} of "
                       f"the 64 sources on the same RMAT-22 graph, {t_tot:.1f} s"}
+
+
+def copy_calibration(nbytes, reps=3):
+    """A plain device-to-device copy of a known size through the library."""
+    with A.Context(0) as ctx:
+        src = ctx.dev_alloc(nbytes)
+        dst = ctx.dev_alloc(nbytes)
+        for _ in range(reps):
+            A._check(A.lib().apspie_memcpy(ctx.h, dst.ptr, src.ptr, nbytes, 2), ctx.h)
+        ctx.sync()
+        src.free()
+        dst.free()
+    log(f"copied {nbytes} bytes x{reps} (device to device)")
+
+
+def cpu_config1(ctx):
+    """The CPU reference path on its own case: undirected R-MAT scale 14
+    (16,384 vertices, edge factor 16, seed 1), source = the highest-degree
+    vertex (lowest id on ties), SimpleReferenceBfs (testBfs.cpp:6-72)
+    restated in the oracle, one pinned core; the GPU BFS on the same graph
+    and source beside it."""
+    from oracle import oracle as O
+    rowptr, col = A.gen_rmat(14)
+    m = len(rowptr) - 1
+    deg = np.diff(rowptr)
+    src = int(np.argmax(deg))
+    rp64 = rowptr.astype(np.int64)
+    reps, t = 0, 0.0
+    with one_core() as pin:
+        while t < 2.0 or reps < 5:
+            t0 = time.perf_counter()
+            d, _ = O.bfs(m, rp64, col, src)
+            t += time.perf_counter() - t0
+            reps += 1
+    edges = int(deg[d >= 0].sum())
+    G = ctx.upload(rowptr, col)
+    g = ctx.bfs(G, src)
+    assert np.array_equal(g, d), "GPU BFS differs from the CPU reference path on RMAT-14"
+    gpu = []
+    for _ in range(20):
+        ctx.bfs(G, src)
+        gpu.append(ctx.stats()["elapsed_ms"])
+    G.close()
+    ms = 1e3 * t / reps
+    return {"workload": "bfs_rmat14_maxdeg_source", "vertices": m, "nnz": len(col),
+            "source": src, "edges_traversed": edges, "cpu_ms": ms,
+            "cpu_gteps": edges / (ms / 1e3) / 1e9, "cpu_runs": reps, "cores": 1,
+            "pinned_core": pin.core, "host_cpu": host_cpu(), "kind": "port",
+            "gpu_ms_median": float(np.median(gpu)),
+            "gpu_gteps": edges / (float(np.median(gpu)) / 1e3) / 1e9,
+            "depths_equal": True}
 
 
 def main():
@@ -96,12 +178,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--extras", default="sssp,mis,tc,spgemm,batched,served",
                     help="comma list of the extra workloads to run after the main line")
-    ap.add_argument("--served-queries", type=int, default=240,
-                    help="length of the served query stream (same stream for both forms)")
+    ap.add_argument("--served-queries", type=int, default=2000,
+                    help="length of the served query stream (same stream for every form)")
     ap.add_argument("--batch-graph", type=int, default=1,
                     help="replay batched level chunks from a captured hipGraph (1) or launch "
                          "them one by one (0)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES bytes device to device three times and exit: the "
+                         "reference point that scales rocprofv3's FETCH_SIZE / WRITE_SIZE "
+                         "counters to bytes (each copy dispatch reads and writes BYTES)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the traversals "
               "are measured on one GPU's share; sharding by source or by rows is a design, "
@@ -120,8 +209,14 @@ def main():
     optr = out.ptr
     deg = np.diff(rowptr)
 
+    # the timed steps stride over all 64 sources (K < 64: every 64/K-th one;
+    # K >= 64: the 64 in turn); the warm-up runs on sources that are not timed
+    # when there are any
+    timed = [srcs[(i * 64) // args.steps] if args.steps <= 64 else srcs[i % 64]
+             for i in range(args.steps)]
+    spare = [s for s in srcs if s not in set(timed)] or srcs
     for i in range(args.warmup):
-        ctx.bfs(G, srcs[i % 64], out=optr, mem=A.DEVICE)
+        ctx.bfs(G, spare[i % len(spare)], out=optr, mem=A.DEVICE)
     ctx.sync()
 
     # ---- timed region: K single-source BFS runs.  The dominant kernel
@@ -134,7 +229,7 @@ def main():
     per_ms, per_gteps = [], []
     t_start = time.perf_counter()
     for i in range(args.steps):
-        ctx.bfs(G, srcs[i % 64], out=optr, mem=A.DEVICE)
+        ctx.bfs(G, timed[i], out=optr, mem=A.DEVICE)
         st = ctx.stats()
         edges += st["edges_traversed"]
         per_ms.append(st["elapsed_ms"])  # hipEvent pair around the entry point
@@ -166,7 +261,9 @@ def main():
            "data": "synthetic (R-MAT generator, seed 1)",
            "config": {"workload": f"bfs_rmat{args.scale}_64src", "scale": args.scale,
                       "edge_factor": 16, "abcd": [0.57, 0.19, 0.19, 0.05], "vertices": m,
-                      "nnz": nnz, "sources": 64, "source_rng": "mt19937(3) % m, degree > 0"},
+                      "nnz": nnz, "sources": 64, "source_rng": "mt19937(3) % m, degree > 0",
+                      "timed_sources": len(set(timed)),
+                      "timed_order": "every 64/K-th of the 64 sources for K < 64 steps"},
            "roofline": roofline,
            "per_source": {"median_ms": float(np.median(per_ms)),
                           "p99_ms": float(np.percentile(per_ms, 99)),
@@ -181,6 +278,8 @@ def main():
                               served_queries=args.served_queries)
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(rowptr, col, srcs)
+        rec["cpu_config1"] = cpu_config1(ctx)
+        log(f"  config 1 (CPU path, RMAT-14): {rec['cpu_config1']}")
     else:
         rec["cpu_baseline"] = None
     G.close()
@@ -208,7 +307,7 @@ def extras(ctx, G, rowptr, col, srcs, optr, cpu=True, which=None, served_queries
 
 
 def sssp_rmat22(ctx, G, rowptr, col, srcs, optr, cpu):
-    """SSSP on the same RMAT-22 graph with integer weights 1..255, 16 sources:
+    """SSSP on the same RMAT-22 graph with integer weights 1..255, all 64 sources:
     GTEPS over the edges of the reached vertices, the roofline fraction at
     12 B per traversed edge (column, weight, distance probe) + 20 B per
     reached vertex + 4 B per vertex (init) per source, and one source of the
@@ -220,7 +319,7 @@ def sssp_rmat22(ctx, G, rowptr, col, srcs, optr, cpu):
     Gw = ctx.upload(rowptr, col, w)
     fout = ctx.dev_alloc(4 * m)
     edges, relaxed, t, alg = 0, 0, 0.0, 0.0
-    for s in srcs[:16]:
+    for s in srcs:
         ctx.bfs(G, s, out=optr, mem=A.DEVICE)  # same reached set -> E_r, n_r
         st = ctx.stats()
         edges += st["edges_traversed"]
@@ -229,14 +328,15 @@ def sssp_rmat22(ctx, G, rowptr, col, srcs, optr, cpu):
         st = ctx.stats()
         relaxed += st["edges_traversed"]
         t += st["elapsed_ms"]
-    res = {"gteps": edges / (t / 1e3) / 1e9, "ms_per_source": t / 16,
-           "sources": 16, "expanded_edges_per_traversed": relaxed / edges,
+    res = {"gteps": edges / (t / 1e3) / 1e9, "ms_per_source": t / len(srcs),
+           "sources": len(srcs), "expanded_edges_per_traversed": relaxed / edges,
            "roofline_frac": alg / (t / 1e3) / HBM_PEAK}
     if cpu:
         deg = np.diff(rowptr)
-        t0 = time.perf_counter()
-        d = O.sssp(m, rp64, col, w, srcs[0])
-        tc_ = time.perf_counter() - t0
+        with one_core():
+            t0 = time.perf_counter()
+            d = O.sssp(m, rp64, col, w, srcs[0])
+            tc_ = time.perf_counter() - t0
         res["cpu_baseline"] = {
             "value": int(deg[d >= 0].sum()) / tc_ / 1e9, "unit": "GTEPS", "cores": 1,
             "kind": "port", "sample": f"oracle binary-heap SSSP (SimpleReferenceSssp restated), "

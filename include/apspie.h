@@ -119,7 +119,8 @@ int apspie_ctx_sync(apspie_ctx *ctx);
 typedef struct {
     int64_t edges_traversed; /* sum of out-degrees of expanded frontier vertices */
     int64_t vertices_reached;
-    int32_t iterations;      /* BFS/SSSP levels, MIS rounds */
+    int32_t iterations;      /* BFS/SSSP levels (near-far SSSP: rounds, splits included),
+                                MIS rounds */
     int32_t host_syncs;      /* host round trips taken */
     double elapsed_ms;       /* device time of the call (hipEvent) */
     double kernel_ms;        /* summed time of the dominant kernel (profiling on) */
@@ -177,7 +178,12 @@ int apspie_bfs(apspie_ctx *ctx, const apspie_graph *g, int32_t source, int32_t *
                int mem);
 /* Single-source shortest paths under (min,+) over the graph's FP32 weights;
  * unreached = APSPIE_SSSP_INF.  Replaces sssp<V>() (sssp.cuh:27-115) +
- * spmspvSssp() (spmspvSssp.cuh:91-291), with real weights in every round. */
+ * spmspvSssp() (spmspvSssp.cuh:91-291), with real weights in every round.
+ * Weights must be finite and non-negative: a graph with a negative, NaN or
+ * infinite weight is refused with APSPIE_ERR_UNSUPPORTED (the same holds for
+ * apspie_sssp_batch).  dist[v] is the smallest FP32 left-fold sum
+ * (((w1 + w2) + w3) ...) over the paths to v, which does not depend on the
+ * order of relaxation; with integer weights and sums below 2^24 it is exact. */
 int apspie_sssp(apspie_ctx *ctx, const apspie_graph *g, int32_t source, float *dist,
                 int mem);
 /* Maximal independent set by Luby rounds under the max-priority semiring, on
@@ -261,6 +267,20 @@ int apspie_gen_weights(int32_t m, const int32_t *rowptr, const int32_t *col, uin
 int apspie_read_mtx(const char *path, int32_t *m, int32_t *n, int64_t *nnz, int32_t *row,
                     int32_t *col, float *val);
 
+/* Binary CSR cache: a graph's CSR arrays behind a 64-byte header (magic
+ * "APSPCSR1", version, flags, m, n, nnz, checksum), loaded without parsing.
+ * Replaces re-reading the text file on every run (readEdge / readMtx,
+ * util.cuh:354-430) and the COO->CSR conversion after it (util.cuh:456-471).
+ * Host-only write / read (two-call protocol for read: NULL arrays give the
+ * sizes), and device-side save / load of a resident graph.  A file whose
+ * size, magic or checksum does not match its header is refused with
+ * APSPIE_ERR_INVALID. */
+int apspie_csr_write(const char *path, const apspie_csr *host);
+int apspie_csr_read(const char *path, int32_t *m, int32_t *n, int64_t *nnz, int32_t *has_val,
+                    int32_t *rowptr, int32_t *col, float *val);
+int apspie_graph_save(apspie_ctx *ctx, const apspie_graph *g, const char *path);
+int apspie_graph_load(apspie_ctx *ctx, const char *path, apspie_graph **out);
+
 /* ------------------------------------------------------------------ serving */
 typedef enum { APSPIE_Q_BFS = 0, APSPIE_Q_SSSP = 1, APSPIE_Q_TC = 2 } apspie_query_kind;
 
@@ -276,6 +296,15 @@ int apspie_server_create(int device, const apspie_server_config *cfg, apspie_ser
 int apspie_server_destroy(apspie_server *s);
 /* Admit a host CSR into the resident pool; fails with APSPIE_ERR_BUDGET. */
 int apspie_server_add_graph(apspie_server *s, const apspie_csr *host, int32_t *graph_id);
+/* Admit a graph built on the device from COO triples (host or device memory;
+ * symmetrize as apspie_graph_from_coo), or loaded from a binary CSR cache
+ * file.  Admissions may run while queries are in flight; the pool's budget
+ * is checked against the built graph's size. */
+int apspie_server_add_graph_coo(apspie_server *s, const apspie_coo *coo, int mem, int symmetrize,
+                                int32_t *graph_id);
+int apspie_server_add_graph_file(apspie_server *s, const char *path, int32_t *graph_id);
+/* Message of the last failed admission. */
+const char *apspie_server_last_error(apspie_server *s);
 /* Enqueue a query; returns a ticket.  source ignored for TC. */
 int apspie_server_submit(apspie_server *s, int32_t graph_id, int kind, int32_t source,
                          int64_t *ticket);
@@ -291,6 +320,12 @@ int apspie_server_wait_view(apspie_server *s, int64_t ticket, const void **data,
                             double *latency_ms);
 int apspie_server_release(apspie_server *s, int64_t ticket);
 int apspie_server_drain(apspie_server *s);
+/* Work done on one graph for one query kind since the last reset: batches
+ * launched, queries answered and the workers' wall time on them (launch to
+ * results in host memory; concurrent workers add up).  reset != 0 zeroes
+ * the counters after reading. */
+int apspie_server_busy(apspie_server *s, int32_t graph_id, int kind, int64_t *batches,
+                       int64_t *queries, double *busy_ms, int reset);
 
 #ifdef __cplusplus
 }

@@ -803,3 +803,66 @@ int64_t or_dcsc(int32_t m, const int64_t *rowptr, const int32_t *col, const floa
     free(off); free(kp); free(rows); free(end); free(scan);
     return L;
 }
+
+/* ======================================================================== */
+/* Result invariants (checks, not restatements)                             */
+/* ======================================================================== */
+/* Invariants that fix a BFS depth array exactly on a graph whose CSR equals
+ * its transpose (the depth semantics of testBfs.cpp:6-72): d[src] = 0,
+ * every other entry >= -1; every edge joins two reached or two unreached
+ * vertices and spans at most one level; every reached v != src has a
+ * neighbour one level up.  By induction on the level these leave exactly
+ * one array.  Returns 0, or the number of the first failed check. */
+int32_t or_check_bfs(int32_t m, const int64_t *rowptr, const int32_t *col, const int32_t *d,
+                     int32_t src) {
+    if (src < 0 || src >= m || d[src] != 0) return 1;
+    uint8_t *par = (uint8_t *)calloc((size_t)(m ? m : 1), 1);
+    if (!par) return -1;
+    int32_t bad = 0;
+    for (int32_t v = 0; v < m && !bad; ++v) {
+        const int32_t dv = d[v];
+        if (dv < -1) { bad = 5; break; }
+        for (int64_t e = rowptr[v]; e < rowptr[v + 1]; ++e) {
+            const int32_t u = col[e], du = d[u];
+            if ((dv >= 0) != (du >= 0)) { bad = 2; break; }
+            if (dv >= 0 && (du > dv + 1 || dv > du + 1)) { bad = 3; break; }
+            if (dv >= 0 && du == dv + 1) par[u] = 1; /* v is one level above u */
+        }
+    }
+    for (int32_t v = 0; v < m && !bad; ++v)
+        if (d[v] > 0 && !par[v]) bad = 4;
+    free(par);
+    return bad;
+}
+
+/* Invariants that fix SSSP distances exactly for positive weights (the
+ * semantics of testSssp.cpp:33-115 in FP32): d[src] = 0; every reached u
+ * and edge (u, v) satisfy d[v] <= fl32(d[u] + w) (so d is at most the
+ * shortest left-fold sum, and unreached vertices have no reached
+ * in-neighbour); every reached v != src is tight on some in-edge,
+ * d[v] == fl32(d[u] + w) (following tight edges strictly lowers d, so d[v]
+ * is the sum of a real path and at least the shortest).  Unreached = inf
+ * (the caller's sentinel).  Returns 0, or the number of the first failed
+ * check. */
+int32_t or_check_sssp(int32_t m, const int64_t *rowptr, const int32_t *col, const float *w,
+                      const float *d, int32_t src, float inf) {
+    if (src < 0 || src >= m || d[src] != 0.0f) return 1;
+    uint8_t *tight = (uint8_t *)calloc((size_t)(m ? m : 1), 1);
+    if (!tight) return -1;
+    int32_t bad = 0;
+    for (int32_t u = 0; u < m && !bad; ++u) {
+        const float du = d[u];
+        if (!(du >= 0.0f)) { bad = 4; break; }
+        if (du == inf) continue;
+        for (int64_t e = rowptr[u]; e < rowptr[u + 1]; ++e) {
+            const int32_t v = col[e];
+            const float c = du + w[e];
+            if (d[v] > c) { bad = 2; break; }
+            if (d[v] == c) tight[v] = 1;
+        }
+    }
+    for (int32_t v = 0; v < m && !bad; ++v)
+        if (v != src && d[v] != inf && !tight[v]) bad = 3;
+    free(tight);
+    return bad;
+}

@@ -156,4 +156,10 @@ int64_t or_dcsc(int32_t m, const int64_t *rowptr, const int32_t *col, const floa
 #ifdef __cplusplus
 }
 #endif
+/* result invariants that fix BFS depths / SSSP distances exactly (0 = all hold) */
+int32_t or_check_bfs(int32_t m, const int64_t *rowptr, const int32_t *col, const int32_t *d,
+                     int32_t src);
+int32_t or_check_sssp(int32_t m, const int64_t *rowptr, const int32_t *col, const float *w,
+                      const float *d, int32_t src, float inf);
+
 #endif

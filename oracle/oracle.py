@@ -65,6 +65,10 @@ def _declare(L):
     L.or_bfs.restype = i32
     L.or_sssp.argtypes = [i32, vp, vp, vp, i32, vp]
     L.or_sssp.restype = i32
+    L.or_check_bfs.argtypes = [i32, vp, vp, vp, i32]
+    L.or_check_bfs.restype = i32
+    L.or_check_sssp.argtypes = [i32, vp, vp, vp, vp, i32, C.c_float]
+    L.or_check_sssp.restype = i32
     L.or_mis_greedy.argtypes = [i32, vp, vp, vp, vp]
     L.or_mis_luby.argtypes = [i32, vp, vp, vp, f32, vp]
     L.or_tc.argtypes = [i32, vp, vp, vp]
@@ -217,6 +221,22 @@ def sssp(m, rowptr, col, w, src):
     if rc:
         raise ValueError("bad source")
     return dist
+
+
+def check_bfs(m, rowptr, col, depth, src):
+    """0 if the depths satisfy the invariants that fix them exactly (on a
+    symmetric CSR), else the number of the first failed check."""
+    return int(lib().or_check_bfs(m, _p(np.ascontiguousarray(rowptr, np.int64)), _p(col),
+                                  _p(np.ascontiguousarray(depth, np.int32)), src))
+
+
+def check_sssp(m, rowptr, col, w, dist, src, inf=np.float32(1.70141e38)):
+    """0 if the distances satisfy the invariants that fix them exactly
+    (positive weights; unreached = inf), else the first failed check."""
+    return int(lib().or_check_sssp(m, _p(np.ascontiguousarray(rowptr, np.int64)), _p(col),
+                                   _p(np.ascontiguousarray(w, np.float32)),
+                                   _p(np.ascontiguousarray(dist, np.float32)), src,
+                                   float(inf)))
 
 
 def mis_greedy(m, rowptr, col, prio):

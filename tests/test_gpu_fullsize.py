@@ -1,25 +1,32 @@
-"""Parity at the workload sizes: R-MAT scale 22 (BFS, SSSP), R-MAT scale 20
-(MIS, TC) and the 2048^2 lattice (plain and batched traversals).
+"""Parity at the workload sizes: R-MAT scale 22 (BFS, SSSP: every one of the 64
+sources the benchmark times), R-MAT scale 20 (MIS, TC), the 2048^2 lattice
+(plain, and batched at the benchmark's 32 / 128 / 512 sources) and the
+8-graph served pool.
 
 Known answers come from tests/golden/golden_full.json (oracle outputs, the
-oracle being pinned at smaller sizes); the rest is checked through
-size-independent properties: every edge spans at most one BFS level and
-every reached vertex has a parent one level up; SSSP distances satisfy the
-triangle inequality on every edge and are tight on some in-edge; batched
-results equal the single-source results; per-row triangle counts add up to
-the total."""
+oracle being pinned at smaller sizes); every other output is checked by the
+oracle's exact invariant checks (check_bfs / check_sssp: invariants that
+leave exactly one depth / distance array, run 16 at a time on host threads),
+batched rows equal the single-source results, served results equal the plain
+one-query-per-launch results, and per-row triangle counts add up to the
+total."""
 import hashlib
 import json
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 import numpy as np
 import pytest
 
 import apspie_amd as A
+from apspie_amd import servebench as SB
+from apspie_amd.server import Server
+from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
 GF = json.loads((Path(__file__).parent / "golden" / "golden_full.json").read_text())
+CHECKERS = ThreadPoolExecutor(16)  # ctypes releases the GIL: checks run in parallel
 
 
 def sha(a):
@@ -50,67 +57,75 @@ def rmat22(ctx):
     w = A.gen_weights(rp, col)
     assert sha(w) == GF["rmat22"]["weights_sha"]
     G = ctx.upload(rp, col, w)
-    yield rp, col, w, G
+    yield rp, rp.astype(np.int64), col, w, G
     G.close()
 
 
-def bfs_properties(rp, col, d, src):
-    rows = np.repeat(np.arange(len(rp) - 1, dtype=np.int32), np.diff(rp))
-    du, dv = d[rows], d[col]
-    assert d[src] == 0
-    both = (du >= 0) & (dv >= 0)
-    assert np.all(np.abs(du[both] - dv[both]) <= 1)
-    assert not np.any((du >= 0) ^ (dv >= 0))  # reached set is closed
-    parent = np.zeros(len(d), bool)
-    parent[col[(du >= 0) & (dv == du + 1)]] = True
-    assert np.all(parent[(d > 0)])
+def bench_sources(rp, k=64, seed=3):
+    """The benchmark's sources: v = draw % m over std::mt19937(seed), degree > 0."""
+    draws = O.mt_stream(seed, 4096 * k)
+    m = len(rp) - 1
+    deg = np.diff(rp)
+    out, seen = [], set()
+    for d in draws.tolist():
+        v = d % m
+        if deg[v] > 0 and v not in seen:
+            seen.add(v)
+            out.append(v)
+            if len(out) == k:
+                return out
+    raise AssertionError("not enough sources")
 
 
-def sssp_properties(rp, col, w, d, src):
-    rows = np.repeat(np.arange(len(rp) - 1, dtype=np.int32), np.diff(rp))
-    inf = A.SSSP_INF
-    du, dv = d[rows], d[col]
-    reach = du < inf
-    assert d[src] == 0
-    assert np.all(dv[reach] <= du[reach] + w[reach])
-    tight = np.zeros(len(d), bool)
-    tight[col[reach & (dv == du + w)]] = True
-    assert np.all(tight[(d > 0) & (d < inf)])
-
-
-def test_rmat22_bfs(ctx, rmat22):
-    rp, col, w, G = rmat22
+@pytest.mark.timeout(600)
+def test_rmat22_bfs_every_source(ctx, rmat22):
+    """All 64 sources: the golden hash where one exists, the exact invariant
+    check for every source, and the traversed-edge count (the GTEPS
+    numerator) equal to the degree sum of the reached vertices."""
+    rp, rp64, col, w, G = rmat22
     g = GF["rmat22"]
     src = g["sources"]
-    assert len(src) == 64
+    assert src == bench_sources(rp)
+    m = len(rp) - 1
+    deg = np.diff(rp).astype(np.int64)
+    jobs = []
     for s in src:
         d = ctx.bfs(G, s)
         if str(s) in g["bfs_sha"]:
             assert sha(d) == g["bfs_sha"][str(s)], s
         st = ctx.stats()
-        assert st["edges_traversed"] > 0
-    for s in src[4:8]:
-        bfs_properties(rp, col, ctx.bfs(G, s), s)
+        assert st["edges_traversed"] == int(deg[d >= 0].sum()), s
+        assert st["vertices_reached"] == int((d >= 0).sum()), s
+        jobs.append((s, CHECKERS.submit(O.check_bfs, m, rp64, col, d, s)))
+    for s, f in jobs:
+        assert f.result() == 0, s
 
 
 def test_rmat22_bfs_batch_equals_plain(ctx, rmat22):
-    rp, col, w, G = rmat22
+    rp, rp64, col, w, G = rmat22
     src = GF["rmat22"]["sources"][:32]
     got = ctx.bfs_batch(G, src)
-    for i in (0, 1, 17, 31):
-        assert np.array_equal(got[i], ctx.bfs(G, src[i]))
+    for i in range(32):
+        assert np.array_equal(got[i], ctx.bfs(G, src[i])), i
 
 
-def test_rmat22_sssp(ctx, rmat22):
-    rp, col, w, G = rmat22
+@pytest.mark.timeout(600)
+def test_rmat22_sssp_every_source(ctx, rmat22):
+    """All 64 sources (the benchmark times the first 16): golden hashes where
+    they exist, the exact invariant check for every source."""
+    rp, rp64, col, w, G = rmat22
     g = GF["rmat22"]
-    for s_str, h in g["sssp_sha"].items():
-        d = ctx.sssp(G, int(s_str))
-        want = d.copy()
-        want[want == A.SSSP_INF] = -1  # oracle convention for unreached
-        assert sha(want) == h
-    s = g["sources"][5]
-    sssp_properties(rp, col, w, ctx.sssp(G, s), s)
+    m = len(rp) - 1
+    jobs = []
+    for s in g["sources"]:
+        d = ctx.sssp(G, s)
+        if str(s) in g["sssp_sha"]:
+            want = d.copy()
+            want[want == A.SSSP_INF] = -1  # oracle convention for unreached
+            assert sha(want) == g["sssp_sha"][str(s)], s
+        jobs.append((s, CHECKERS.submit(O.check_sssp, m, rp64, col, w, d, s)))
+    for s, f in jobs:
+        assert f.result() == 0, s
 
 
 def test_rmat20_mis_tc(ctx):
@@ -126,23 +141,85 @@ def test_rmat20_mis_tc(ctx):
     G.close()
 
 
-def test_lattice2048_plain_and_batched(ctx):
+@pytest.fixture(scope="module")
+def lattice2048(ctx):
     rp, col = build("lattice2048")
     w = A.gen_weights(rp, col)
-    g = GF["lattice2048"]
-    assert sha(w) == g["weights_sha"]
+    assert sha(w) == GF["lattice2048"]["weights_sha"]
     G = ctx.upload(rp, col, w)
+    yield rp, rp.astype(np.int64), col, w, G
+    G.close()
+
+
+def test_lattice2048_plain(ctx, lattice2048):
+    rp, rp64, col, w, G = lattice2048
+    g = GF["lattice2048"]
     for s_str, h in g["bfs_sha"].items():
         assert sha(ctx.bfs(G, int(s_str))) == h
     for s_str, h in g["sssp_sha"].items():
         d = ctx.sssp(G, int(s_str))
         d[d == A.SSSP_INF] = -1
         assert sha(d) == h
-    src = g["sources"]
-    got = ctx.bfs_batch(G, src)
-    for i, s in enumerate(src):
-        assert np.array_equal(got[i], ctx.bfs(G, s))
-    gs = ctx.sssp_batch(G, src[:2])
-    for i, s in enumerate(src[:2]):
-        assert np.array_equal(gs[i], ctx.sssp(G, s))
-    G.close()
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("kind", ["bfs", "sssp"])
+def test_lattice2048_batched_widths(ctx, lattice2048, kind):
+    """The benchmark's batches (32, 128 and 512 sources uniform over the
+    largest component; one, four and sixteen source words per vertex): every
+    row equals the plain single-source result, and every plain result passes
+    the exact invariant check."""
+    rp, rp64, col, w, G = lattice2048
+    m = len(rp) - 1
+    depth = ctx.bfs(G, int(np.argmax(np.diff(rp))))
+    comp = np.nonzero(depth >= 0)[0]
+    srcs = [int(x) for x in np.random.default_rng(4).choice(comp, 512, replace=False)]
+    plain = {}
+    jobs = []
+    for s in srcs:
+        d = ctx.bfs(G, s) if kind == "bfs" else ctx.sssp(G, s)
+        plain[s] = d
+        check = O.check_bfs if kind == "bfs" else O.check_sssp
+        args = (m, rp64, col, d, s) if kind == "bfs" else (m, rp64, col, w, d, s)
+        jobs.append((s, CHECKERS.submit(check, *args)))
+    for s, f in jobs:
+        assert f.result() == 0, (kind, s)
+    for B in (32, 128, 512):
+        got = ctx.bfs_batch(G, srcs[:B]) if kind == "bfs" else ctx.sssp_batch(G, srcs[:B])
+        for i in range(B):
+            assert np.array_equal(got[i], plain[srcs[i]]), (kind, B, i)
+        del got
+
+
+@pytest.mark.timeout(1200)
+def test_served_pool_equals_plain(ctx):
+    """The benchmark's served stream over its 8-graph pool (R-MAT 16-22,
+    lattices 512^2-4096^2): every result of the continuous-batching server
+    equals the plain one-query-per-launch result for that query."""
+    pool = SB.build_pool()
+    stream = SB.make_stream(pool, 320)
+    with Server(slots=4, max_batch=128, hbm_budget=64 << 30) as srv:
+        ids = [srv.add_graph(g[1], g[2], g[3]) for g in pool]
+        tickets = [srv.submit(ids[gi], kind, s) for gi, kind, s in stream]
+        served = [srv.wait(t, kind, len(pool[gi][1]) - 1)[0]
+                  for t, (gi, kind, s) in zip(tickets, stream)]
+    by_graph = {}
+    for i, (gi, kind, s) in enumerate(stream):
+        by_graph.setdefault(gi, []).append(i)
+    kinds = set()
+    for gi, idx in by_graph.items():
+        name, rp, col, w = pool[gi]
+        G = ctx.upload(rp, col, w)
+        for i in idx:
+            _, kind, s = stream[i]
+            kinds.add((name, kind))
+            if kind == A.Q_BFS:
+                assert np.array_equal(served[i], ctx.bfs(G, s)), (name, s)
+            elif kind == A.Q_SSSP:
+                assert np.array_equal(served[i], ctx.sssp(G, s)), (name, s)
+            else:
+                assert served[i] == ctx.tc(G, per_row=False)[0], name
+            served[i] = None
+        G.close()
+    assert {k for _, k in kinds} == {A.Q_BFS, A.Q_SSSP, A.Q_TC}
+    assert len({n for n, _ in kinds}) == 8

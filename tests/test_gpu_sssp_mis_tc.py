@@ -197,6 +197,78 @@ def test_sssp_edge_cases(ctx):
     G.close()
 
 
+@pytest.mark.parametrize("bad", [-1.0, np.nan, -np.nan, np.inf, -np.inf])
+def test_sssp_rejects_unorderable_weights(ctx, bad):
+    """Distances are ordered by their bit patterns, valid for finite
+    non-negative floats only: one negative, NaN or infinite weight makes both
+    SSSP entry points refuse the graph (status -6) instead of answering."""
+    g = T.rmat(10)
+    w = g.w.copy()
+    w[len(w) // 3] = bad
+    G = ctx.upload(g.rowptr32, g.col, w)
+    for call in (lambda: ctx.sssp(G, 1), lambda: ctx.sssp_batch(G, [1, 2, 3])):
+        with pytest.raises(A.ApspieError) as e:
+            call()
+        assert e.value.status == -6
+    assert ctx.bfs(G, 1)[1] == 0  # the graph itself stays usable
+    G.close()
+    w = g.w.copy()
+    w[::7] = 0.0  # zero weights are fine
+    G = ctx.upload(g.rowptr32, g.col, w)
+    src = T.sources(g, 1)[0]
+    assert np.array_equal(ctx.sssp(G, src), to_ref_sentinel(O.sssp(g.m, g.rowptr, g.col, w, src)))
+    G.close()
+
+
+def big_weights(g, lo, hi, seed):
+    rng = np.random.default_rng(seed)
+    return rng.integers(lo, hi + 1, len(g.col)).astype(np.float32)
+
+
+@pytest.mark.parametrize("delta", [-1, 0, 16])
+@pytest.mark.parametrize("lo,hi", [(1, 10 ** 8), (10 ** 9, 10 ** 10)])
+@pytest.mark.parametrize("key", ["rmat12", "lattice64"])
+def test_sssp_large_weights_vs_oracle(ctx, key, lo, hi, delta):
+    """Weights up to 1e10 (sums far past 2^24, so FP32 rounds; the result is
+    the smallest left-fold FP32 path sum, order-independent): the default
+    width (chosen from the weights, no cap), Bellman-Ford rounds, and a width
+    of 16, where the split threshold must jump to the pending distances."""
+    g = T.rmat(int(key[4:])) if key.startswith("rmat") else T.lattice(int(key[7:]))
+    w = big_weights(g, lo, hi, seed=lo % 101)
+    ctx.set_option("sssp_delta", delta)
+    try:
+        G = ctx.upload(g.rowptr32, g.col, w)
+        for src in T.sources(g, 3):
+            want = to_ref_sentinel(O.sssp(g.m, g.rowptr, g.col, w, src))
+            assert np.array_equal(ctx.sssp(G, src), want), (key, lo, delta, src)
+            assert ctx.stats()["iterations"] <= 4 * g.m + 8
+        G.close()
+    finally:
+        ctx.set_option("sssp_delta", -1)
+
+
+@pytest.mark.parametrize("bdelta", [-1, 16])
+@pytest.mark.parametrize("nsrc", [32, 128])
+@pytest.mark.parametrize("lo,hi", [(1, 10 ** 8), (10 ** 9, 10 ** 10)])
+def test_sssp_batch_large_weights_vs_oracle(ctx, lo, hi, nsrc, bdelta):
+    """The batched near-far rounds with the same weights: default width and
+    a width of 16 (threshold jumps), 32 and 128 sources (one and four source
+    words per vertex)."""
+    g = T.lattice(48)
+    w = big_weights(g, lo, hi, seed=7)
+    ctx.set_option("batch_sssp_delta", bdelta)
+    try:
+        G = ctx.upload(g.rowptr32, g.col, w)
+        src = T.sources(g, nsrc, seed=11)
+        got = ctx.sssp_batch(G, src)
+        for i in range(0, nsrc, 7):
+            want = to_ref_sentinel(O.sssp(g.m, g.rowptr, g.col, w, src[i]))
+            assert np.array_equal(got[i], want), (lo, nsrc, bdelta, i)
+        G.close()
+    finally:
+        ctx.set_option("batch_sssp_delta", -1)
+
+
 # ----------------------------------------------------------------------- MIS
 @pytest.mark.parametrize("name", T.FIXTURES)
 def test_mis_fixtures(ctx, name):

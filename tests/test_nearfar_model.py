@@ -1,0 +1,140 @@
+"""The near-far split schedule of the SSSP rounds, restated on the CPU.
+
+The device rounds (sssp.hip k_sssp_nf_round, batch.hip k_batch_sssp_nf) keep a
+threshold T: a relaxation that leaves a vertex at or above T only marks it
+pending; a round that finds the frontier empty is a split: T rises to
+max(T + delta, next float above T) and -- when the previous round was a split
+that expanded nothing -- to just above the smallest distance that split left
+pending; the sweep then moves the pending vertices below the new T into the
+frontier, drops those already below the old T and keeps the rest.
+
+This model runs that schedule sequentially (FP32 sums, one round at a time)
+and checks what the rule is for: the fixed point equals the oracle's
+Dijkstra whatever the width, and the number of rounds stays bounded when the
+weights dwarf the width (weights up to 1e10 against a width of 16), where
+rising by T + delta alone would take about max-distance / delta splits."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import graphs as T
+
+F32_INF = np.float32(np.inf)
+
+
+def near_far(m, rowptr, col, w, src, delta, jump=True, max_rounds=200_000):
+    """(distances, rounds, splits); rounds = -1 when max_rounds ran out."""
+    w = np.asarray(w, np.float32)
+    dist = np.full(m, F32_INF, np.float32)
+    dist[src] = 0
+    pend = np.zeros(m, bool)
+    deg = np.diff(rowptr)
+    T = np.float32(delta)
+    frontier = [src] if deg[src] > 0 else []
+    split_last, left_prev, min_prev = False, 0, None
+    rounds = splits = 0
+    while rounds < max_rounds:
+        rounds += 1
+        if frontier:
+            nxt = []
+            seen = set()
+            for u in frontier:
+                du = dist[u]
+                for e in range(rowptr[u], rowptr[u + 1]):
+                    v = col[e]
+                    nd = np.float32(du + w[e])
+                    if nd < dist[v]:
+                        dist[v] = nd
+                        if nd >= T:
+                            pend[v] = True
+                        elif v not in seen:
+                            seen.add(v)
+                            if deg[v] > 0:
+                                nxt.append(v)
+            frontier = nxt
+            split_last = False
+            continue
+        if split_last and left_prev == 0:
+            return dist, rounds, splits
+        splits += 1
+        tn = np.float32(T + np.float32(delta))
+        if tn <= T:
+            tn = np.nextafter(T, F32_INF)
+        if jump and split_last and min_prev is not None:
+            tn = max(tn, np.nextafter(min_prev, F32_INF))
+        left, mn = 0, None
+        for v in np.nonzero(pend)[0]:
+            d = dist[v]
+            if d < T:
+                pend[v] = False
+            elif d < tn:
+                pend[v] = False
+                if deg[v] > 0:
+                    frontier.append(v)
+            else:
+                left += 1
+                mn = d if mn is None else min(mn, d)
+        left_prev, min_prev = left, mn
+        T = tn
+        split_last = True
+    return dist, -1, splits
+
+
+def oracle(m, rowptr, col, w, src):
+    d = O.sssp(m, rowptr, col, w, src)
+    return np.where(d == -1, F32_INF, d).astype(np.float32)
+
+
+def weights(g, lo, hi, seed):
+    rng = np.random.default_rng(seed)
+    return rng.integers(lo, hi + 1, len(g.col)).astype(np.float32)
+
+
+@pytest.mark.parametrize("lo,hi", [(1, 255), (1, 10 ** 8), (10 ** 9, 10 ** 10)])
+@pytest.mark.parametrize("delta", [16, 512, 2.0 ** 32])
+def test_fixed_point_equals_oracle(lo, hi, delta):
+    g = T.rmat(8)
+    w = weights(g, lo, hi, seed=lo % 97)
+    for src in T.sources(g, 3):
+        d, rounds, _ = near_far(g.m, g.rowptr, g.col, w, src, delta)
+        assert rounds > 0
+        assert np.array_equal(d, oracle(g.m, g.rowptr, g.col, w, src)), (lo, hi, delta, src)
+
+
+def test_rounds_bounded_with_large_weights():
+    """Weights in [1e9, 1e10] against a width of 16: with the jump every
+    second consecutive split moves a vertex, so the rounds stay within a
+    small multiple of the vertex count; rising by the width alone (T + 16,
+    which stops moving once T passes 16 * 2^24) never reaches the distances."""
+    g = T.lattice(8)
+    w = weights(g, 10 ** 9, 10 ** 10, seed=3)
+    src = T.sources(g, 1)[0]
+    d, rounds, splits = near_far(g.m, g.rowptr, g.col, w, src, 16)
+    assert np.array_equal(d, oracle(g.m, g.rowptr, g.col, w, src))
+    assert 0 < rounds <= 4 * g.m + 8
+    _, rounds_nojump, _ = near_far(g.m, g.rowptr, g.col, w, src, 16, jump=False,
+                                   max_rounds=20_000)
+    assert rounds_nojump == -1  # did not finish
+
+
+def test_threshold_steps_past_fp32_rounding():
+    """Once T is large, T + delta == T in FP32; the step is then one ulp."""
+    T0 = np.float32(1e12)
+    assert np.float32(T0 + np.float32(16)) == T0
+    g = T.path_graph(40)
+    w = np.full(len(g.col), 1e12, np.float32)
+    d, rounds, _ = near_far(g.m, g.rowptr, g.col, w, 0, 16)
+    assert np.array_equal(d, oracle(g.m, g.rowptr, g.col, w, 0))
+    assert rounds > 0
+
+
+def test_default_width_rounds_unchanged_on_small_weights():
+    """With weights 1..255 and the widths the library picks (64, 512), the
+    jump never fires before the schedule ends: same rounds with and without."""
+    g = T.rmat(10)
+    for delta in (64, 512):
+        for src in T.sources(g, 2):
+            a = near_far(g.m, g.rowptr, g.col, g.w, src, delta)
+            b = near_far(g.m, g.rowptr, g.col, g.w, src, delta, jump=False)
+            assert np.array_equal(a[0], b[0])
+            assert a[1] <= b[1]

@@ -234,6 +234,79 @@ def test_sssp_invariants():
     assert np.all(tight[(d > 0)])
 
 
+def test_result_checkers_accept_exact_and_reject_planted_errors():
+    """check_bfs / check_sssp (the full-size parity checks): the oracle's
+    answers pass; each planted mistake -- a depth or distance one off in
+    either direction, a reached vertex reported unreached, an unreached one
+    reported reached -- fails."""
+    rng = np.random.default_rng(8)
+    for m, rp, c in (O.rmat_graph(10), O.lattice_graph(40)):
+        w = O.assign_weights(m, rp, c)
+        deg = np.diff(rp)
+        for src in (int(np.argmax(deg)), int(np.nonzero(deg)[0][-1])):
+            d, _ = O.bfs(m, rp, c, src)
+            assert O.check_bfs(m, rp, c, d, src) == 0
+            reached = np.nonzero(d > 0)[0]
+            for delta in (1, -1):
+                bad = d.copy()
+                bad[rng.choice(reached)] += delta
+                assert O.check_bfs(m, rp, c, bad, src) != 0
+            bad = d.copy()
+            bad[rng.choice(reached)] = -1
+            assert O.check_bfs(m, rp, c, bad, src) != 0
+            s = O.sssp(m, rp, c, w, src)
+            s = np.where(s == -1, np.float32(1.70141e38), s).astype(np.float32)
+            assert O.check_sssp(m, rp, c, w, s, src) == 0
+            for delta in (1.0, -1.0):
+                bad = s.copy()
+                bad[rng.choice(reached)] += delta
+                assert O.check_sssp(m, rp, c, w, bad, src) != 0
+            bad = s.copy()
+            bad[rng.choice(reached)] = np.float32(1.70141e38)
+            assert O.check_sssp(m, rp, c, w, bad, src) != 0
+    # an isolated vertex reported reached
+    m = 6
+    rp, c = O.edges_to_csr(m, np.array([0, 1], np.int32), np.array([1, 2], np.int32))
+    d, _ = O.bfs(m, rp, c, 0)
+    assert O.check_bfs(m, rp, c, d, 0) == 0
+    bad = d.copy()
+    bad[5] = 3
+    assert O.check_bfs(m, rp, c, bad, 0) != 0
+
+
+@pytest.mark.parametrize("lo,hi", [(1, 10 ** 8), (10 ** 9, 10 ** 10)])
+def test_sssp_fp32_sums_with_large_weights(lo, hi):
+    """Weights whose sums pass 2^24 round in FP32.  The oracle's distances
+    are then the smallest left-fold FP32 path sums: they equal the fixed
+    point of d[v] = min(d[v], fl32(d[u] + w)) iterated over all edges from
+    d[src] = 0 (a vectorised Bellman-Ford in numpy float32, independent of
+    the heap order), and they differ from the float64 shortest paths."""
+    rng = np.random.default_rng(lo % 1000)
+    m, rp, c = O.rmat_graph(8)
+    w = rng.integers(lo, hi + 1, len(c)).astype(np.float32)
+    rows = np.repeat(np.arange(m), np.diff(rp))
+    differs = 0
+    for src in (0, 17, 100):
+        d = O.sssp(m, rp, c, w, src)
+        bf = np.full(m, np.inf, np.float32)
+        bf[src] = 0
+        while True:
+            cand = (bf[rows] + w).astype(np.float32)
+            nb = bf.copy()
+            np.minimum.at(nb, c, cand)
+            if np.array_equal(nb, bf):
+                break
+            bf = nb
+        want = np.where(np.isinf(bf), np.float32(-1), bf)
+        assert np.array_equal(d, want), src
+        import scipy.sparse as sp
+        import scipy.sparse.csgraph as cg
+        A64 = sp.csr_matrix((w.astype(np.float64), c, rp), shape=(m, m))
+        d64 = cg.dijkstra(A64, indices=src)
+        differs += int(np.sum(np.isfinite(d64) & (d64 != d.astype(np.float64))))
+    assert differs > 0  # the test does exercise rounding
+
+
 # ------------------------------------------------------------------------ MIS
 def test_mis_independent_maximal_and_luby_fixed_point():
     for seed in range(5):
