@@ -326,10 +326,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
 // T shared by all sources): a relaxation that lowers dist[v][s] below T sets
 // bit s of next[v]; one that leaves it at or above T sets bit s of pend[v].
 // A round that finds the union frontier empty is a split round: T grows by
-// delta and a sweep (a group of W lanes per vertex, lane w owning word w)
+// the step (delta, doubled by each split after a split that moved nothing,
+// see nf_next_step) and a sweep (a group of W lanes per vertex, lane w owning word w)
 // moves the pending bits now below T into next, drops the ones that fell
 // below the old T since (already expanded) and keeps the rest.
-// ctl: [12 + (r & 1)] T of round r, [14 + (r & 1)] round r - 1 was a split,
+// ctl: [10 + (r & 1)] the step of round r, [12 + (r & 1)] T of round r,
+// [14 + (r & 1)] round r - 1 was a split,
 // pending bits left by the split of round r in the spread slots (r % 3).
 template <int W>
 struct SsspBatchNearOp {
@@ -427,7 +429,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_sssp_bf(
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_batch_sssp_nf(
     const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col, int32_t nnz, int32_t m,
-    SsspBatchNearOp<W> op, float delta, FrontierBufs cur, FrontierBufs nxt, uint64_t *ctl,
+    SsspBatchNearOp<W> op, FrontierBufs cur, FrontierBufs nxt, uint64_t *ctl,
     int32_t r) {
     constexpr int B = 32 * W;
     __shared__ BatchLds lds;
@@ -440,12 +442,15 @@ __global__ __launch_bounds__(kBlock) void k_batch_sssp_nf(
     const uint64_t left_before = nf_left(ctl, r + 2);
     const bool split = F == 0;
     const bool done = split && split_last && left_before == 0;
+    const uint64_t min_before = nf_min_left(ctl, r + 2);
+    const bool jump = split && !done && split_last && min_before != ~0ull;
+    const float step = nf_next_step(dev::bitsf((int32_t)ctl[10 + (r & 1)]), jump);
     const int32_t tnext =
-        split && !done ? nf_next_threshold(tbits, delta, split_last, nf_min_left(ctl, r + 2))
-                       : tbits;
+        split && !done ? nf_next_threshold(tbits, step, split_last, min_before) : tbits;
     __syncthreads();  // every thread has read the control words
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl[(r + 2) % 3] = 0;
+        ctl[10 + ((r + 1) & 1)] = (uint64_t)(uint32_t)dev::fbits(step);
         ctl[12 + ((r + 1) & 1)] = (uint64_t)(uint32_t)tnext;
         ctl[14 + ((r + 1) & 1)] = split ? 1 : 0;
         nf_reset(ctl, r + 1);
@@ -749,7 +754,10 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
     fill_i32(c, (int32_t *)fs.ctl, 0, 2 * kCtlWords);
     int32_t *lvl = (int32_t *)(fs.ctl + 8);  // level / round counter
     fill_i32(c, lvl, 1, 1);
-    if (near_far) fill_i32(c, (int32_t *)(fs.ctl + 12), __builtin_bit_cast(int32_t, delta), 1);
+    if (near_far) {  // T of round 0 and its split step
+        fill_i32(c, (int32_t *)(fs.ctl + 12), __builtin_bit_cast(int32_t, delta), 1);
+        fill_i32(c, (int32_t *)(fs.ctl + 10), __builtin_bit_cast(int32_t, delta), 1);
+    }
     k_batch_seed<W><<<1, 256, 0, c->stream>>>(srcs, nsrc, vis, fa, depth, m, dist);
     k_build<kBlock, 1, kTile, true><<<1, kBlock, 0, c->stream>>>(
         SeedOp{}, g->rowptr, ulist, (int32_t)uniq.size(), nullptr, fs.f[0], fs.ctl + 0);
@@ -762,8 +770,8 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
         if (near_far)
             k_batch_sssp_nf<W><<<grid, kBlock, 0, c->stream>>>(
                 g->rowptr, g->col, (int32_t)g->nnz, m,
-                SsspBatchNearOp<W>{front, next, pend, stamp, dist, g->val, lvl, 0}, delta,
-                fs.f[cur], fs.f[cur ^ 1], fs.ctl, l % 6);
+                SsspBatchNearOp<W>{front, next, pend, stamp, dist, g->val, lvl, 0}, fs.f[cur],
+                fs.f[cur ^ 1], fs.ctl, l % 6);
         else if (SSSP)
             k_batch_sssp_bf<W><<<grid, kBlock, 0, c->stream>>>(
                 g->rowptr, g->col, (int32_t)g->nnz,

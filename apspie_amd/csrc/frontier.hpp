@@ -285,13 +285,20 @@ __device__ __forceinline__ void nf_report(uint64_t *ctl, int r, uint64_t left, u
                                __HIP_MEMORY_SCOPE_AGENT);
     }
 }
+// The step of a split: the width delta at first, doubled by every split that
+// follows a split that expanded nothing (the bucket [T, T + step) was empty:
+// the distances are sparse against the step), capped at 2^100.  ctl[10 +
+// (r & 1)] holds the step of round r (float bits), like T.
+__device__ __forceinline__ float nf_next_step(float step, bool jump) {
+    return jump ? fminf(step * 2.0f, 0x1p100f) : step;
+}
 // The threshold a split round raises T (float bits, non-negative) to: T +
-// delta, at least the next float above T, and -- when the previous round was
+// step, at least the next float above T, and -- when the previous round was
 // a split that expanded nothing -- above the smallest distance that split
 // left pending, so that this split moves it.
-__device__ __forceinline__ int32_t nf_next_threshold(int32_t tbits, float delta, bool jump,
+__device__ __forceinline__ int32_t nf_next_threshold(int32_t tbits, float step, bool jump,
                                                      uint64_t min_left) {
-    int32_t t = dev::fbits(dev::bitsf(tbits) + delta);
+    int32_t t = dev::fbits(dev::bitsf(tbits) + step);
     if (t <= tbits) t = tbits + 1;
     if (jump && min_left != ~0ull && (int64_t)min_left + 1 > (int64_t)t)
         t = (int32_t)min_left + 1;

@@ -376,18 +376,20 @@ void sssp_do(Ctx *c, const Graph *g0, int32_t source, float *dist_out, int32_t p
 //
 // ctl words (every word a round reads was written by the previous round, and
 // no word a round reads is written in that round): [0..2] frontier counter
-// ring, [4..7] stats, [12 + (r & 1)] T of round r (float bits), [14 + (r & 1)]
+// ring, [4..7] stats, [10 + (r & 1)] the split step of round r (float bits),
+// [12 + (r & 1)] T of round r (float bits), [14 + (r & 1)]
 // 1 if round r - 1 was a split, and, in the spread slots, the pending-vertex
 // count left by the split of round r ([16 + 16 i + r % 3]) and the smallest
 // distance among them ([16 + 16 i + 3 + r % 3], float bits, ~0 if none).
 //
-// The threshold of a split is T + delta, at least the next float above T (T +
-// delta rounds to T once T exceeds delta * 2^24).  A split that follows a
-// split that expanded nothing jumps further, to just above the smallest
-// pending distance the earlier sweep left: with weights much larger than
-// delta, T would otherwise climb in maxdist / delta empty splits.  Any
-// threshold above T gives the same fixed point; the jump only bounds the
-// rounds (every second consecutive split moves a vertex).
+// The threshold of a split is T + step, at least the next float above T (T +
+// step rounds to T once T exceeds step * 2^24); the step starts at delta.  A
+// split that follows a split that expanded nothing doubles the step and
+// jumps at least to just above the smallest pending distance the earlier
+// sweep left: with weights much larger than delta, T would otherwise climb in
+// maxdist / delta empty splits.  Any threshold above T gives the same fixed
+// point; the jump and the doubling only bound the rounds (every second
+// consecutive split moves a vertex, and sparse distances widen the step).
 constexpr int kNfBlock = 1024;
 constexpr int kNfPer = 8;  // vertices per thread per sweep pass
 
@@ -415,7 +417,7 @@ template <int TILE>
 __global__ __launch_bounds__(kNfBlock) __attribute__((amdgpu_waves_per_eu(8))) void
 k_sssp_nf_round(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
                 const float *w, int32_t nnz, int32_t m, int32_t *dist, uint32_t *fb_next,
-                uint32_t *fb_clear, uint8_t *pend, float delta, FrontierBufs cur,
+                uint32_t *fb_clear, uint8_t *pend, FrontierBufs cur,
                 FrontierBufs nxt, uint64_t *ctl, int32_t r) {
     __shared__ ExpandLds<kNfBlock, TILE, true> lds;
     __shared__ uint64_t s_part[kNfBlock / 64];
@@ -427,12 +429,15 @@ k_sssp_nf_round(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ 
     const uint64_t left_before = nf_left(ctl, r + 2);  // left by the previous round's split
     const bool split = F == 0;
     const bool done = split && split_last && left_before == 0;
+    const uint64_t min_before = nf_min_left(ctl, r + 2);
+    const bool jump = split && !done && split_last && min_before != ~0ull;
+    const float step = nf_next_step(dev::bitsf((int32_t)ctl[10 + (r & 1)]), jump);
     const int32_t tnext =
-        split && !done ? nf_next_threshold(tbits, delta, split_last, nf_min_left(ctl, r + 2))
-                       : tbits;
+        split && !done ? nf_next_threshold(tbits, step, split_last, min_before) : tbits;
     __syncthreads();  // every thread has read the control words
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl[(r + 2) % 3] = 0;
+        ctl[10 + ((r + 1) & 1)] = (uint64_t)(uint32_t)dev::fbits(step);
         ctl[12 + ((r + 1) & 1)] = (uint64_t)(uint32_t)tnext;
         ctl[14 + ((r + 1) & 1)] = split ? 1 : 0;
         nf_reset(ctl, r + 1);
@@ -522,6 +527,7 @@ void sssp_near_far(Ctx *c, const Graph *g, int32_t source, float *dist_out, floa
     fill_i32(c, (int32_t *)fs.ctl, 0, 2 * kCtlWords);
     k_sssp_seed<TILE><<<1, 256, 0, c->stream>>>(g->rowptr, source, dist, fs.f[0], fs.ctl);
     fill_i32(c, (int32_t *)(fs.ctl + 12), __builtin_bit_cast(int32_t, delta), 1);  // T of round 0
+    fill_i32(c, (int32_t *)(fs.ctl + 10), __builtin_bit_cast(int32_t, delta), 1);  // its step
     APSPIE_LAUNCH_CHECK();
     const int grid = fill_grid(
         c, kNfBlock, (int)std::max<int64_t>(1, opt(c, "bfs_blocks_per_cu", 8) * 256 / kNfBlock));
@@ -537,8 +543,8 @@ void sssp_near_far(Ctx *c, const Graph *g, int32_t source, float *dist_out, floa
         for (int k = 0; k < chunk; ++k, ++r) {
             k_sssp_nf_round<TILE><<<grid, kNfBlock, 0, c->stream>>>(
                 g->rowptr, g->col, g->val, (int32_t)g->nnz, g->m, dist,
-                fb + ((r + 1) % 3) * words, fb + ((r + 2) % 3) * words, pend, delta,
-                fs.f[r & 1], fs.f[(r + 1) & 1], fs.ctl, r);
+                fb + ((r + 1) % 3) * words, fb + ((r + 2) % 3) * words, pend, fs.f[r & 1],
+                fs.f[(r + 1) & 1], fs.ctl, r);
             APSPIE_LAUNCH_CHECK();
         }
         kt.end(chunk);

@@ -3,16 +3,21 @@
 The device rounds (sssp.hip k_sssp_nf_round, batch.hip k_batch_sssp_nf) keep a
 threshold T: a relaxation that leaves a vertex at or above T only marks it
 pending; a round that finds the frontier empty is a split: T rises to
-max(T + delta, next float above T) and -- when the previous round was a split
-that expanded nothing -- to just above the smallest distance that split left
+max(T + step, next float above T), the step starting at the width delta;
+when the previous round was a split that expanded nothing, the step doubles
+and T rises at least to just above the smallest distance that split left
 pending; the sweep then moves the pending vertices below the new T into the
-frontier, drops those already below the old T and keeps the rest.
+frontier, drops those already below the old T and keeps the rest.  The batch
+(batch.hip) shares one T among all its sources: a (vertex, source) pair is
+pending or in the frontier.
 
 This model runs that schedule sequentially (FP32 sums, one round at a time)
 and checks what the rule is for: the fixed point equals the oracle's
 Dijkstra whatever the width, and the number of rounds stays bounded when the
 weights dwarf the width (weights up to 1e10 against a width of 16), where
-rising by T + delta alone would take about max-distance / delta splits."""
+rising by T + delta alone would take about max-distance / delta splits, and
+where a batch with a fixed step would settle about one (vertex, source) pair
+per three rounds."""
 import numpy as np
 import pytest
 
@@ -22,14 +27,26 @@ from tests import graphs as T
 F32_INF = np.float32(np.inf)
 
 
-def near_far(m, rowptr, col, w, src, delta, jump=True, max_rounds=200_000):
+def next_threshold(T, step, jump, min_prev, grow=True):
+    """(new T, new step) of a split: sssp.hip / batch.hip nf_next_threshold."""
+    if jump and grow:
+        step = np.float32(min(np.float32(step) * 2, np.float32(2.0 ** 100)))
+    tn = np.float32(T + np.float32(step))
+    if tn <= T:
+        tn = np.nextafter(T, F32_INF)
+    if jump:
+        tn = max(tn, np.nextafter(min_prev, F32_INF))
+    return tn, step
+
+
+def near_far(m, rowptr, col, w, src, delta, jump=True, grow=True, max_rounds=200_000):
     """(distances, rounds, splits); rounds = -1 when max_rounds ran out."""
     w = np.asarray(w, np.float32)
     dist = np.full(m, F32_INF, np.float32)
     dist[src] = 0
     pend = np.zeros(m, bool)
     deg = np.diff(rowptr)
-    T = np.float32(delta)
+    T = step = np.float32(delta)
     frontier = [src] if deg[src] > 0 else []
     split_last, left_prev, min_prev = False, 0, None
     rounds = splits = 0
@@ -57,11 +74,8 @@ def near_far(m, rowptr, col, w, src, delta, jump=True, max_rounds=200_000):
         if split_last and left_prev == 0:
             return dist, rounds, splits
         splits += 1
-        tn = np.float32(T + np.float32(delta))
-        if tn <= T:
-            tn = np.nextafter(T, F32_INF)
-        if jump and split_last and min_prev is not None:
-            tn = max(tn, np.nextafter(min_prev, F32_INF))
+        tn, step = next_threshold(T, step, jump and split_last and min_prev is not None,
+                                  min_prev, grow)
         left, mn = 0, None
         for v in np.nonzero(pend)[0]:
             d = dist[v]
@@ -78,6 +92,60 @@ def near_far(m, rowptr, col, w, src, delta, jump=True, max_rounds=200_000):
         T = tn
         split_last = True
     return dist, -1, splits
+
+
+def near_far_batch(m, rowptr, col, w, srcs, delta, grow=True, max_rounds=200_000):
+    """The batched schedule: one T for every source; (distances [m][B], rounds)."""
+    w = np.asarray(w, np.float32)
+    B = len(srcs)
+    dist = np.full((m, B), F32_INF, np.float32)
+    pend = np.zeros((m, B), bool)
+    deg = np.diff(rowptr)
+    front = {}
+    for b, s in enumerate(srcs):
+        dist[s, b] = 0
+        if deg[s] > 0:
+            front.setdefault(s, set()).add(b)
+    T = step = np.float32(delta)
+    split_last, left_prev, min_prev = False, 0, None
+    rounds = 0
+    while rounds < max_rounds:
+        rounds += 1
+        if front:
+            nxt = {}
+            for u, bs in front.items():
+                for e in range(rowptr[u], rowptr[u + 1]):
+                    v = col[e]
+                    for b in bs:
+                        nd = np.float32(dist[u, b] + w[e])
+                        if nd < dist[v, b]:
+                            dist[v, b] = nd
+                            if nd >= T:
+                                pend[v, b] = True
+                            elif deg[v] > 0:
+                                nxt.setdefault(v, set()).add(b)
+            front = nxt
+            split_last = False
+            continue
+        if split_last and left_prev == 0:
+            return dist, rounds
+        tn, step = next_threshold(T, step, split_last and min_prev is not None, min_prev, grow)
+        left, mn = 0, None
+        for v, b in zip(*np.nonzero(pend)):
+            d = dist[v, b]
+            if d < T:
+                pend[v, b] = False
+            elif d < tn:
+                pend[v, b] = False
+                if deg[v] > 0:
+                    front.setdefault(v, set()).add(b)
+            else:
+                left += 1
+                mn = d if mn is None else min(mn, d)
+        left_prev, min_prev = left, mn
+        T = tn
+        split_last = True
+    return dist, -1
 
 
 def oracle(m, rowptr, col, w, src):
@@ -138,3 +206,21 @@ def test_default_width_rounds_unchanged_on_small_weights():
             b = near_far(g.m, g.rowptr, g.col, g.w, src, delta, jump=False)
             assert np.array_equal(a[0], b[0])
             assert a[1] <= b[1]
+
+
+@pytest.mark.parametrize("lo,hi", [(1, 255), (1, 10 ** 8), (10 ** 9, 10 ** 10)])
+def test_batch_rounds_bounded_with_large_weights(lo, hi):
+    """32 sources sharing one threshold at a width of 16: every column equals
+    the oracle, and with the doubling step the rounds stay within the
+    device's guard (8 m + 4096); a fixed step settles about one pair per
+    three rounds (m * 32 pairs) and runs past it."""
+    g = T.lattice(8)
+    w = weights(g, lo, hi, seed=3)
+    srcs = T.sources(g, 32, seed=11)
+    d, rounds = near_far_batch(g.m, g.rowptr, g.col, w, srcs, 16)
+    for i, s in enumerate(srcs):
+        assert np.array_equal(d[:, i], oracle(g.m, g.rowptr, g.col, w, s)), (lo, i)
+    assert 0 < rounds <= 8 * g.m + 4096
+    if hi > 255:
+        _, fixed = near_far_batch(g.m, g.rowptr, g.col, w, srcs, 16, grow=False)
+        assert fixed == -1 or fixed > 8 * g.m + 4096
