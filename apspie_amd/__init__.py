@@ -106,6 +106,7 @@ def _declare(L):
         "apspie_graph_download": ([vp, vp, vp, vp, vp], C.c_int),
         "apspie_graph_transpose": ([vp, vp, pp], C.c_int),
         "apspie_bfs": ([vp, vp, i32, vp, C.c_int], C.c_int),
+        "apspie_bfs_noalloc": ([vp, vp, i32, vp, C.c_int], C.c_int),
         "apspie_sssp": ([vp, vp, i32, vp, C.c_int], C.c_int),
         "apspie_mis": ([vp, vp, vp, C.c_int, vp, C.c_int], C.c_int),
         "apspie_tc": ([vp, vp, C.POINTER(C.c_uint64), vp, C.c_int], C.c_int),

@@ -358,18 +358,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     }
 }
 
+// The device memory of one BFS call: the frontier pair and its control
+// block, the visited bitmap and, when pulling, the three frontier bitmaps and
+// the isolated-vertex bits.  bfs_run takes it from the context's scratch pool;
+// bfs_run_in carves it from a block the caller provides.
+struct BfsBufs {
+    FrontierSet fs;
+    uint32_t *vis = nullptr;
+    uint32_t *fb = nullptr;         // null: every level pushes
+    const uint32_t *iso = nullptr;  // with fb
+    bool kernel_timer = true;       // per-chunk HIP events (created per call)
+};
+
 template <int TILE>
-void bfs_levels(Ctx *c, const Graph *g, int32_t source, int32_t *depth) {
-    const int direction = (int)opt(c, "bfs_direction", 1);
-    // pulling reads a row as the in-edges of its vertex: only valid if A == A^T
-    const bool do_bfs = direction != 0 && graph_is_symmetric(c, const_cast<Graph *>(g));
-    FrontierSet fs = frontier_alloc(c, "bfs", g->m, g->nnz, TILE, false);
+void bfs_levels(Ctx *c, const Graph *g, int32_t source, int32_t *depth, const BfsBufs &b) {
+    const int direction = b.fb ? (int)opt(c, "bfs_direction", 1) : 0;
+    const bool do_bfs = b.fb != nullptr;
+    const FrontierSet &fs = b.fs;
     const int64_t words = g->m / 32 + 1;
-    uint32_t *vis = (uint32_t *)scratch(c, "bfs_bitmap", sizeof(uint32_t) * words);
-    uint32_t *fb = do_bfs ? (uint32_t *)scratch(c, "bfs_fb", sizeof(uint32_t) * 3 * words) : nullptr;
-    // symmetric: a vertex without neighbours is never reached, so it starts
-    // out visited; fb[2 * words ..] is cleared by level 0 itself
-    const uint32_t *iso = do_bfs ? graph_iso_bits(c, const_cast<Graph *>(g)) : nullptr;
+    uint32_t *vis = b.vis, *fb = b.fb;
+    const uint32_t *iso = b.iso;
     k_bfs_init<TILE><<<(int)std::min<int64_t>(ceil_div(g->m, 4 * 256), 4 * c->cus), 256, 0,
                        c->stream>>>(g->rowptr, iso, g->m, source, depth, vis, fb,
                                     fb ? fb + words : nullptr, fs.f[0], fs.ctl);
@@ -381,7 +389,7 @@ void bfs_levels(Ctx *c, const Graph *g, int32_t source, int32_t *depth) {
     const DoParams dp{direction, (int32_t)opt(c, "bfs_alpha", 14), (int32_t)opt(c, "bfs_beta", 24),
                       g->nnz, g->m};
     int chunk = (int)std::max<int64_t>(1, opt(c, "bfs_levels_per_sync", 8));
-    KernelTimer kt(c);
+    KernelTimer kt(c, b.kernel_timer);
     uint64_t *h = (uint64_t *)c->pinned;  // copy of the control block
     int level = 0;
     for (;;) {
@@ -425,13 +433,104 @@ void bfs_levels(Ctx *c, const Graph *g, int32_t source, int32_t *depth) {
     kt.collect();  // every launch, the empty trailing ones included
 }
 
+template <int TILE>
+void bfs_levels_pool(Ctx *c, const Graph *g, int32_t source, int32_t *depth) {
+    // pulling reads a row as the in-edges of its vertex: only valid if A == A^T
+    const bool pull = opt(c, "bfs_direction", 1) != 0 &&
+                      graph_is_symmetric(c, const_cast<Graph *>(g));
+    const int64_t words = g->m / 32 + 1;
+    BfsBufs b;
+    b.fs = frontier_alloc(c, "bfs", g->m, g->nnz, TILE, false);
+    b.vis = (uint32_t *)scratch(c, "bfs_bitmap", sizeof(uint32_t) * words);
+    if (pull) {
+        b.fb = (uint32_t *)scratch(c, "bfs_fb", sizeof(uint32_t) * 3 * words);
+        // symmetric: a vertex without neighbours is never reached, so it
+        // starts out visited; fb[2 * words ..] is cleared by level 0 itself
+        b.iso = graph_iso_bits(c, const_cast<Graph *>(g));
+    }
+    bfs_levels<TILE>(c, g, source, depth, b);
+}
+
+// Pieces of a caller-provided block, each 256-byte aligned.
+struct Carver {
+    char *base;
+    int64_t used = 0;
+    void *take(int64_t bytes) {
+        void *p = base ? base + used : nullptr;
+        used += (bytes + 255) / 256 * 256;
+        return p;
+    }
+};
+
+// Lays the buffers of one call out in `block` (null: only counts the bytes).
+// The tile partition is sized for the smallest tile (1024), so the count
+// holds for every bfs_tile.
+template <int TILE>
+int64_t bfs_carve(const Graph *g, bool pull, bool iso_cached, char *block, BfsBufs *b) {
+    Carver cv{block};
+    const int64_t ent = std::max<int64_t>(g->m, 1) + 1;
+    const int64_t tiles = g->nnz / 1024 + 2;
+    const int64_t words = g->m / 32 + 1;
+    FrontierSet fs;
+    fs.ctl = (uint64_t *)cv.take(sizeof(uint64_t) * kCtlWords);
+    for (int i = 0; i < 2; ++i) {
+        fs.f[i].v = nullptr;
+        fs.f[i].start = (int32_t *)cv.take(sizeof(int32_t) * ent);
+        fs.f[i].off = (int32_t *)cv.take(sizeof(int32_t) * ent);
+        fs.f[i].tfirst = (int32_t *)cv.take(sizeof(int32_t) * tiles);
+    }
+    fs.cap_entries = ent;
+    fs.cap_tiles = tiles;
+    uint32_t *vis = (uint32_t *)cv.take(sizeof(uint32_t) * words);
+    uint32_t *fb = pull ? (uint32_t *)cv.take(sizeof(uint32_t) * 3 * words) : nullptr;
+    uint32_t *iso = pull && !iso_cached ? (uint32_t *)cv.take(sizeof(uint32_t) * words) : nullptr;
+    if (b) {
+        b->fs = fs;
+        b->vis = vis;
+        b->fb = fb;
+        b->iso = iso;
+    }
+    return cv.used;
+}
+
 }  // namespace
 
 void bfs_run(Ctx *c, const Graph *g, int32_t source, int32_t *depth) {
     switch (opt(c, "bfs_tile", 2048)) {
-    case 1024: bfs_levels<1024>(c, g, source, depth); break;
-    case 4096: bfs_levels<4096>(c, g, source, depth); break;
-    default: bfs_levels<2048>(c, g, source, depth); break;
+    case 1024: bfs_levels_pool<1024>(c, g, source, depth); break;
+    case 4096: bfs_levels_pool<4096>(c, g, source, depth); break;
+    default: bfs_levels_pool<2048>(c, g, source, depth); break;
+    }
+}
+
+// Pulls only on a graph already known to be symmetric (built symmetrized, or
+// checked by an earlier call): checking needs a transpose, which allocates.
+static bool bfs_in_pulls(Ctx *c, const Graph *g) {
+    return opt(c, "bfs_direction", 1) != 0 && graph_symmetry_known(const_cast<Graph *>(g)) == 1;
+}
+
+int64_t bfs_block_bytes(Ctx *c, const Graph *g) {
+    const bool pull = bfs_in_pulls(c, g);
+    return bfs_carve<1024>(g, pull, graph_iso_bits_cached(const_cast<Graph *>(g)) != nullptr,
+                           nullptr, nullptr);
+}
+
+void bfs_run_in(Ctx *c, const Graph *g, int32_t source, int32_t *depth, void *block,
+                int64_t bytes) {
+    const bool pull = bfs_in_pulls(c, g);
+    const uint32_t *iso_cached = graph_iso_bits_cached(const_cast<Graph *>(g));
+    BfsBufs b;
+    const int64_t need = bfs_carve<1024>(g, pull, iso_cached != nullptr, (char *)block, &b);
+    APSPIE_CHECK(need <= bytes, APSPIE_ERR_OVERFLOW,
+                 "BFS needs " + std::to_string(need) + " bytes of scratch, the block holds " +
+                     std::to_string(bytes));
+    if (pull && iso_cached) b.iso = iso_cached;
+    else if (pull) iso_bits_into(c, g, const_cast<uint32_t *>(b.iso));
+    b.kernel_timer = false;  // no events created inside the call
+    switch (opt(c, "bfs_tile", 2048)) {
+    case 1024: bfs_levels<1024>(c, g, source, depth, b); break;
+    case 4096: bfs_levels<4096>(c, g, source, depth, b); break;
+    default: bfs_levels<2048>(c, g, source, depth, b); break;
     }
 }
 

@@ -125,6 +125,20 @@ int guard(Ctx *c, F &&f) {
 
 void set_device(Ctx *c);
 void sync(Ctx *c);
+
+// device time of one algorithm call, bracketed by events on the context stream
+template <class F>
+void timed(Ctx *c, F &&f) {
+    c->stats = apspie_stats{};
+    c->ev1_recorded = false;
+    APSPIE_HIP(hipEventRecord(c->ev0, c->stream));
+    f();
+    if (!c->ev1_recorded) APSPIE_HIP(hipEventRecord(c->ev1, c->stream));
+    APSPIE_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0;
+    APSPIE_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->stats.elapsed_ms = ms;
+}
 // copy-in / copy-out helpers honouring apspie_mem
 void copy_to_device(Ctx *c, void *dst, const void *src, size_t bytes, int mem);
 void copy_from_device(Ctx *c, void *dst, const void *src, size_t bytes, int mem);
@@ -137,6 +151,14 @@ int fill_grid(Ctx *c, int threads, int per_cu);
 // ----------------------------------------------------------- kernel entry
 // (host-side launchers defined in the .hip files)
 void bfs_run(Ctx *c, const Graph *g, int32_t source, int32_t *depth_dev);
+// BFS over a caller-provided device block instead of the scratch pool (no
+// allocation): the bytes a graph needs, and the run (APSPIE_ERR_OVERFLOW when
+// the block is too small).  Pulls only on a graph already known symmetric.
+int64_t bfs_block_bytes(Ctx *c, const Graph *g);
+// device address of the no-alloc BFS's static array, looked up once per device
+void noalloc_init(int device);
+void bfs_run_in(Ctx *c, const Graph *g, int32_t source, int32_t *depth_dev, void *block,
+                int64_t bytes);
 void sssp_run(Ctx *c, const Graph *g, int32_t source, float *dist_dev);
 void mis_run(Ctx *c, const Graph *g, const uint32_t *prio_dev, int32_t *out_dev);
 void tc_run(Ctx *c, const Graph *g, uint64_t *total, int32_t *per_row_dev);
@@ -167,6 +189,12 @@ bool graph_is_symmetric(Ctx *c, Graph *g);
 // for w(u, v))
 bool graph_is_symmetric_weighted(Ctx *c, Graph *g);
 const uint32_t *graph_iso_bits(Ctx *c, Graph *g);
+// What is known without computing anything: the symmetry flag (-1 unknown,
+// 0, 1) and the isolated-vertex bits (null until built)
+int graph_symmetry_known(Graph *g);
+const uint32_t *graph_iso_bits_cached(Graph *g);
+// the isolated-vertex bits of g into caller memory (ceil(m / 32) words)
+void iso_bits_into(Ctx *c, const Graph *g, uint32_t *bits);
 int32_t graph_max_degree(Ctx *c, Graph *g);
 // Smallest and largest value of a weighted graph with entries, computed once
 // (false: no values or no entries).

@@ -180,20 +180,6 @@ static void check_sssp_weights(Ctx *c, const Graph *g) {
                      ", largest " + std::to_string(hi) + ")");
 }
 
-// device time of one algorithm call, bracketed by events on the context stream
-template <class F>
-static void timed(Ctx *c, F &&f) {
-    c->stats = apspie_stats{};
-    c->ev1_recorded = false;
-    APSPIE_HIP(hipEventRecord(c->ev0, c->stream));
-    f();
-    if (!c->ev1_recorded) APSPIE_HIP(hipEventRecord(c->ev1, c->stream));
-    APSPIE_HIP(hipEventSynchronize(c->ev1));
-    float ms = 0;
-    APSPIE_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    c->stats.elapsed_ms = ms;
-}
-
 }  // namespace apspie
 
 using namespace apspie;
@@ -235,6 +221,7 @@ int apspie_ctx_create(int device, apspie_ctx **out) {
         APSPIE_HIP(hipEventCreate(&c->ev0));
         APSPIE_HIP(hipEventCreate(&c->ev1));
         APSPIE_HIP(hipHostMalloc(&c->pinned, 4096, hipHostMallocDefault));
+        noalloc_init(device);  // the no-alloc BFS's static array, once per device
     });
     if (st != APSPIE_OK) {
         delete c;

@@ -380,6 +380,24 @@ const uint32_t *graph_iso_bits(Ctx *c, Graph *g) {
     return bits;
 }
 
+int graph_symmetry_known(Graph *g) {
+    std::lock_guard<std::mutex> lock(g->derived_mu);
+    return g->symmetric;
+}
+
+const uint32_t *graph_iso_bits_cached(Graph *g) {
+    std::lock_guard<std::mutex> lock(g->derived_mu);
+    return g->iso_bits;
+}
+
+void iso_bits_into(Ctx *c, const Graph *g, uint32_t *bits) {
+    const int64_t words = ((int64_t)g->m + 31) / 32;
+    if (words == 0) return;
+    k_iso_bits<<<(int)std::min<int64_t>(ceil_div(words, kBlock), 4 * c->cus), kBlock, 0,
+                 c->stream>>>(g->rowptr, g->m, bits);
+    APSPIE_LAUNCH_CHECK();
+}
+
 bool graph_is_symmetric(Ctx *c, Graph *g) {
     std::lock_guard<std::mutex> lock(g->derived_mu);
     check_symmetry(c, g);

@@ -10,7 +10,7 @@ namespace apspie {
 
 class KernelTimer {
   public:
-    explicit KernelTimer(Ctx *c) : c_(c), on_(c->profiling) {}
+    explicit KernelTimer(Ctx *c, bool allowed = true) : c_(c), on_(allowed && c->profiling) {}
     ~KernelTimer() {
         for (auto &p : ev_) {
             (void)hipEventDestroy(p.first);

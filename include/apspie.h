@@ -176,6 +176,21 @@ int apspie_graph_to_dcsc(apspie_ctx *ctx, const apspie_graph *g, int32_t part_si
  * runs to convergence (the reference's depth argument is not needed). */
 int apspie_bfs(apspie_ctx *ctx, const apspie_graph *g, int32_t source, int32_t *depth,
                int mem);
+/* apspie_bfs that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture elsewhere in the process, serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and depths as apspie_bfs; its frontier,
+ * bitmaps and control block live in one static 96 MiB device array per GPU
+ * (held until the process ends).  The graph needs 16(m+1) + 8(nnz/1024+2)
+ * + 20(m/32+1) bytes plus 2 KiB (and 4m more for a host output): R-MAT
+ * scale 22 fits; larger graphs return APSPIE_ERR_OVERFLOW.  Levels pull only
+ * on a graph already known to be symmetric (built with symmetrize = 1, or
+ * after an apspie_bfs on it); otherwise they push, with the same depths.
+ * Calls on one GPU share the array and keep the last call's contents;
+ * overlapping calls would race on it, so the library serialises this
+ * process's calls per GPU (a host mutex; each call is synchronous). */
+int apspie_bfs_noalloc(apspie_ctx *ctx, const apspie_graph *g, int32_t source, int32_t *depth,
+                       int mem);
 /* Single-source shortest paths under (min,+) over the graph's FP32 weights;
  * unreached = APSPIE_SSSP_INF.  Replaces sssp<V>() (sssp.cuh:27-115) +
  * spmspvSssp() (spmspvSssp.cuh:91-291), with real weights in every round.
