@@ -107,6 +107,7 @@ def _declare(L):
         "apspie_graph_transpose": ([vp, vp, pp], C.c_int),
         "apspie_bfs": ([vp, vp, i32, vp, C.c_int], C.c_int),
         "apspie_bfs_noalloc": ([vp, vp, i32, vp, C.c_int], C.c_int),
+        "apspie_mxv_dense": ([vp, vp, vp, vp, C.POINTER(i64), C.c_int], C.c_int),
         "apspie_sssp": ([vp, vp, i32, vp, C.c_int], C.c_int),
         "apspie_mis": ([vp, vp, vp, C.c_int, vp, C.c_int], C.c_int),
         "apspie_tc": ([vp, vp, C.POINTER(C.c_uint64), vp, C.c_int], C.c_int),
@@ -341,6 +342,16 @@ class Context:
             out = np.empty(g.m, np.float32) if out is None else out
         _check(self._L.apspie_sssp(self.h, g.h, int(source), _p(out), mem), self.h)
         return out
+
+    def mxv_dense(self, g, x, y=None):
+        """Dense mXv: (y, frontier edge count); x > 0.5 is the frontier.  y
+        (host float32, n entries) is left as given when the frontier is
+        empty (zeros when not given)."""
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros(g.n, np.float32) if y is None else y
+        t = C.c_int64()
+        _check(self._L.apspie_mxv_dense(self.h, g.h, _p(x), _p(y), C.byref(t), HOST), self.h)
+        return y, t.value
 
     def mis(self, g, prio=None, out=None, mem=HOST, prio_mem=HOST):
         if mem == HOST:

@@ -165,6 +165,8 @@ void tc_run(Ctx *c, const Graph *g, uint64_t *total, int32_t *per_row_dev);
 int64_t spmspv_run(Ctx *c, const Graph *g, int semiring, int64_t nx, const int32_t *x_ind,
                    const float *x_val, const int32_t *mask, int32_t *y_ind, float *y_val,
                    int path);
+// dense x (m) -> dense y (n) under (+,x); returns the frontier's edge count
+int64_t mxv_dense_run(Ctx *c, const Graph *g, const float *x, float *y, int path);
 Graph *spgemm_run(Ctx *c, const Graph *a, const Graph *b);  // returns new apspie_graph
 // (A * B) o M on M's pattern into cval (device, nnz(M)); returns the entries
 // of M with at least one product

@@ -704,6 +704,33 @@ int apspie_spmspv(apspie_ctx *c, const apspie_graph *g, int semiring, int64_t nx
     });
 }
 
+int apspie_mxv_dense(apspie_ctx *c, const apspie_graph *g, const float *x, float *y,
+                     int64_t *frontier_edges, int mem) {
+    if (!c || !x || !y) return APSPIE_ERR_INVALID;
+    return guard(c, [&] {
+        check_graph(c, g);
+        APSPIE_CHECK(mem == APSPIE_MEM_HOST || mem == APSPIE_MEM_DEVICE, APSPIE_ERR_INVALID,
+                     "mem must be APSPIE_MEM_HOST or APSPIE_MEM_DEVICE");
+        set_device(c);
+        const float *xd = x;
+        float *yd = y;
+        if (mem == APSPIE_MEM_HOST) {
+            float *d = (float *)scratch(c, "mxd_x", sizeof(float) * std::max(g->m, 1));
+            copy_to_device(c, d, x, sizeof(float) * g->m, APSPIE_MEM_HOST);
+            xd = d;
+            yd = (float *)scratch(c, "mxd_y", sizeof(float) * std::max(g->n, 1));
+        }
+        int64_t total = 0;
+        timed(c, [&] { total = mxv_dense_run(c, g, xd, yd, APSPIE_PATH_AUTO); });
+        if (frontier_edges) *frontier_edges = total;
+        // an empty frontier leaves y as it was: nothing to copy back
+        if (mem == APSPIE_MEM_HOST && c->stats.vertices_reached > 0) {
+            copy_from_device(c, y, yd, sizeof(float) * g->n, APSPIE_MEM_HOST);
+            sync(c);
+        }
+    });
+}
+
 int apspie_spgemm(apspie_ctx *c, const apspie_graph *a, const apspie_graph *b,
                   apspie_graph **out) {
     if (!c || !a || !b || !out) return APSPIE_ERR_INVALID;

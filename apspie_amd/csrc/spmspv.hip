@@ -184,6 +184,19 @@ __global__ void k_gather_dense(const int32_t *idx, int64_t k, const float *ydens
         y_val[i] = ydense[idx[i]];
 }
 
+// the dense mXv's frontier: entries above 0.5 (the reference's bitify)
+__global__ void k_above_half(const float *x, int64_t n, int32_t *flag) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        flag[i] = x[i] > 0.5f ? 1 : 0;
+}
+
+__global__ void k_scatter_dense(const int32_t *idx, const float *val, int64_t k, float *y) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < k;
+         i += (int64_t)gridDim.x * blockDim.x)
+        y[idx[i]] = val[i];
+}
+
 float host_identity(int sr) {
     switch (sr) {
     case APSPIE_SR_MIN_PLUS: return INFINITY;
@@ -262,6 +275,37 @@ int64_t spmspv_run(Ctx *c, const Graph *g, int sr, int64_t nx, const int32_t *x_
     k_gather_dense<<<grid, kBlock, 0, c->stream>>>(y_ind, k, yd, y_val);
     APSPIE_LAUNCH_CHECK();
     return k;
+}
+
+// Dense mXv (mXv.cuh:110-247): the entries of x above 0.5 (bitify,
+// mXv.cuh:50-55) compacted in index order (Scan + streamCompact, :142-147)
+// with their values, expanded and reduced by the SpMSpV stages above, y
+// zeroed (preprocessFlag, :185) and the sums scattered (scatterFloat, :205).
+// Returns the sum of the frontier rows' lengths (`total`, :229); with no
+// frontier y is left as it was and 0 is returned (:143-145).
+int64_t mxv_dense_run(Ctx *c, const Graph *g, const float *x, float *y, int path) {
+    const int grid = fill_grid(c, kBlock, 4);
+    int32_t *flag = (int32_t *)scratch(c, "mxd_flag", sizeof(int32_t) * std::max(g->m, 1));
+    int32_t *xi = (int32_t *)scratch(c, "mxd_xi", sizeof(int32_t) * std::max(g->m, 1));
+    k_above_half<<<grid, kBlock, 0, c->stream>>>(x, g->m, flag);
+    APSPIE_LAUNCH_CHECK();
+    const int64_t nx = compact_flags(c, flag, g->m, xi);
+    if (nx == 0) {
+        c->stats.iterations = 1;
+        return 0;
+    }
+    float *xv = (float *)scratch(c, "mxd_xv", sizeof(float) * nx);
+    k_gather_dense<<<grid, kBlock, 0, c->stream>>>(xi, nx, x, xv);
+    int32_t *yi = (int32_t *)scratch(c, "mxd_yi", sizeof(int32_t) * std::max(g->n, 1));
+    float *yv = (float *)scratch(c, "mxd_yv", sizeof(float) * std::max(g->n, 1));
+    APSPIE_LAUNCH_CHECK();
+    const int64_t k = spmspv_run(c, g, APSPIE_SR_PLUS_TIMES, nx, xi, xv, nullptr, yi, yv, path);
+    const int64_t total = c->stats.edges_traversed;
+    c->stats.vertices_reached = nx;
+    k_fill_f32<<<grid, kBlock, 0, c->stream>>>(y, 0.0f, g->n);
+    if (k > 0) k_scatter_dense<<<grid, kBlock, 0, c->stream>>>(yi, yv, k, y);
+    APSPIE_LAUNCH_CHECK();
+    return total;
 }
 
 }  // namespace apspie

@@ -224,6 +224,16 @@ int apspie_tc(apspie_ctx *ctx, const apspie_graph *g, uint64_t *total, int32_t *
 int apspie_spmspv(apspie_ctx *ctx, const apspie_graph *g, int semiring, int64_t nx,
                   const int32_t *x_ind, const float *x_val, const int32_t *mask,
                   int64_t *ny, int32_t *y_ind, float *y_val, int mem, int path);
+/* Dense mXv (mXv.cuh:110-247): y = A^T x over (+,x) with x (m entries) and
+ * y (n entries) dense.  The entries of x above 0.5 form the frontier (the
+ * reference's bitify, mXv.cuh:50-55) with their values as x_v; y is zeroed,
+ * then y[r] = sum over frontier v of x_v * a_vr (a_vr = 1 for a graph
+ * without values).  *frontier_edges (host, may be NULL) = the sum of the
+ * frontier rows' lengths, the reference's return value (mXv.cuh:229).  An
+ * empty frontier leaves y untouched and gives 0 (mXv.cuh:143-145).  x and y
+ * live in `mem`. */
+int apspie_mxv_dense(apspie_ctx *ctx, const apspie_graph *g, const float *x, float *y,
+                     int64_t *frontier_edges, int mem);
 /* C = A * B over (+,x) by row-binned hash SpGEMM (spgemm(), spgemm.cuh:131-342
  * + LaunchKernel spgemmKernel.cuh:438-738).  C is a new device graph with
  * sorted rows. */

@@ -334,12 +334,140 @@ inline int64_t spgemmMasked(float *d_cVal, d_matrix *A, d_matrix *B, d_matrix *M
     return nnz;
 }
 
-// mXv (mXv.cuh:249-250), sparse in / sparse out: y = A^T x under (+,x) on
-// device arrays; returns nnz(y) (the reference's in/out `nnz`).
+// The context reference callers get when a reference entry point takes no
+// context (allocScratch): one per process, on device 0, as the reference
+// drivers create it (mgpu::CreateCudaDevice(0), test/bfs/main.cu:27).
+inline Context &default_context() {
+    static Context ctx(0);
+    return ctx;
+}
+
+// d_scratch (scratch.hpp:5-32): the workspace a reference caller allocates
+// once with allocScratch and hands to mXv.  The device arrays are the
+// reference's, sized as it sizes them, taken from a context's memory
+// (apspie_dev_alloc: counted against its HBM budget) so that callers that
+// fill them (d_index, d_ones, d_cscColDiff, tc.cuh:236-254) keep working;
+// mXv itself draws its working memory from the context's scratch pool.
+struct d_scratch {
+    int *d_cscVecInd = nullptr, *d_cscSwapInd = nullptr;
+    float *d_cscVecVal = nullptr, *d_cscSwapVal = nullptr, *d_cscTempVal = nullptr;
+    int *d_cscColGood = nullptr, *d_cscColBad = nullptr, *d_cscColDiff = nullptr;
+    int *d_ones = nullptr, *d_index = nullptr;
+    void *d_temp_storage = nullptr;
+    int *d_randVecInd = nullptr;
+    int *h_cscVecInd = nullptr;
+    float *h_cscVecVal = nullptr;
+    int *h_cscColDiff = nullptr, *h_ones = nullptr, *h_index = nullptr, *h_bfsResult = nullptr;
+    float *h_spmvResult = nullptr, *h_bfsValA = nullptr;
+    apspie_ctx *ctx = nullptr;  // the context the device arrays came from
+    int edge = 0, m = 0;
+};
+
+// allocScratch (util.cuh:646-672): device arrays of edge or m entries as the
+// reference allocates them, host arrays by malloc.
+inline void allocScratch(d_scratch **d, const int edge, const int m, Context &ctx) {
+    d_scratch *s = new d_scratch();
+    s->ctx = ctx.get();
+    s->edge = edge;
+    s->m = m;
+    const int64_t e = edge > 0 ? edge : 1, n = m > 0 ? m : 1;
+    auto dev = [&](void **p, int64_t count) {
+        check(apspie_dev_alloc(s->ctx, 4 * count, p), s->ctx, "allocScratch");
+    };
+    dev((void **)&s->d_cscVecInd, e);
+    dev((void **)&s->d_cscSwapInd, e);
+    dev((void **)&s->d_cscVecVal, e);
+    dev((void **)&s->d_cscSwapVal, e);
+    dev((void **)&s->d_cscTempVal, e);
+    dev((void **)&s->d_cscColGood, e);
+    dev((void **)&s->d_cscColBad, n);
+    dev((void **)&s->d_cscColDiff, n);
+    dev((void **)&s->d_ones, n);
+    dev((void **)&s->d_index, n);
+    dev((void **)&s->d_randVecInd, n);
+    s->h_cscVecInd = static_cast<int *>(std::malloc(4 * (size_t)e));
+    s->h_cscVecVal = static_cast<float *>(std::malloc(4 * (size_t)e));
+    s->h_cscColDiff = static_cast<int *>(std::malloc(4 * (size_t)n));
+    s->h_ones = static_cast<int *>(std::malloc(4 * (size_t)n));
+    s->h_index = static_cast<int *>(std::malloc(4 * (size_t)n));
+    s->h_bfsResult = static_cast<int *>(std::malloc(4 * (size_t)n));
+    s->h_spmvResult = static_cast<float *>(std::malloc(4 * (size_t)n));
+    s->h_bfsValA = static_cast<float *>(std::malloc(4 * (size_t)e));
+    *d = s;
+}
+
+// The reference signature: the arrays come from default_context().
+inline void allocScratch(d_scratch **d, const int edge, const int m) {
+    allocScratch(d, edge, m, default_context());
+}
+
+// The reference never frees its scratch; this returns everything
+// allocScratch took (device arrays to their context).
+inline void freeScratch(d_scratch *d) {
+    if (!d) return;
+    const int64_t e = d->edge > 0 ? d->edge : 1, n = d->m > 0 ? d->m : 1;
+    void *dev_e[] = {d->d_cscVecInd, d->d_cscSwapInd, d->d_cscVecVal,
+                     d->d_cscSwapVal, d->d_cscTempVal, d->d_cscColGood};
+    void *dev_n[] = {d->d_cscColBad, d->d_cscColDiff, d->d_ones, d->d_index, d->d_randVecInd};
+    for (void *p : dev_e)
+        if (p) apspie_dev_free(d->ctx, p, 4 * e);
+    for (void *p : dev_n)
+        if (p) apspie_dev_free(d->ctx, p, 4 * n);
+    void *host[] = {d->h_cscVecInd, d->h_cscVecVal, d->h_cscColDiff, d->h_ones,
+                    d->h_index,     d->h_bfsResult, d->h_spmvResult, d->h_bfsValA};
+    for (void *p : host) std::free(p);
+    delete d;
+}
+
+// Dense mXv (mXv.cuh:110-111): d_mmResult = A x over (+,x) with A in CSC
+// (column v's rows d_cscRowInd[d_cscColPtr[v] ..]) and x = d_randVec, both
+// vectors dense on the device.  The entries of x above 0.5 form the frontier;
+// d_mmResult is zeroed and receives the sums; returns the sum of the
+// frontier columns' lengths.  With no frontier d_mmResult is left as it was
+// and 0 is returned.  `d` is accepted for the reference's signature.
 template <typename T>
-int mXv(const int *d_xInd, const T *d_xVal, int edge, int m, int nnz_x, const T *d_cscVal,
-        const int *d_cscColPtr, const int *d_cscRowInd, int *d_resultInd, T *d_resultVal,
-        Context &ctx, int semiring = APSPIE_SR_PLUS_TIMES) {
+int mXv(const T *d_randVec, const int edge, const int m, const T *d_cscVal,
+        const int *d_cscColPtr, const int *d_cscRowInd, T *d_mmResult, d_scratch * /*d*/,
+        Context &context) {
+    static_assert(sizeof(T) == sizeof(float), "values are FP32");
+    GraphView g(context, m, m, edge, d_cscColPtr, d_cscRowInd,
+                reinterpret_cast<const float *>(d_cscVal));
+    int64_t total = 0;
+    check(apspie_mxv_dense(context.get(), g.get(), reinterpret_cast<const float *>(d_randVec),
+                           reinterpret_cast<float *>(d_mmResult), &total, APSPIE_MEM_DEVICE),
+          context.get(), "mXv");
+    return (int)total;
+}
+
+// Sparse mXv (mXv.cuh:249-250): y = A x over (+,x) with x as nnz sorted
+// indices d_randVecInd and values d_randVecVal (device), A in CSC.  On return
+// nnz is y's entry count and d_resultInd / d_resultVal (device, capacity m)
+// hold y's indices in ascending order and their sums.  nnz == 0 returns at
+// once (mXv.cuh:281-283).  `d` is accepted for the reference's signature.
+template <typename T>
+void mXv(const int *d_randVecInd, const T *d_randVecVal, const int edge, const int m, int &nnz,
+         const T *d_cscVal, const int *d_cscColPtr, const int *d_cscRowInd, int *d_resultInd,
+         T *d_resultVal, d_scratch * /*d*/, Context &context) {
+    static_assert(sizeof(T) == sizeof(float), "values are FP32");
+    if (nnz == 0) return;
+    GraphView g(context, m, m, edge, d_cscColPtr, d_cscRowInd,
+                reinterpret_cast<const float *>(d_cscVal));
+    int64_t ny = 0;
+    check(apspie_spmspv(context.get(), g.get(), APSPIE_SR_PLUS_TIMES, nnz, d_randVecInd,
+                        reinterpret_cast<const float *>(d_randVecVal), nullptr, &ny, d_resultInd,
+                        reinterpret_cast<float *>(d_resultVal), APSPIE_MEM_DEVICE,
+                        APSPIE_PATH_AUTO),
+          context.get(), "mXv");
+    nnz = (int)ny;
+}
+
+// SpMSpV under any of the four semirings (the per-algorithm copies of mXv:
+// spmspvBfs.cuh:174-222, spmspvSssp.cuh:172-241, spmspvMis.cuh:245-293);
+// returns nnz(y).
+template <typename T>
+int spmspv(const int *d_xInd, const T *d_xVal, int edge, int m, int nnz_x, const T *d_cscVal,
+           const int *d_cscColPtr, const int *d_cscRowInd, int *d_resultInd, T *d_resultVal,
+           int semiring, Context &ctx) {
     GraphView g(ctx, m, m, edge, d_cscColPtr, d_cscRowInd,
                 reinterpret_cast<const float *>(d_cscVal));
     int64_t ny = 0;
@@ -347,7 +475,7 @@ int mXv(const int *d_xInd, const T *d_xVal, int edge, int m, int nnz_x, const T 
                         reinterpret_cast<const float *>(d_xVal), nullptr, &ny, d_resultInd,
                         reinterpret_cast<float *>(d_resultVal), APSPIE_MEM_DEVICE,
                         APSPIE_PATH_AUTO),
-          ctx.get(), "mXv");
+          ctx.get(), "spmspv");
     return (int)ny;
 }
 
@@ -370,6 +498,34 @@ int mXm(int edge, int m, const T *d_cscValA, const int *d_cscColPtrA, const int 
     check(apspie_mxm(ctx.get(), a.get(), b.get(), &c), ctx.get(), "mXm");
     attach(C, c, ctx);
     return C->nnz;
+}
+
+// mXm with the reference's argument list (tc.cuh:314-316): C's column
+// offsets (m+1), row indices and values go into caller-owned device arrays.
+// The reference sizes C's entry arrays at 16 * edge (tc.cuh:328-329); a
+// product with more entries than that returns APSPIE_ERR_OVERFLOW instead of
+// writing past them.  Returns nnz(C).
+template <typename T>
+int mXm(const int edge, const int m, const T *d_cscValA, const int *d_cscColPtrA,
+        const int *d_cscRowIndA, const T *d_cscValB, const int *h_cscColPtrB,
+        const int *d_cscColPtrB, const int *d_cscRowIndB, T *d_cscValC, int *d_cscColPtrC,
+        int *d_cscRowIndC, Context &ctx) {
+    d_matrix C;
+    const int nnz = mXm<T>(edge, m, d_cscValA, d_cscColPtrA, d_cscRowIndA, d_cscValB,
+                           h_cscColPtrB, d_cscColPtrB, d_cscRowIndB, &C, ctx);
+    if ((int64_t)nnz > 16 * (int64_t)edge) {
+        matrix_delete(&C);
+        check(APSPIE_ERR_OVERFLOW, nullptr, "mXm: C has more than 16 * edge entries");
+    }
+    check(apspie_memcpy(ctx.get(), d_cscColPtrC, C.d_cscColPtr, 4 * ((int64_t)m + 1), 2),
+          ctx.get(), "mXm");
+    check(apspie_memcpy(ctx.get(), d_cscRowIndC, C.d_cscRowInd, 4 * (int64_t)nnz, 2), ctx.get(),
+          "mXm");
+    check(apspie_memcpy(ctx.get(), d_cscValC, C.d_cscVal, 4 * (int64_t)nnz, 2), ctx.get(),
+          "mXm");
+    check(apspie_ctx_sync(ctx.get()), ctx.get(), "mXm");
+    matrix_delete(&C);
+    return nnz;
 }
 
 }  // namespace apspie

@@ -277,6 +277,26 @@ def spmspv(m, rowptr, col, aval, semiring, x_ind, x_val=None, mask=None):
     return y_ind[:ny].copy(), y_val[:ny].copy()
 
 
+def mxv_dense(m, n, rowptr, col, aval, x, y=None):
+    """Dense mXv (mXv.cuh:110-247): returns (y, total).  The frontier is the
+    entries of x above 0.5 (bitify, mXv.cuh:50-55) in index order (Scan +
+    streamCompact, :142-147); y (n entries) is zeroed (preprocessFlag, :185)
+    and receives the sparse mXv of the frontier with x's values (the same
+    expand / sort / reduce-by-key stages, :153-205, restated by or_spmspv);
+    total = the sum of the frontier rows' lengths (:162, returned at :229).
+    An empty frontier returns y as given and 0 (:143-145)."""
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.zeros(n, np.float32) if y is None else np.array(y, np.float32)
+    x_ind = np.nonzero(x > np.float32(0.5))[0].astype(np.int32)
+    if len(x_ind) == 0:
+        return y, 0
+    total = int((rowptr[x_ind + 1] - rowptr[x_ind]).sum())
+    yi, yv = spmspv(max(m, n), rowptr, col, aval, SR_PLUS_TIMES, x_ind, x[x_ind])
+    y[:] = 0
+    y[yi] = yv
+    return y, total
+
+
 def spgemm(m, k, n, a_rowptr, a_col, a_val, b_rowptr, b_col, b_val):
     L = lib()
     a_val = None if a_val is None else np.ascontiguousarray(a_val, np.float32)

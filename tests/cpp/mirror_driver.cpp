@@ -111,6 +111,106 @@ int main(int argc, char **argv) {
         std::printf("\n");
         apspie::matrix_delete(&D);
 
+        // mXm with the reference's argument list: C into the caller's arrays
+        {
+            const int64_t cap = 16 * (int64_t)A.nnz;
+            int *d_cp = nullptr, *d_ri = nullptr;
+            float *d_cv = nullptr;
+            apspie::check(apspie_dev_alloc(ctx.get(), 4 * ((int64_t)A.m + 1), (void **)&d_cp),
+                          ctx.get(), "alloc");
+            apspie::check(apspie_dev_alloc(ctx.get(), 4 * cap, (void **)&d_ri), ctx.get(), "alloc");
+            apspie::check(apspie_dev_alloc(ctx.get(), 4 * cap, (void **)&d_cv), ctx.get(), "alloc");
+            const int cn = apspie::mXm<float>(A.nnz, A.m, d_w, A.d_cscColPtr, A.d_cscRowInd, d_w,
+                                              A.h_cscColPtr, A.d_cscColPtr, A.d_cscRowInd, d_cv,
+                                              d_cp, d_ri, ctx);
+            std::vector<int> cp(A.m + 1), ri(cn);
+            std::vector<float> cv(cn);
+            apspie::check(apspie_memcpy(ctx.get(), cp.data(), d_cp, 4 * ((int64_t)A.m + 1), 1),
+                          ctx.get(), "copy");
+            apspie::check(apspie_memcpy(ctx.get(), ri.data(), d_ri, 4 * (int64_t)cn, 1), ctx.get(),
+                          "copy");
+            apspie::check(apspie_memcpy(ctx.get(), cv.data(), d_cv, 4 * (int64_t)cn, 1), ctx.get(),
+                          "copy");
+            std::printf("mxm3_colptr");
+            for (int v : cp) std::printf(" %d", v);
+            std::printf("\nmxm3_rows");
+            for (int v : ri) std::printf(" %d", v);
+            std::printf("\nmxm3_vals");
+            for (float v : cv) std::printf(" %.9g", v);
+            std::printf("\n");
+            apspie_dev_free(ctx.get(), d_cp, 4 * ((int64_t)A.m + 1));
+            apspie_dev_free(ctx.get(), d_ri, 4 * cap);
+            apspie_dev_free(ctx.get(), d_cv, 4 * cap);
+        }
+
+        // allocScratch (util.cuh:646) + the two mXv forms (mXv.cuh:110, :249)
+        {
+            apspie::d_scratch *d = nullptr;
+            apspie::allocScratch(&d, A.nnz, A.m);
+            // dense x: every fifth vertex in the frontier with value 1..3,
+            // 0.5 and 0.25 (not above 0.5) elsewhere in a pattern
+            std::vector<float> x(A.m), y(A.m);
+            for (int i = 0; i < A.m; ++i)
+                x[i] = i % 5 == 0 ? (float)(i % 3 + 1) : i % 7 == 1 ? 0.5f : i % 7 == 2 ? 0.25f : 0;
+            float *d_x = nullptr, *d_y = nullptr;
+            apspie::check(apspie_dev_alloc(ctx.get(), 4 * (int64_t)A.m, (void **)&d_x), ctx.get(),
+                          "alloc");
+            apspie::check(apspie_dev_alloc(ctx.get(), 4 * (int64_t)A.m, (void **)&d_y), ctx.get(),
+                          "alloc");
+            apspie::check(apspie_memcpy(ctx.get(), d_x, x.data(), 4 * (int64_t)A.m, 0), ctx.get(),
+                          "copy");
+            const int total = apspie::mXv<float>(d_x, A.nnz, A.m, d_w, A.d_cscColPtr,
+                                                 A.d_cscRowInd, d_y, d, ctx);
+            apspie::check(apspie_memcpy(ctx.get(), y.data(), d_y, 4 * (int64_t)A.m, 1), ctx.get(),
+                          "copy");
+            std::printf("mxv_dense_total %d\nmxv_dense_y", total);
+            for (float v : y) std::printf(" %.9g", v);
+            std::printf("\n");
+            // no entry above 0.5: y keeps what it held, 0 is returned
+            for (int i = 0; i < A.m; ++i) x[i] = i % 2 ? 0.5f : 0.0f;
+            apspie::check(apspie_memcpy(ctx.get(), d_x, x.data(), 4 * (int64_t)A.m, 0), ctx.get(),
+                          "copy");
+            const int none = apspie::mXv<float>(d_x, A.nnz, A.m, d_w, A.d_cscColPtr,
+                                                A.d_cscRowInd, d_y, d, ctx);
+            std::vector<float> y2(A.m);
+            apspie::check(apspie_memcpy(ctx.get(), y2.data(), d_y, 4 * (int64_t)A.m, 1), ctx.get(),
+                          "copy");
+            std::printf("mxv_dense_empty %d %d\n", none, (int)(y2 == y));
+
+            // sparse x: the source's row with its weights (a column of A)
+            const int r0 = A.h_cscColPtr[source], r1 = A.h_cscColPtr[source + 1];
+            int nnz_x = r1 - r0;
+            int *d_yi = nullptr;
+            float *d_yv = nullptr;
+            apspie::check(apspie_dev_alloc(ctx.get(), 4 * (int64_t)A.m, (void **)&d_yi), ctx.get(),
+                          "alloc");
+            apspie::check(apspie_dev_alloc(ctx.get(), 4 * (int64_t)A.m, (void **)&d_yv), ctx.get(),
+                          "alloc");
+            int nnz = nnz_x;
+            apspie::mXv<float>(A.d_cscRowInd + r0, d_w + r0, A.nnz, A.m, nnz, d_w,
+                               A.d_cscColPtr, A.d_cscRowInd, d_yi, d_yv, d, ctx);
+            std::vector<int> yi(nnz);
+            std::vector<float> yv(nnz);
+            apspie::check(apspie_memcpy(ctx.get(), yi.data(), d_yi, 4 * (int64_t)nnz, 1), ctx.get(),
+                          "copy");
+            apspie::check(apspie_memcpy(ctx.get(), yv.data(), d_yv, 4 * (int64_t)nnz, 1), ctx.get(),
+                          "copy");
+            std::printf("mxv_sparse_nnz %d\nmxv_sparse_ind", nnz);
+            for (int v : yi) std::printf(" %d", v);
+            std::printf("\nmxv_sparse_val");
+            for (float v : yv) std::printf(" %.9g", v);
+            std::printf("\n");
+            int zero = 0;  // nnz == 0 returns at once
+            apspie::mXv<float>(A.d_cscRowInd, d_w, A.nnz, A.m, zero, d_w, A.d_cscColPtr,
+                               A.d_cscRowInd, d_yi, d_yv, d, ctx);
+            std::printf("mxv_sparse_zero %d\n", zero);
+            apspie_dev_free(ctx.get(), d_x, 4 * (int64_t)A.m);
+            apspie_dev_free(ctx.get(), d_y, 4 * (int64_t)A.m);
+            apspie_dev_free(ctx.get(), d_yi, 4 * (int64_t)A.m);
+            apspie_dev_free(ctx.get(), d_yv, 4 * (int64_t)A.m);
+            apspie::freeScratch(d);
+        }
+
         // csr_to_dcsc (spgemm.cuh:138-139 call pattern) + copy_part
         const int pn = A.m < 4 ? A.m : 4, ps = A.m / pn;
         apspie::csr_to_dcsc<float>(&A, ps, pn, ctx, true);

@@ -56,3 +56,20 @@ def test_mirror_driver_matches_oracle(name):
     assert [int(x) for x in out["dcsc_col_ind"]] == ci.tolist()
     assert [int(x) for x in out["dcsc_col_off"]] == co.tolist()
     assert [int(x) for x in out["dcsc_row_ind"]] == ri.tolist()
+    # mXm with the reference's argument list: the same C as the d_matrix form
+    assert out["mxm3_colptr"] == out["mxm_colptr"]
+    assert out["mxm3_rows"] == out["mxm_rows"] and out["mxm3_vals"] == out["mxm_vals"]
+    # dense mXv (mXv.cuh:110): the driver's x, frontier = entries above 0.5
+    i = np.arange(g.m)
+    x = np.where(i % 5 == 0, i % 3 + 1, np.where(i % 7 == 1, 0.5, np.where(i % 7 == 2, 0.25, 0)))
+    y, total = O.mxv_dense(g.m, g.m, g.rowptr, g.col, g.w, x.astype(np.float32))
+    assert int(out["mxv_dense_total"][0]) == total
+    assert np.array_equal(np.array([float(v) for v in out["mxv_dense_y"]], np.float32), y)
+    assert out["mxv_dense_empty"] == ["0", "1"]  # no frontier: 0, y unchanged
+    # sparse mXv (mXv.cuh:249): x = the source's row with its weights
+    r0, r1 = g.rowptr[src], g.rowptr[src + 1]
+    yi, yv = O.spmspv(g.m, g.rowptr, g.col, g.w, O.SR_PLUS_TIMES, g.col[r0:r1], g.w[r0:r1])
+    assert int(out["mxv_sparse_nnz"][0]) == len(yi)
+    assert [int(v) for v in out["mxv_sparse_ind"]] == yi.tolist()
+    assert np.array_equal(np.array([float(v) for v in out["mxv_sparse_val"]], np.float32), yv)
+    assert out["mxv_sparse_zero"] == ["0"]

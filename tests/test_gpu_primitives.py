@@ -90,6 +90,33 @@ def test_spmspv_empty_and_zero_degree(ctx):
     G.close()
 
 
+def test_mxv_dense_vs_oracle(ctx):
+    """Dense mXv through the C ABI: frontier = x above 0.5 (0.5 itself
+    excluded), y zeroed then summed, the frontier edge count returned; a
+    rectangular matrix, R-MAT and a lattice, at several frontier sizes; an
+    empty frontier leaves y untouched."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(5)
+    R = sp.random(700, 450, 0.02, random_state=3, format="csr")
+    R.sort_indices()
+    cases = [(700, 450, R.indptr.astype(np.int64), R.indices.astype(np.int32))]
+    for g in (T.rmat(12), T.lattice(32)):
+        cases.append((g.m, g.m, g.rowptr, g.col))
+    for m, n, rp, col in cases:
+        a = rng.integers(1, 9, len(col)).astype(np.float32)
+        G = ctx.upload(rp.astype(np.int32), col, a, n=n)
+        for frac in (0.001, 0.05, 0.6):
+            x = np.where(rng.random(m) < frac, rng.integers(1, 20, m), 0).astype(np.float32)
+            x[rng.random(m) < 0.05] = 0.5
+            y0 = np.full(n, 3, np.float32)
+            got, total = ctx.mxv_dense(G, x, y0.copy())
+            want, wt = O.mxv_dense(m, n, rp, col, a, x, y0)
+            assert np.array_equal(got, want) and total == wt, (m, n, frac)
+        got, total = ctx.mxv_dense(G, np.full(m, 0.5, np.float32), np.full(n, 3, np.float32))
+        assert total == 0 and (got == 3).all()
+        G.close()
+
+
 def _scipy_csr(g_or_mat):
     return g_or_mat
 

@@ -545,3 +545,42 @@ def test_planted_tc_mistake_is_caught():
                  for k in L_c[L_rp[r]:L_rp[r + 1]]) for r in range(m)]
     assert rows.tolist() == [r * (r - 1) // 2 for r in range(n)]
     assert wrong != rows.tolist()
+
+
+def test_mxv_dense_vs_scipy():
+    """Dense mXv: y = A^T (x restricted to x > 0.5), every y entry written
+    (zeros included), total = the frontier rows' lengths.  A is rectangular
+    (a transposed operand fails), x holds 0.5 exactly (>= instead of > fails)
+    and 0.75 (exact sums: the products are multiples of 0.25),
+    negative and small positive entries (not in the frontier) and a frontier
+    row with no entries."""
+    rng = np.random.default_rng(11)
+    for seed in range(3):
+        A = sp.random(60, 45, 0.08, random_state=seed, format="csr")
+        A.data = rng.integers(1, 9, A.nnz).astype(np.float64)
+        A.sort_indices()
+        rp = A.indptr.astype(np.int64)
+        x = rng.choice([0.0, 0.3, 0.5, 0.75, 1.0, 2.0, -3.0], 60).astype(np.float32)
+        x[np.diff(rp) == 0] = 2.0  # rows without entries in the frontier
+        y, total = O.mxv_dense(60, 45, rp, A.indices.astype(np.int32),
+                               A.data.astype(np.float32), x, np.full(45, 7, np.float32))
+        xf = np.where(x > 0.5, x, 0).astype(np.float64)
+        assert y.tolist() == (A.T @ xf).astype(np.float32).tolist()
+        assert total == int(np.diff(rp)[x > 0.5].sum())
+
+
+def test_mxv_dense_hand_cases():
+    # A = [[1 2] [0 3]], x = [1, 2]: y = [1, 2 + 6] = [1, 8]; total = 3 entries
+    rp, col, val = np.array([0, 2, 3]), np.array([0, 1], np.int32), None
+    col = np.array([0, 1, 1], np.int32)
+    val = np.array([1, 2, 3], np.float32)
+    y, t = O.mxv_dense(2, 2, rp, col, val, np.array([1, 2], np.float32))
+    assert y.tolist() == [1.0, 8.0] and t == 3
+    # x = [0.5, 0.2]: no frontier -> y untouched, 0
+    y, t = O.mxv_dense(2, 2, rp, col, val, np.array([0.5, 0.2], np.float32),
+                       np.array([4, 5], np.float32))
+    assert y.tolist() == [4.0, 5.0] and t == 0
+    # frontier row 1 only: y[0] is zeroed even though nothing lands there
+    y, t = O.mxv_dense(2, 2, rp, col, val, np.array([0, 1], np.float32),
+                       np.array([4, 5], np.float32))
+    assert y.tolist() == [0.0, 3.0] and t == 1
