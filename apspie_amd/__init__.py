@@ -64,6 +64,11 @@ class _ServerCfg(C.Structure):
                 ("queue_capacity", C.c_int32)]
 
 
+class _ServerStats(C.Structure):
+    _fields_ = [("batches", C.c_int64), ("queries", C.c_int64), ("busy_ms", C.c_double),
+                ("traverse_ms", C.c_double), ("copy_ms", C.c_double), ("alloc_ms", C.c_double)]
+
+
 def lib():
     """The loaded libapspie.so (raises if it was not built)."""
     global _lib
@@ -108,6 +113,7 @@ def _declare(L):
         "apspie_bfs": ([vp, vp, i32, vp, C.c_int], C.c_int),
         "apspie_bfs_noalloc": ([vp, vp, i32, vp, C.c_int], C.c_int),
         "apspie_mxv_dense": ([vp, vp, vp, vp, C.POINTER(i64), C.c_int], C.c_int),
+        "apspie_components": ([vp, vp, vp, C.c_int], C.c_int),
         "apspie_sssp": ([vp, vp, i32, vp, C.c_int], C.c_int),
         "apspie_mis": ([vp, vp, vp, C.c_int, vp, C.c_int], C.c_int),
         "apspie_tc": ([vp, vp, C.POINTER(C.c_uint64), vp, C.c_int], C.c_int),
@@ -141,9 +147,13 @@ def _declare(L):
         "apspie_server_last_error": ([vp], C.c_char_p),
         "apspie_server_busy": ([vp, i32, C.c_int, C.POINTER(i64), C.POINTER(i64),
                                 C.POINTER(f64), C.c_int], C.c_int),
+        "apspie_server_busy_stats": ([vp, i32, C.c_int, C.POINTER(_ServerStats), C.c_int],
+                                     C.c_int),
         "apspie_server_destroy": ([vp], C.c_int),
         "apspie_server_add_graph": ([vp, C.POINTER(_Csr), C.POINTER(i32)], C.c_int),
         "apspie_server_submit": ([vp, i32, C.c_int, i32, C.POINTER(i64)], C.c_int),
+        "apspie_server_submit_into": ([vp, i32, C.c_int, i32, vp, C.c_int, C.POINTER(i64)],
+                                      C.c_int),
         "apspie_server_wait": ([vp, i64, vp, i64, C.POINTER(f64)], C.c_int),
         "apspie_server_wait_view": ([vp, i64, C.POINTER(vp), C.POINTER(i64), C.POINTER(f64)],
                                     C.c_int),
@@ -341,6 +351,12 @@ class Context:
         if mem == HOST:
             out = np.empty(g.m, np.float32) if out is None else out
         _check(self._L.apspie_sssp(self.h, g.h, int(source), _p(out), mem), self.h)
+        return out
+
+    def components(self, g):
+        """Connected-component labels (smallest vertex id of each component)."""
+        out = np.empty(g.m, np.int32)
+        _check(self._L.apspie_components(self.h, g.h, _p(out), HOST), self.h)
         return out
 
     def mxv_dense(self, g, x, y=None):

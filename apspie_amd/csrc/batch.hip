@@ -103,10 +103,16 @@ struct BfsBatchOp {
 // next frontier kQueue at a time: one 64-bit reservation per flush instead of
 // one per pass (a pass covers only 256 / W * 4 vertices, and thousands of
 // reservations on one counter serialise in the L2).
-constexpr int kQueue = 2048;
+constexpr int kQueue = 1536;
+// rows longer than kLongRow still open after their first probes are finished
+// after the pass by a whole wave each (kLongCap per block and pass, every
+// vertex of a pass at W = 1)
+constexpr int32_t kLongRow = 64;
+constexpr int kLongCap = 1024;
 struct PullQueue {
     int32_t v[kQueue], rs[kQueue], d[kQueue];
-    uint32_t n;
+    int32_t lv[kLongCap];
+    uint32_t n, nl;
     uint32_t scan32[kBlock / 64 + 1];
     uint64_t scan[kBlock / 64 + 1];
     uint64_t base;
@@ -136,12 +142,99 @@ __device__ __forceinline__ void queue_flush(PullQueue &q, FrontierBufs nxt, uint
     __syncthreads();
 }
 
+// Writes one pulled vertex's word w: nw = the bits found and not yet visited
+// (uv = the word's unvisited bits).  Returns whether anything was found.
+template <int W>
+__device__ __forceinline__ void pull_finish(int64_t i, uint32_t nw, uint32_t uv, uint32_t *vis,
+                                            uint32_t *next, uint32_t *snap, int32_t *depth,
+                                            int32_t level, int64_t v, int w) {
+    // with a snapshot the level's discoveries are one full bit row per
+    // vertex (the depths are read back from it at the end)
+    if (snap) snap[i] = nw;
+    next[i] = nw;  // every row: the buffer needs no clearing before a pull
+    if (nw) {
+        vis[i] = ~uv | nw;
+        if (!snap)
+            while (nw) {
+                const int b = __ffs(nw) - 1;
+                nw &= nw - 1;
+                depth[v * (32 * W) + w * 32 + b] = level;
+            }
+    }
+}
+
+// The deferred long rows of a pull pass (q.lv[0 .. nl)), one wave per row:
+// lane l ORs the frontier rows of entries l, l + 64, ...; every kLongStep
+// entries the wave checks whether every unvisited bit of every word has been
+// found.  Lane w < W then writes word w like pull_finish, and lane 0 appends
+// a claim to the queue (room was made for nl claims).
+constexpr int kLongStep = 256;
+template <int W>
+__device__ __forceinline__ void pull_long(PullQueue &q, uint32_t nl,
+                                          const int32_t *__restrict__ rowptr,
+                                          const int32_t *__restrict__ col,
+                                          const uint32_t *__restrict__ front, uint32_t *vis,
+                                          uint32_t *next, uint32_t *snap, int32_t *depth,
+                                          int32_t level) {
+    const int lane = dev::lane_id();
+    const int wave = threadIdx.x / 64, nwaves = kBlock / 64;
+    for (uint32_t j = wave; j < nl; j += nwaves) {
+        const int64_t v = q.lv[j];
+        const int32_t rs = rowptr[v], re = rowptr[v + 1];
+        uint32_t uv[W], acc[W];
+        load_row<W>(vis + v * W, uv);
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            uv[w] = ~uv[w];
+            acc[w] = 0;
+        }
+        for (int32_t base = rs; base < re; base += kLongStep) {
+#pragma unroll
+            for (int t = 0; t < kLongStep / 64; ++t) {
+                const int32_t e = base + t * 64 + lane;
+                if (e < re) {
+                    uint32_t f[W];
+                    load_row<W>(front + (int64_t)col[e] * W, f);
+#pragma unroll
+                    for (int w = 0; w < W; ++w) acc[w] |= f[w];
+                }
+            }
+            bool done = true;
+#pragma unroll
+            for (int w = 0; w < W; ++w) done = done && (dev::wave_or_u32(acc[w]) & uv[w]) == uv[w];
+            if (done) break;
+        }
+        uint32_t mine = 0, my_uv = 0, any = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const uint32_t nw = dev::wave_or_u32(acc[w]) & uv[w];
+            any |= nw;
+            if (lane == w) {
+                mine = nw;
+                my_uv = uv[w];
+            }
+        }
+        if (lane < W) pull_finish<W>(v * W + lane, mine, my_uv, vis, next, snap, depth, level, v, lane);
+        if (lane == 0 && any) {
+            const uint32_t pos = atomicAdd(&q.n, 1u);
+            q.v[pos] = (int32_t)v;
+            q.rs[pos] = rs;
+            q.d[pos] = re - rs;
+        }
+    }
+}
+
 // Pull form of one batched BFS level, for symmetric CSR (a row lists the
 // vertex's in-neighbours): a group of W lanes owns vertex v, lane w its source
 // word w.  The lane ORs word w of its neighbours' frontier rows (coalesced:
 // the group reads a W-word row at once), keeps the bits not yet visited, and
 // writes the visited / next words and the depths itself -- no atomics.  The
-// loads of KP vertices per thread are issued stage by stage.
+// loads of KP vertices per thread are issued stage by stage.  A row longer
+// than kLongRow that is still open after its first Q neighbours (a hub whose
+// sources are not all in reach at this level would otherwise be walked entry
+// by entry by its own lanes: 30 ms levels at R-MAT-22) is queued in LDS and
+// finished after the pass by one wave, its 64 lanes striding the row and
+// checking every 256 entries whether every unvisited bit has been found.
 template <int W>
 __device__ __forceinline__ void batch_pull(PullQueue &q, const int32_t *__restrict__ rowptr,
                                            const int32_t *__restrict__ col, int32_t m,
@@ -151,11 +244,15 @@ __device__ __forceinline__ void batch_pull(PullQueue &q, const int32_t *__restri
     constexpr int VPB = kBlock / W;  // vertices per block row
     constexpr int KP = 4;            // block rows per pass
     constexpr int Q = 4;             // neighbours loaded at once
-    if (threadIdx.x == 0) q.n = 0;
+    if (threadIdx.x == 0) {
+        q.n = 0;
+        q.nl = 0;
+    }
     __syncthreads();
     const int g = threadIdx.x / W, w = threadIdx.x % W;
     const int lane = dev::lane_id();
     const uint64_t gmask = W == 64 ? ~0ull : (((1ull << W) - 1) << ((lane / W) * W));
+    const int leader = (lane / W) * W;  // the group's first lane
     for (int64_t pbase = (int64_t)blockIdx.x * VPB * KP; pbase < m;
          pbase += (int64_t)gridDim.x * VPB * KP) {
         int32_t cv[KP], crs[KP], cd[KP], rs[KP], re[KP];
@@ -184,8 +281,21 @@ __device__ __forceinline__ void batch_pull(PullQueue &q, const int32_t *__restri
 #pragma unroll
             for (int q = 0; q < Q; ++q)
                 if (u[k][q] >= 0) acc[k] |= front[(int64_t)u[k][q] * W + w];
+        // open long rows go to the LDS list (the group's leader reserves)
+        bool deferred[KP];
 #pragma unroll
         for (int k = 0; k < KP; ++k) {
+            const bool open = (__ballot((acc[k] & uv[k]) != uv[k]) & gmask) != 0;
+            int slot = -1;
+            if (open && re[k] - rs[k] > kLongRow && w == 0)
+                slot = (int)atomicAdd(&q.nl, 1u);
+            slot = __shfl(slot, leader, 64);
+            deferred[k] = slot >= 0 && slot < kLongCap;
+            if (deferred[k] && w == 0) q.lv[slot] = cv[k];
+        }
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            if (deferred[k]) continue;
             for (int32_t e = rs[k] + Q; e < re[k] && (acc[k] & uv[k]) != uv[k]; e += Q) {
                 int32_t uu[Q];
 #pragma unroll
@@ -197,24 +307,12 @@ __device__ __forceinline__ void batch_pull(PullQueue &q, const int32_t *__restri
         }
 #pragma unroll
         for (int k = 0; k < KP; ++k) {
-            uint32_t nw = acc[k] & uv[k];
-            const int64_t i = (int64_t)cv[k] * W + w;
-            // with a snapshot the level's discoveries are one full bit row per
-            // vertex (the depths are read back from it at the end)
-            if (cv[k] < m) {
-                if (snap) snap[i] = nw;
-                next[i] = nw;  // every row: the buffer needs no clearing before a pull
-            }
-            if (nw) {
-                vis[i] = ~uv[k] | nw;
-                if (!snap)
-                    while (nw) {
-                        const int b = __ffs(nw) - 1;
-                        nw &= nw - 1;
-                        depth[(int64_t)cv[k] * (32 * W) + w * 32 + b] = level;
-                    }
-            }
-            const uint64_t bm = __ballot((acc[k] & uv[k]) != 0);
+            if (deferred[k]) continue;
+            const uint32_t nw = acc[k] & uv[k];
+            if (cv[k] < m)
+                pull_finish<W>((int64_t)cv[k] * W + w, nw, uv[k], vis, next, snap, depth, level,
+                               cv[k], w);
+            const uint64_t bm = __ballot(nw != 0);
             if (w == 0 && (bm & gmask)) {
                 crs[k] = rs[k];
                 cd[k] = re[k] - rs[k];
@@ -239,6 +337,15 @@ __device__ __forceinline__ void batch_pull(PullQueue &q, const int32_t *__restri
         __syncthreads();  // every thread has read q.n
         if (threadIdx.x == 0) q.n += total;
         __syncthreads();
+        // the pass's long rows, one wave each; their claims join the queue
+        const uint32_t nl = min(q.nl, (uint32_t)kLongCap);
+        if (nl > 0) {
+            if (q.n + nl > (uint32_t)kQueue) queue_flush(q, nxt, cnt_next);
+            pull_long<W>(q, nl, rowptr, col, front, vis, next, snap, depth, level);
+            __syncthreads();  // every wave is done with q.lv and q.n
+            if (threadIdx.x == 0) q.nl = 0;
+            __syncthreads();
+        }
     }
     if (q.n > 0) queue_flush(q, nxt, cnt_next);
 }
@@ -471,11 +578,15 @@ __global__ __launch_bounds__(kBlock) void k_batch_sssp_nf(
     constexpr int VPB = kBlock / W;
     constexpr int KP = 4;
     PullQueue &q = lds.pq;
-    if (threadIdx.x == 0) q.n = 0;
+    if (threadIdx.x == 0) {
+        q.n = 0;
+        q.nl = 0;
+    }
     __syncthreads();
     const int g = threadIdx.x / W, w = threadIdx.x % W;
     const int lane = dev::lane_id();
     const uint64_t gmask = W == 64 ? ~0ull : (((1ull << W) - 1) << ((lane / W) * W));
+    const int leader = (lane / W) * W;  // the group's first lane
     const int32_t round = *op.lvl;
     uint64_t left = 0;
     uint32_t min_left = ~0u;  // smallest distance left pending (bits)
@@ -558,18 +669,30 @@ __global__ __launch_bounds__(kBlock) void k_batch_sssp_nf(
 }
 
 // visited words at the start: bits of sources beyond nsrc count as visited
-// (the last word of a batch), and with a symmetric CSR a vertex without
-// neighbours counts as visited by every source (it is never reached)
+// (the last word of a batch), and with a symmetric CSR (comp given) a vertex
+// counts as visited by every source of another connected component -- it is
+// never reached from them, and a pulled level would otherwise scan its whole
+// row for their bits at every pulled level (a vertex without neighbours is
+// its own component).
 template <int W>
-__global__ void k_batch_vis_init(const int32_t *__restrict__ rowptr, int32_t m, int32_t nsrc,
-                                 bool iso_visited, uint32_t *vis) {
+__global__ void k_batch_vis_init(int32_t m, int32_t nsrc, const int32_t *__restrict__ comp,
+                                 const int32_t *__restrict__ srcs, uint32_t *vis) {
+    constexpr int B = 32 * W;
+    __shared__ int32_t s_comp[B];
+    for (int i = threadIdx.x; i < B; i += blockDim.x)
+        s_comp[i] = comp && i < nsrc ? comp[srcs[i]] : -1;
+    __syncthreads();
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)m * W;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t v = i / W;
         const int w = (int)(i % W);
         const int valid = min(max(nsrc - w * 32, 0), 32);
         uint32_t x = valid == 32 ? 0u : ~((1u << valid) - 1u);
-        if (iso_visited && rowptr[v] == rowptr[v + 1]) x = ~0u;
+        if (comp) {
+            const int32_t cv = comp[v];
+            for (int b = 0; b < valid; ++b)
+                if (s_comp[w * 32 + b] != cv) x |= 1u << b;
+        }
         vis[i] = x;
     }
 }
@@ -631,17 +754,20 @@ __global__ void k_batch_npull(uint64_t *npull, const int32_t *lvl) { npull[0] = 
 // (T * W = 256 words, one per thread): pairs discovered in a snapshotted
 // level take that level's number; the others are reached (visited bit set)
 // with the depth written where they were discovered (vertex-major dep
-// [m][32W]), or unreached (-1).  With iso set, isolated vertices were marked
-// visited up front, so there only the source itself counts as reached.  The
-// snapshots are read kSnapRun at a time (independent loads in flight).
+// [m][32W]), or unreached (-1).  With comp given, pairs of a vertex and a
+// source in different components were marked visited up front and are
+// unreached.  The snapshots are read kSnapRun at a time (independent loads
+// in flight).
 constexpr int kSnapRun = 16;
 template <int W>
 __global__ __launch_bounds__(256) void k_batch_depths(
-    const int32_t *dep, const uint32_t *vis, const int32_t *rowptr, const int32_t *srcs, bool iso,
+    const int32_t *dep, const uint32_t *vis, const int32_t *comp, const int32_t *srcs,
     int32_t m, int32_t nsrc, const uint32_t *snap, int64_t stride, const uint64_t *npull,
     const int32_t *level_of, int32_t *out) {
     constexpr int B = 32 * W, T = 256 / W;
     __shared__ int32_t d[T][B + 1];
+    __shared__ int32_t s_comp[B];
+    for (int i = threadIdx.x; i < B; i += 256) s_comp[i] = comp && i < nsrc ? comp[srcs[i]] : -1;
     const int64_t v0 = (int64_t)blockIdx.x * T;
     const int t = threadIdx.x / W, w = threadIdx.x % W;
     const int64_t v = v0 + t;
@@ -672,8 +798,7 @@ __global__ __launch_bounds__(256) void k_batch_depths(
         if (r == -2) {
             r = -1;
             if ((vis[vv * W + s / 32] >> (s % 32)) & 1u)
-                if (!iso || rowptr[vv + 1] > rowptr[vv] || srcs[s] == (int32_t)vv)
-                    r = dep[vv * B + s];
+                if (!comp || comp[vv] == s_comp[s]) r = dep[vv * B + s];
         }
         out[(int64_t)s * m + vv] = r;
     }
@@ -730,6 +855,8 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
         near_far ? (uint32_t *)scratch(c, "batch_pend", sizeof(uint32_t) * words) : nullptr;
     // pulling reads a row as in-edges: only for a symmetric CSR
     const bool sym = !SSSP && graph_is_symmetric(c, const_cast<Graph *>(g));
+    // components (built once per graph): pairs in different components start settled
+    const int32_t *comp = sym ? graph_components(c, const_cast<Graph *>(g)) : nullptr;
     // A pulled level reads W words per vertex and a pushed one W words per
     // expanded edge, so narrow batches (W <= 4) on graphs whose rows are all
     // short gain from pulling earlier: their threshold divisor is four times
@@ -743,7 +870,7 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
         sym ? (int32_t)std::min<int64_t>(div_opt * (W <= 4 && short_rows ? 4 : 1), 1 << 22)
             : 0;
     if (!SSSP) {
-        k_batch_vis_init<W><<<grid, kBlock, 0, c->stream>>>(g->rowptr, m, nsrc, sym, vis);
+        k_batch_vis_init<W><<<grid, kBlock, 0, c->stream>>>(m, nsrc, comp, srcs, vis);
     } else {
         fill_i32(c, dist, __builtin_bit_cast(int32_t, APSPIE_SSSP_INF), (int64_t)m * B);
     }
@@ -851,7 +978,7 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
         // that found the frontier empty; copy it to slot 0 for the reader
         k_batch_npull<<<1, 1, 0, c->stream>>>(sn.npull, lvl);
         k_batch_depths<W><<<(unsigned)ceil_div(m, 256 / W), 256, 0, c->stream>>>(
-            depth, vis, g->rowptr, srcs, sym, m, nsrc, sn.snap, sn.stride, sn.npull, sn.level_of,
+            depth, vis, comp, srcs, m, nsrc, sn.snap, sn.stride, sn.npull, sn.level_of,
             depth_out);
     }
     APSPIE_LAUNCH_CHECK();
@@ -861,11 +988,16 @@ template <bool SSSP>
 void batch_run(Ctx *c, const Graph *g, const int32_t *src, int32_t nsrc, int32_t *depth,
                float *dist) {
     // BFS on a graph with long rows (low diameter, skewed degrees) runs in
-    // groups of 32 sources: a wider batch pushes W words per edge and cannot
-    // stop a vertex at its first parent for every source at once (R-MAT-22,
-    // 128 sources: 0.61 ms per query in one batch, 0.47 in groups of 32)
+    // groups of batch_long_row_group sources (default 128): at 512 sources a
+    // pulled vertex waits for all 16 words before it stops scanning (R-MAT-22:
+    // 0.109 / 0.119 / 0.142 ms per query at 512 sources in groups of 128 /
+    // 128 in one group / 512 in one group, plain 0.206)
+    static_assert(kQueue % kBlock == 0, "queue flushes take whole block rows");
     const int32_t group =
-        !SSSP && graph_max_degree(c, const_cast<Graph *>(g)) > 64 ? 32 : kMaxBatch;
+        !SSSP && graph_max_degree(c, const_cast<Graph *>(g)) > 64
+            ? (int32_t)std::min<int64_t>(std::max<int64_t>(opt(c, "batch_long_row_group", 128), 1),
+                                         kMaxBatch)
+            : kMaxBatch;
     for (int32_t s0 = 0; s0 < nsrc; s0 += group) {
         const int32_t k = std::min(group, nsrc - s0);
         int32_t *d = depth ? depth + (int64_t)s0 * g->m : nullptr;

@@ -61,6 +61,7 @@ struct Graph {
     // derived, built lazily and cached with the graph
     int32_t *lowcnt = nullptr;  // per row: number of cols < row (TC)
     uint32_t *iso_bits = nullptr;  // bit per vertex without entries (BFS visited start)
+    int32_t *comp = nullptr;       // connected-component label per vertex (symmetric graphs)
     int32_t max_deg = -1;          // longest row, once known
     int w_range = -1;              // 1 once w_lo / w_hi are known
     float w_lo = 0.0f, w_hi = 0.0f; // smallest and largest value
@@ -191,6 +192,10 @@ bool graph_is_symmetric(Ctx *c, Graph *g);
 // for w(u, v))
 bool graph_is_symmetric_weighted(Ctx *c, Graph *g);
 const uint32_t *graph_iso_bits(Ctx *c, Graph *g);
+// Connected components of a symmetric graph, built once and kept with it:
+// label[v] = the smallest vertex id of v's component.
+const int32_t *graph_components(Ctx *c, Graph *g);
+void components_into(Ctx *c, const Graph *g, int32_t *label_dev);
 // What is known without computing anything: the symmetry flag (-1 unknown,
 // 0, 1) and the isolated-vertex bits (null until built)
 int graph_symmetry_known(Graph *g);

@@ -29,6 +29,8 @@ static const char *kOptionNames[] = {
     "batch_sssp_delta",  // batched SSSP near-far threshold step (0 = Bellman-Ford frontier,
                          // -1 = chosen per graph from its weights and mean row length)
     "batch_snapshot_mb", // HBM for the bit rows of pulled batched BFS levels (0 = none)
+    "batch_long_row_group", // batched BFS on graphs with rows longer than 64: sources per
+                            // pass (1..512)
     "profiling",         // 1 = time the dominant kernel with hipEvents
     nullptr};
 
@@ -49,6 +51,7 @@ static bool option_default(const std::string &n, int64_t *v) {
         {"batch_pull_div", 4},
         {"batch_sssp_delta", -1},
         {"batch_snapshot_mb", 24576},
+        {"batch_long_row_group", 128},
         {"profiling", 0}};
     for (const auto &d : kDefaults)
         if (n == d.first) {
@@ -277,6 +280,7 @@ int apspie_ctx_set_option(apspie_ctx *c, const char *name, int64_t v) {
     if (n == "bfs_levels_per_sync" && (v < 1 || v > 4096)) return APSPIE_ERR_INVALID;
     if (n == "batch_pull_div" && (v < 0 || v > (1 << 20))) return APSPIE_ERR_INVALID;
     if (n == "batch_snapshot_mb" && (v < 0 || v > (1 << 22))) return APSPIE_ERR_INVALID;
+    if (n == "batch_long_row_group" && (v < 1 || v > 512)) return APSPIE_ERR_INVALID;
     if (n == "batch_sssp_delta" && (v < -1 || v > (1 << 30))) return APSPIE_ERR_INVALID;
     if (n == "sssp_delta" && (v < -1 || v > (1 << 30))) return APSPIE_ERR_INVALID;
     if (n == "sssp_pull_div" && (v < 0 || v > (1 << 20))) return APSPIE_ERR_INVALID;
@@ -470,6 +474,7 @@ int apspie_graph_destroy(apspie_graph *g) {
     }
     if (g->lowcnt) (void)hipFree(g->lowcnt);
     if (g->iso_bits) (void)hipFree(g->iso_bits);
+    if (g->comp) (void)hipFree(g->comp);
     if (g->heavy_mem) (void)hipFree(g->heavy_mem);
     delete g;
     return APSPIE_OK;
@@ -701,6 +706,24 @@ int apspie_spmspv(apspie_ctx *c, const apspie_graph *g, int semiring, int64_t nx
             copy_from_device(c, y_val, yv, sizeof(float) * k, APSPIE_MEM_HOST);
             sync(c);
         }
+    });
+}
+
+int apspie_components(apspie_ctx *c, const apspie_graph *g, int32_t *label, int mem) {
+    if (!c || !label) return APSPIE_ERR_INVALID;
+    return guard(c, [&] {
+        check_graph(c, g);
+        APSPIE_CHECK(g->m == g->n, APSPIE_ERR_INVALID, "components need a square matrix");
+        APSPIE_CHECK(mem == APSPIE_MEM_HOST || mem == APSPIE_MEM_DEVICE, APSPIE_ERR_INVALID,
+                     "mem must be APSPIE_MEM_HOST or APSPIE_MEM_DEVICE");
+        set_device(c);
+        Graph *gg = const_cast<Graph *>(static_cast<const Graph *>(g));
+        APSPIE_CHECK(graph_is_symmetric(c, gg), APSPIE_ERR_UNSUPPORTED,
+                     "components need a symmetric CSR (an undirected graph)");
+        const int32_t *lab = nullptr;
+        timed(c, [&] { lab = graph_components(c, gg); });
+        copy_from_device(c, label, lab, sizeof(int32_t) * g->m, mem);
+        sync(c);
     });
 }
 

@@ -71,6 +71,12 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) {
     return x;
 }
 
+__device__ __forceinline__ uint32_t wave_or_u32(uint32_t x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x |= __shfl_xor(x, d, 64);
+    return x;
+}
+
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {

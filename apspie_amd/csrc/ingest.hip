@@ -380,6 +380,123 @@ const uint32_t *graph_iso_bits(Ctx *c, Graph *g) {
     return bits;
 }
 
+// ------------------------------------------------- connected components
+// Lock-free union-find over the edges of a symmetric CSR: every root hooks
+// under a smaller root with one compare-and-swap on its parent pointer, so a
+// tree's root is always the smallest id in it and the final label of a
+// vertex is the smallest vertex id of its component.  Parents only ever move
+// to smaller ids of the same set, so any value a parent pointer ever held is
+// an ancestor: a find may halve its path, and a stale read only costs a
+// retry.  Every write to the parent array while unions run is an atomic
+// read-modify-write (the hook, the halving swap, the final label exchange):
+// the workgroups of different XCDs write neighbouring words of one line.
+__device__ __forceinline__ int32_t cc_parent(const int32_t *p, int32_t x) {
+    return __hip_atomic_load(p + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int32_t cc_find(int32_t *p, int32_t x) {
+    int32_t px = cc_parent(p, x);
+    while (px != x) {
+        const int32_t gp = cc_parent(p, px);
+        if (gp != px) {  // halve: x's parent becomes its grandparent, if still px
+            int32_t expected = px;
+            __hip_atomic_compare_exchange_strong(p + x, &expected, gp, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        x = gp;
+        px = cc_parent(p, x);
+    }
+    return x;
+}
+
+__device__ __forceinline__ void cc_unite(int32_t *p, int32_t u, int32_t v) {
+    u = cc_find(p, u);
+    v = cc_find(p, v);
+    while (u != v) {
+        if (u < v) {
+            const int32_t t = u;
+            u = v;
+            v = t;
+        }
+        // hook the larger root u under v; a failed swap means u got a parent
+        int32_t expected = u;
+        if (__hip_atomic_compare_exchange_strong(p + u, &expected, v, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            return;
+        u = cc_find(p, expected);
+        v = cc_find(p, v);
+    }
+}
+
+__global__ void k_cc_init(int32_t m, int32_t *p) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = (int32_t)i;
+}
+
+constexpr int32_t kCcShort = 32;
+
+// rows of at most kCcShort entries: one thread per row; each undirected edge
+// is united from its larger end only
+__global__ void k_cc_short(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                           int32_t m, int32_t *p) {
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < m;
+         u += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t rs = rowptr[u], re = rowptr[u + 1];
+        if (re - rs > kCcShort) continue;
+        for (int32_t e = rs; e < re; ++e) {
+            const int32_t v = col[e];
+            if (v < u) cc_unite(p, (int32_t)u, v);
+        }
+    }
+}
+
+// longer rows: one wave per row, lanes striding it
+__global__ void k_cc_long(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                          int32_t m, int32_t *p) {
+    const int lane = dev::lane_id();
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / 64;
+    const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / 64;
+    for (int64_t u = wave; u < m; u += nwaves) {
+        const int32_t rs = rowptr[u], re = rowptr[u + 1];
+        if (re - rs <= kCcShort) continue;
+        for (int32_t e = rs + lane; e < re; e += 64) {
+            const int32_t v = col[e];
+            if (v < u) cc_unite(p, (int32_t)u, v);
+        }
+    }
+}
+
+__global__ void k_cc_label(int32_t m, int32_t *p) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t r = cc_find(p, (int32_t)i);
+        __hip_atomic_exchange(p + i, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+void components_into(Ctx *c, const Graph *g, int32_t *label) {
+    if (g->m == 0) return;
+    const int grid = fill_grid(c, kBlock, 8);
+    k_cc_init<<<grid, kBlock, 0, c->stream>>>(g->m, label);
+    k_cc_short<<<grid, kBlock, 0, c->stream>>>(g->rowptr, g->col, g->m, label);
+    k_cc_long<<<grid, kBlock, 0, c->stream>>>(g->rowptr, g->col, g->m, label);
+    k_cc_label<<<grid, kBlock, 0, c->stream>>>(g->m, label);
+    APSPIE_LAUNCH_CHECK();
+}
+
+const int32_t *graph_components(Ctx *c, Graph *g) {
+    std::lock_guard<std::mutex> lock(g->derived_mu);
+    if (g->comp) return g->comp;
+    int32_t *label = nullptr;
+    // owned by the graph (freed with it), like its other derived arrays
+    APSPIE_HIP(hipMalloc(&label, sizeof(int32_t) * std::max<int64_t>(g->m, 1)));
+    components_into(c, g, label);
+    sync(c);
+    g->comp = label;
+    return label;
+}
+
 int graph_symmetry_known(Graph *g) {
     std::lock_guard<std::mutex> lock(g->derived_mu);
     return g->symmetric;

@@ -83,6 +83,8 @@ struct Request {
     int32_t m = 0;
     Clock::time_point t_submit, t_done;
     void *buf = nullptr;    // pinned result (BFS depths / SSSP distances)
+    void *out = nullptr;    // caller-owned result memory (submit_into), else null
+    int out_mem = APSPIE_MEM_HOST;
     uint64_t triangles = 0;  // TC result
     int status = APSPIE_OK;
     bool done = false;
@@ -92,8 +94,17 @@ struct Request {
 // wall time workers spent on them (launch to results in host memory).
 struct Busy {
     int64_t batches = 0, queries = 0;
-    double busy_ms = 0;
+    double busy_ms = 0, traverse_ms = 0, copy_ms = 0, alloc_ms = 0;
 };
+
+// wall time of the phases of one batch
+struct Phases {
+    double traverse_ms = 0, copy_ms = 0, alloc_ms = 0;
+};
+
+double ms_since(Clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+}
 
 struct Resident {
     apspie_graph *g = nullptr;
@@ -140,7 +151,8 @@ int64_t result_bytes(int kind, int32_t m) {
 // The batch's graph and size come from its requests (fixed at submit under
 // the server lock), so a batch reads nothing of the resident pool while
 // graphs are being admitted.
-int run_batch(apspie_server *s, apspie_ctx *ctx, std::vector<std::shared_ptr<Request>> &batch) {
+int run_batch(apspie_server *s, apspie_ctx *ctx, std::vector<std::shared_ptr<Request>> &batch,
+              Phases &ph) {
     struct {
         const apspie_graph *g;
         int32_t m;
@@ -148,19 +160,31 @@ int run_batch(apspie_server *s, apspie_ctx *ctx, std::vector<std::shared_ptr<Req
     const int kind = batch[0]->kind;
     const size_t n = batch.size();
     const size_t row = 4 * (size_t)R.m;
+    auto t0 = Clock::now();
     if (kind == APSPIE_Q_TC) {
         uint64_t total = 0;
         const int st = apspie_tc(ctx, R.g, &total, nullptr, APSPIE_MEM_HOST);
         for (auto &r : batch) r->triangles = total;
+        ph.traverse_ms += ms_since(t0);
         return st;
     }
     if (n == 1) {
         auto &r = batch[0];
-        if (!r->buf) r->buf = s->pinned.get(row);
-        if (!r->buf) return APSPIE_ERR_OOM;
-        return kind == APSPIE_Q_BFS
-                   ? apspie_bfs(ctx, R.g, r->source, (int32_t *)r->buf, APSPIE_MEM_HOST)
-                   : apspie_sssp(ctx, R.g, r->source, (float *)r->buf, APSPIE_MEM_HOST);
+        void *dst = r->out;
+        int mem = r->out_mem;
+        if (!dst) {
+            if (!r->buf) r->buf = s->pinned.get(row);
+            ph.alloc_ms += ms_since(t0);
+            if (!r->buf) return APSPIE_ERR_OOM;
+            dst = r->buf;
+            mem = APSPIE_MEM_HOST;
+        }
+        t0 = Clock::now();
+        const int st = kind == APSPIE_Q_BFS
+                           ? apspie_bfs(ctx, R.g, r->source, (int32_t *)dst, mem)
+                           : apspie_sssp(ctx, R.g, r->source, (float *)dst, mem);
+        ph.traverse_ms += ms_since(t0);  // a host result's copy included
+        return st;
     }
     std::vector<int32_t> src(n);
     for (size_t i = 0; i < n; ++i) src[i] = batch[i]->source;
@@ -178,19 +202,33 @@ int run_batch(apspie_server *s, apspie_ctx *ctx, std::vector<std::shared_ptr<Req
                                     APSPIE_MEM_DEVICE)
                  : apspie_sssp_batch(ctx, R.g, src.data(), (int32_t)n, (float *)dout,
                                      APSPIE_MEM_DEVICE);
+    ph.traverse_ms += ms_since(t0);
     // each row straight to its query's pinned buffer: the copies queue on the
     // worker's stream and one wait covers them all
+    // (or to the caller's own memory, device or host)
+    t0 = Clock::now();
     for (size_t i = 0; i < n && st == APSPIE_OK; ++i) {
-        if (!batch[i]->buf) batch[i]->buf = s->pinned.get(row);
+        if (batch[i]->out || batch[i]->buf) continue;
+        batch[i]->buf = s->pinned.get(row);
         if (!batch[i]->buf) st = APSPIE_ERR_OOM;
     }
+    ph.alloc_ms += ms_since(t0);
+    t0 = Clock::now();
     if (st == APSPIE_OK)
         st = apspie::guard(ctx, [&] {
-            for (size_t i = 0; i < n; ++i)
-                apspie::copy_from_device(ctx, batch[i]->buf, (uint8_t *)dout + row * i, row,
-                                         APSPIE_MEM_HOST);
+            for (size_t i = 0; i < n; ++i) {
+                const auto &r = batch[i];
+                const void *rowp = (uint8_t *)dout + row * i;
+                if (r->out && r->out_mem == APSPIE_MEM_DEVICE)
+                    APSPIE_HIP(hipMemcpyAsync(r->out, rowp, row, hipMemcpyDeviceToDevice,
+                                              ctx->stream));
+                else
+                    apspie::copy_from_device(ctx, r->out ? r->out : r->buf, rowp, row,
+                                             APSPIE_MEM_HOST);
+            }
             apspie::sync(ctx);
         });
+    ph.copy_ms += ms_since(t0);
     return st;
 }
 
@@ -199,10 +237,11 @@ int run_batch(apspie_server *s, apspie_ctx *ctx, std::vector<std::shared_ptr<Req
 // tries again; a batch that still does not fit is split in halves.
 void execute(apspie_server *s, apspie_ctx *ctx, std::vector<std::shared_ptr<Request>> &batch) {
     const auto t0 = Clock::now();
-    int st = run_batch(s, ctx, batch);
+    Phases ph;
+    int st = run_batch(s, ctx, batch, ph);
     if (st == APSPIE_ERR_BUDGET || st == APSPIE_ERR_OOM) {
         apspie_ctx_trim(ctx);
-        st = run_batch(s, ctx, batch);
+        st = run_batch(s, ctx, batch, ph);
         if ((st == APSPIE_ERR_BUDGET || st == APSPIE_ERR_OOM) && batch.size() > 1) {
             const size_t n = batch.size();
             std::vector<std::shared_ptr<Request>> lo(batch.begin(), batch.begin() + n / 2),
@@ -223,6 +262,9 @@ void execute(apspie_server *s, apspie_ctx *ctx, std::vector<std::shared_ptr<Requ
     b.batches += 1;
     b.queries += (int64_t)batch.size();
     b.busy_ms += std::chrono::duration<double, std::milli>(now - t0).count();
+    b.traverse_ms += ph.traverse_ms;
+    b.copy_ms += ph.copy_ms;
+    b.alloc_ms += ph.alloc_ms;
     s->cv_done.notify_all();
 }
 
@@ -405,9 +447,10 @@ const char *apspie_server_last_error(apspie_server *s) {
     return s ? apspie_last_error(s->admin) : "no server";
 }
 
-int apspie_server_submit(apspie_server *s, int32_t graph_id, int kind, int32_t source,
-                         int64_t *ticket) {
+static int submit(apspie_server *s, int32_t graph_id, int kind, int32_t source, void *out,
+                  int out_mem, int64_t *ticket) {
     if (!s || !ticket || kind < APSPIE_Q_BFS || kind > APSPIE_Q_TC) return APSPIE_ERR_INVALID;
+    if (out_mem != APSPIE_MEM_HOST && out_mem != APSPIE_MEM_DEVICE) return APSPIE_ERR_INVALID;
     auto r = std::make_shared<Request>();
     {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -423,6 +466,8 @@ int apspie_server_submit(apspie_server *s, int32_t graph_id, int kind, int32_t s
         r->g = R.g;
         r->m = R.m;
         r->bytes = result_bytes(kind, R.m);
+        r->out = kind == APSPIE_Q_TC ? nullptr : out;  // a count is returned by the wait
+        r->out_mem = out_mem;
         r->t_submit = Clock::now();
         s->queue.push_back(r);
         s->live[r->ticket] = r;
@@ -430,6 +475,17 @@ int apspie_server_submit(apspie_server *s, int32_t graph_id, int kind, int32_t s
     }
     s->cv_work.notify_one();
     return APSPIE_OK;
+}
+
+int apspie_server_submit(apspie_server *s, int32_t graph_id, int kind, int32_t source,
+                         int64_t *ticket) {
+    return submit(s, graph_id, kind, source, nullptr, APSPIE_MEM_HOST, ticket);
+}
+
+int apspie_server_submit_into(apspie_server *s, int32_t graph_id, int kind, int32_t source,
+                              void *out, int mem, int64_t *ticket) {
+    if (kind != APSPIE_Q_TC && !out) return APSPIE_ERR_INVALID;
+    return submit(s, graph_id, kind, source, out, mem, ticket);
 }
 
 int apspie_server_wait(apspie_server *s, int64_t ticket, void *out, int64_t out_bytes,
@@ -449,6 +505,7 @@ int apspie_server_wait(apspie_server *s, int64_t ticket, void *out, int64_t out_
     if (st == APSPIE_OK && out) {
         if (out_bytes < need) st = APSPIE_ERR_INVALID;
         else if (r->kind == APSPIE_Q_TC) std::memcpy(out, &r->triangles, sizeof r->triangles);
+        else if (r->out) st = APSPIE_ERR_INVALID;  // the result is in the submitter's memory
         else std::memcpy(out, r->buf, (size_t)need);
     }
     s->pinned.put(r->buf);
@@ -476,7 +533,7 @@ int apspie_server_wait_view(apspie_server *s, int64_t ticket, const void **data,
         return r->status;
     }
     *bytes = r->bytes;
-    *data = r->kind == APSPIE_Q_TC ? (const void *)&r->triangles : r->buf;
+    *data = r->kind == APSPIE_Q_TC ? (const void *)&r->triangles : r->out ? r->out : r->buf;
     s->viewed[ticket] = r;
     return APSPIE_OK;
 }
@@ -505,6 +562,18 @@ int apspie_server_busy(apspie_server *s, int32_t graph_id, int kind, int64_t *ba
     if (batches) *batches = b.batches;
     if (queries) *queries = b.queries;
     if (busy_ms) *busy_ms = b.busy_ms;
+    if (reset) b = Busy{};
+    return APSPIE_OK;
+}
+
+int apspie_server_busy_stats(apspie_server *s, int32_t graph_id, int kind,
+                             apspie_server_stats *out, int reset) {
+    if (!s || !out || kind < APSPIE_Q_BFS || kind > APSPIE_Q_TC) return APSPIE_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (graph_id < 0 || graph_id >= (int32_t)s->graphs.size()) return APSPIE_ERR_INVALID;
+    Busy &b = s->graphs[graph_id].busy[kind];
+    *out = apspie_server_stats{b.batches, b.queries, b.busy_ms, b.traverse_ms, b.copy_ms,
+                               b.alloc_ms};
     if (reset) b = Busy{};
     return APSPIE_OK;
 }

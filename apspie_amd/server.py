@@ -7,7 +7,8 @@ import weakref
 
 import numpy as np
 
-from . import HOST, Q_BFS, Q_SSSP, Q_TC, ApspieError, _Coo, _Csr, _ServerCfg, _check, _p, lib
+from . import (HOST, Q_BFS, Q_SSSP, Q_TC, ApspieError, _Coo, _Csr, _ServerCfg, _ServerStats,
+               _check, _p, lib)
 
 __all__ = ["Server"]
 
@@ -81,6 +82,21 @@ class Server:
         _check(self._L.apspie_server_submit(self.h, graph_id, kind, int(source), C.byref(t)))
         return t.value
 
+    def submit_into(self, graph_id, kind, source, out_ptr, mem):
+        """A BFS / SSSP query whose result the server writes to out_ptr (m
+        entries of caller memory, host or device per `mem`); finish it with
+        wait_done.  A TC query ignores out_ptr (its count comes from wait)."""
+        t = C.c_int64()
+        _check(self._L.apspie_server_submit_into(self.h, graph_id, kind, int(source),
+                                                 out_ptr, int(mem), C.byref(t)))
+        return t.value
+
+    def wait_done(self, ticket):
+        """Wait for a submit_into query: latency_ms (the result is in the caller's memory)."""
+        lat = C.c_double()
+        _check(self._L.apspie_server_wait(self.h, ticket, None, 0, C.byref(lat)))
+        return lat.value
+
     def wait(self, ticket, kind, m, copy=True):
         """Returns (result, latency_ms): int32[m] depths, float32[m] distances or an int.
         With copy=False a depth / distance result is a read-only view of the
@@ -118,6 +134,13 @@ class Server:
         _check(self._L.apspie_server_busy(self.h, graph_id, kind, C.byref(b), C.byref(q),
                                           C.byref(ms), int(reset)))
         return b.value, q.value, ms.value
+
+    def busy_stats(self, graph_id, kind, reset=False):
+        """The busy counters split by phase: dict of batches, queries, busy_ms,
+        traverse_ms, copy_ms, alloc_ms."""
+        st = _ServerStats()
+        _check(self._L.apspie_server_busy_stats(self.h, graph_id, kind, C.byref(st), int(reset)))
+        return {k: getattr(st, k) for k, _ in _ServerStats._fields_}
 
     def drain(self):
         _check(self._L.apspie_server_drain(self.h))

@@ -224,6 +224,13 @@ int apspie_tc(apspie_ctx *ctx, const apspie_graph *g, uint64_t *total, int32_t *
 int apspie_spmspv(apspie_ctx *ctx, const apspie_graph *g, int semiring, int64_t nx,
                   const int32_t *x_ind, const float *x_val, const int32_t *mask,
                   int64_t *ny, int32_t *y_ind, float *y_val, int mem, int path);
+/* Connected components of an undirected graph (a symmetric CSR; otherwise
+ * APSPIE_ERR_UNSUPPORTED): label[v] = the smallest vertex id in v's
+ * component (m entries in `mem`).  Built once per graph by lock-free
+ * union-find on the device and kept with the graph; the batched BFS uses it
+ * to mark pairs of a vertex and a source in different components as
+ * settled from the start.  No reference counterpart. */
+int apspie_components(apspie_ctx *ctx, const apspie_graph *g, int32_t *label, int mem);
 /* Dense mXv (mXv.cuh:110-247): y = A^T x over (+,x) with x (m entries) and
  * y (n entries) dense.  The entries of x above 0.5 form the frontier (the
  * reference's bitify, mXv.cuh:50-55) with their values as x_v; y is zeroed,
@@ -333,6 +340,15 @@ const char *apspie_server_last_error(apspie_server *s);
 /* Enqueue a query; returns a ticket.  source ignored for TC. */
 int apspie_server_submit(apspie_server *s, int32_t graph_id, int kind, int32_t source,
                          int64_t *ticket);
+/* Enqueue a BFS / SSSP query whose result the server writes into the
+ * caller's memory: `out` holds m int32 depths / m float distances, in host
+ * or device memory (`mem`), and must stay valid until the wait returns --
+ * the reference's entry points take their result arrays from the caller the
+ * same way (d_bfsResult, bfs.cuh:32-33).  A TC query ignores `out`; its count
+ * comes back from the wait.  The wait then only reports status and latency
+ * (pass out = NULL there). */
+int apspie_server_submit_into(apspie_server *s, int32_t graph_id, int kind, int32_t source,
+                              void *out, int mem, int64_t *ticket);
 /* Wait for a ticket; copies the result (m int32 depths / m float distances /
  * 1 uint64 count) into `out` (host) and reports arrival->completion latency. */
 int apspie_server_wait(apspie_server *s, int64_t ticket, void *out, int64_t out_bytes,
@@ -351,6 +367,16 @@ int apspie_server_drain(apspie_server *s);
  * the counters after reading. */
 int apspie_server_busy(apspie_server *s, int32_t graph_id, int kind, int64_t *batches,
                        int64_t *queries, double *busy_ms, int reset);
+/* The same counters split by phase: the traversal / count calls, the copies
+ * of result rows to host memory, and obtaining pinned result buffers (wall
+ * time of each, summed over batches); busy_ms also holds the rest (retries
+ * after a budget failure, bookkeeping).  No reference counterpart. */
+typedef struct {
+    int64_t batches, queries;
+    double busy_ms, traverse_ms, copy_ms, alloc_ms;
+} apspie_server_stats;
+int apspie_server_busy_stats(apspie_server *s, int32_t graph_id, int kind,
+                             apspie_server_stats *out, int reset);
 
 #ifdef __cplusplus
 }

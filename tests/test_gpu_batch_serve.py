@@ -248,6 +248,61 @@ def test_server_mixed_stream_equals_alone():
         srv.drain()
 
 
+def test_server_results_into_caller_memory():
+    """submit_into: results written straight into the caller's arrays, device
+    and host, batched and single; each equals the oracle, and the same stream
+    served with the pinned-buffer results gives identical bytes.  Also the
+    one-query-per-launch form (max_batch 1) and a TC query among them."""
+    graphs = [T.rmat(12), T.lattice(64)]
+    rng = np.random.default_rng(3)
+    stream = []
+    for _ in range(150):
+        gi = int(rng.integers(0, len(graphs)))
+        r = rng.random()
+        kind = A.Q_BFS if r < 0.55 else (A.Q_SSSP if r < 0.95 else A.Q_TC)
+        stream.append((gi, kind, int(rng.choice(np.nonzero(graphs[gi].deg)[0]))))
+    with A.Context(0) as ctx:
+        for max_batch in (32, 1):
+            with S.Server(slots=3, max_batch=max_batch, hbm_budget=1 << 30) as srv:
+                ids = [srv.add_graph(g.rowptr32, g.col, g.w) for g in graphs]
+                bufs, tickets = [], []
+                for i, (gi, kind, src) in enumerate(stream):
+                    m = graphs[gi].m
+                    if kind == A.Q_TC:
+                        bufs.append(None)
+                        tickets.append(srv.submit_into(ids[gi], kind, src, None, A.DEVICE))
+                    elif i % 3 == 0:  # host memory of the caller's own
+                        h = np.full(m, 99, np.int32 if kind == A.Q_BFS else np.float32)
+                        bufs.append(h)
+                        tickets.append(srv.submit_into(ids[gi], kind, src, h.ctypes.data,
+                                                       A.HOST))
+                    else:
+                        d = ctx.dev_alloc(4 * m)
+                        bufs.append(d)
+                        tickets.append(srv.submit_into(ids[gi], kind, src, d.ptr, A.DEVICE))
+                for i, (gi, kind, src) in enumerate(stream):
+                    g = graphs[gi]
+                    if kind == A.Q_TC:
+                        got, _ = srv.wait(tickets[i], kind, g.m)
+                        assert got == O.tc(g.m, g.rowptr, g.col)[0]
+                        continue
+                    assert srv.wait_done(tickets[i]) > 0
+                    b = bufs[i]
+                    dt = np.int32 if kind == A.Q_BFS else np.float32
+                    got = b if isinstance(b, np.ndarray) else b.to_host(dt, g.m)
+                    want = (O.bfs(g.m, g.rowptr, g.col, src)[0] if kind == A.Q_BFS
+                            else oracle_sssp(g, src))
+                    assert np.array_equal(got, want), (max_batch, i)
+                    # the pinned-buffer form of the same query
+                    res, _ = srv.wait(srv.submit(ids[gi], kind, src), kind, g.m)
+                    assert res.tobytes() == got.tobytes()
+                    if not isinstance(b, np.ndarray):
+                        b.free()
+                # a BFS / SSSP query needs an output
+                with pytest.raises(A.ApspieError):
+                    srv.submit_into(ids[0], A.Q_BFS, 0, None, A.DEVICE)
+
+
 def test_server_tight_budget_sizes_batches():
     """A worker budget that holds only part of what a batch of 128 would
     need: batches are sized to fit (or split after a budget failure), and
@@ -302,3 +357,28 @@ def test_server_budget_and_bad_requests():
             srv.submit(gi, A.Q_BFS, 99)
         with pytest.raises(A.ApspieError):
             srv.submit(gi + 5, A.Q_BFS, 0)
+
+
+@pytest.mark.parametrize("group", [32, 128])
+def test_bfs_batch_sources_in_small_components(ctx, group):
+    """Sources spread over a giant component and many small ones (and
+    isolated vertices): pairs in different components start settled and come
+    out unreached; every row equals the plain BFS.  Also with the long-row
+    graph's batch grouping widened to 128 sources."""
+    big = T.rmat(12)
+    small = T.random_graph(3000, 0.0004, 3)  # many small components
+    m = big.m + small.m
+    rows = np.concatenate([np.repeat(np.arange(big.m), np.diff(big.rowptr)),
+                           big.m + np.repeat(np.arange(small.m), np.diff(small.rowptr))])
+    cols = np.concatenate([big.col, big.m + small.col])
+    G = ctx.from_coo(m, rows.astype(np.int32), cols.astype(np.int32), symmetrize=True)
+    rng = np.random.default_rng(9)
+    srcs = [int(x) for x in rng.choice(m, 100, replace=False)]
+    ctx.set_option("batch_long_row_group", group)
+    try:
+        got = ctx.bfs_batch(G, srcs)
+        for i, s in enumerate(srcs):
+            assert np.array_equal(got[i], ctx.bfs(G, s)), (group, i, s)
+    finally:
+        ctx.set_option("batch_long_row_group", 128)
+        G.close()

@@ -446,3 +446,34 @@ def test_hbm_budget_is_enforced(ctx):
         G = c2.upload(g.rowptr32, g.col)
         assert c2.hbm_used() >= G.hbm_bytes > 0
         G.close()
+
+
+@pytest.mark.parametrize("case", ["rmat", "random", "lattice", "fixtures"])
+def test_components_vs_oracle(ctx, case):
+    """Connected-component labels (smallest vertex id per component) equal
+    the oracle's union-find on graphs with many small components, isolated
+    vertices, long paths and the reference's fixtures (the unions race
+    across the whole device: several seeds and sizes)."""
+    if case == "fixtures":
+        gs = [T.fixture(n) for n in T.FIXTURES]
+    elif case == "random":
+        gs = [T.random_graph(n, p, s) for n, p in ((5000, 0.0003), (20000, 0.00008))
+              for s in range(1, 6)]
+    elif case == "rmat":
+        gs = [T.rmat(12), T.rmat(16)]
+    else:
+        gs = [T.lattice(64), T.lattice(256)]
+    for g in gs:
+        G = ctx.upload(g.rowptr32, g.col)
+        got = ctx.components(G)
+        assert np.array_equal(got, O.components(g.m, g.rowptr, g.col)), g
+        assert np.array_equal(ctx.components(G), got)  # cached with the graph
+        G.close()
+
+
+def test_components_need_symmetric(ctx):
+    rp = np.array([0, 1, 1], np.int32)
+    G = ctx.upload(rp, np.array([1], np.int32))
+    with pytest.raises(A.ApspieError):
+        ctx.components(G)
+    G.close()
