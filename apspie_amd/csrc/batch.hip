@@ -511,6 +511,119 @@ struct SsspBatchNearOp {
     }
 };
 
+// Pulled form of one batched SSSP round, for a CSR whose rows are all short
+// and whose weights are symmetric (w(u, v) sits in v's row too): a group of W
+// lanes owns vertex v, lane w its source word w.  The lane walks v's row; a
+// neighbour u whose frontier word w has bits set relaxes each of those
+// sources into v's own distances, which the lane reads and writes with
+// plain stores -- nothing else writes them this round.  The improved bits
+// go to next[v] (below the threshold, or always without one) or pend[v],
+// written by the lane alone; a vertex with a near bit joins the next
+// frontier through the block's LDS queue.  Every row of next is written, so
+// the buffer needs no clearing before a pulled round.  A neighbour's
+// distance read while its owner lowers it is either value, both upper
+// bounds: the relaxations are those of the push, and so is the fixed point.
+template <int W>
+__device__ __forceinline__ void batch_sssp_pull(PullQueue &q, const int32_t *__restrict__ rowptr,
+                                                const int32_t *__restrict__ col, int32_t m,
+                                                const SsspBatchNearOp<W> &op, int32_t tbits,
+                                                FrontierBufs nxt, uint64_t *cnt_next) {
+    constexpr int B = 32 * W;
+    constexpr int VPB = kBlock / W;
+    constexpr int KP = 2;  // vertices per group and pass
+    constexpr int Q = 4;   // row entries loaded at once
+    if (threadIdx.x == 0) q.n = 0;
+    __syncthreads();
+    const int g = threadIdx.x / W, w = threadIdx.x % W;
+    const int lane = dev::lane_id();
+    const uint64_t gmask = W == 64 ? ~0ull : (((1ull << W) - 1) << ((lane / W) * W));
+    for (int64_t pbase = (int64_t)blockIdx.x * VPB * KP; pbase < m;
+         pbase += (int64_t)gridDim.x * VPB * KP) {
+        int32_t cv[KP], crs[KP], cd[KP];
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            const int64_t v = pbase + (int64_t)k * VPB + g;
+            cv[k] = (int32_t)v;
+            crs[k] = 0;
+            cd[k] = 0;
+            uint32_t near = 0, far = 0;
+            int32_t rs = 0, re = 0;
+            if (v < m) {
+                rs = rowptr[v];
+                re = rowptr[v + 1];
+                int32_t *dv_row = op.dist + v * B + w * 32;
+                for (int32_t e0 = rs; e0 < re; e0 += Q) {
+                    int32_t u[Q];
+                    uint32_t f[Q];
+#pragma unroll
+                    for (int t = 0; t < Q; ++t) u[t] = e0 + t < re ? col[e0 + t] : -1;
+#pragma unroll
+                    for (int t = 0; t < Q; ++t) f[t] = u[t] >= 0 ? op.front[(int64_t)u[t] * W + w] : 0u;
+#pragma unroll
+                    for (int t = 0; t < Q; ++t) {
+                        if (!f[t]) continue;
+                        const float we = op.wt[e0 + t];
+                        const int32_t *du_row = op.dist + (int64_t)u[t] * B + w * 32;
+                        uint32_t x = f[t];
+                        while (x) {  // four sources at a time, their loads in flight
+                            int b[4];
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                b[j] = x ? __ffs(x) - 1 : -1;
+                                x &= x - 1;
+                            }
+                            int32_t du[4], dv[4];
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                du[j] = b[j] >= 0 ? du_row[b[j]] : 0;
+                                dv[j] = b[j] >= 0 ? dv_row[b[j]] : 0;
+                            }
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                if (b[j] < 0) continue;
+                                const int32_t nd = dev::fbits(dev::bitsf(du[j]) + we);
+                                if (nd < dv[j]) {
+                                    dv_row[b[j]] = nd;
+                                    if (!op.pend || nd < tbits) near |= 1u << b[j];
+                                    else far |= 1u << b[j];
+                                }
+                            }
+                        }
+                    }
+                }
+                far &= ~near;
+                const int64_t i = v * W + w;
+                op.next[i] = near;
+                if (far) op.pend[i] |= far;
+            }
+            const uint64_t bm = __ballot(near != 0);
+            if (w == 0 && (bm & gmask)) {
+                crs[k] = rs;
+                cd[k] = re - rs;
+            }
+        }
+        uint32_t c = 0;
+#pragma unroll
+        for (int k = 0; k < KP; ++k) c += cd[k] > 0 ? 1u : 0u;
+        uint32_t total;
+        uint32_t pos = dev::block_exclusive_add32<kBlock>(c, q.scan32, total);
+        if (q.n + total > (uint32_t)kQueue) queue_flush(q, nxt, cnt_next);
+        pos += q.n;
+#pragma unroll
+        for (int k = 0; k < KP; ++k)
+            if (cd[k] > 0) {
+                q.v[pos] = cv[k];
+                q.rs[pos] = crs[k];
+                q.d[pos] = cd[k];
+                ++pos;
+            }
+        __syncthreads();  // every thread has read q.n
+        if (threadIdx.x == 0) q.n += total;
+        __syncthreads();
+    }
+    if (q.n > 0) queue_flush(q, nxt, cnt_next);
+}
+
 // Bellman-Ford rounds of the batch (no threshold): every improvement is near.
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_batch_sssp_bf(
@@ -537,7 +650,7 @@ template <int W>
 __global__ __launch_bounds__(kBlock) void k_batch_sssp_nf(
     const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col, int32_t nnz, int32_t m,
     SsspBatchNearOp<W> op, FrontierBufs cur, FrontierBufs nxt, uint64_t *ctl,
-    int32_t r) {
+    int32_t r, int32_t pull_div) {
     constexpr int B = 32 * W;
     __shared__ BatchLds lds;
     __shared__ uint64_t s_part[kBlock / 64];
@@ -568,6 +681,10 @@ __global__ __launch_bounds__(kBlock) void k_batch_sssp_nf(
         }
     }
     if (done) return;
+    if (!split && pull_div > 0 && (int64_t)E * pull_div > (int64_t)nnz) {
+        batch_sssp_pull<W>(lds.pq, rowptr, col, m, op, tbits, nxt, ctl + (r + 1) % 3);
+        return;
+    }
     if (!split) {
         op.tbits = tbits;
         expand_tiles<kBlock, kTile, true, true, true>(lds.ex, op, rowptr, col, nnz, cur, F, E,
@@ -869,6 +986,14 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
     const int32_t pull_div =
         sym ? (int32_t)std::min<int64_t>(div_opt * (W <= 4 && short_rows ? 4 : 1), 1 << 22)
             : 0;
+    // near-far SSSP rounds pull on graphs whose rows are all short and whose
+    // weights are symmetric (a pulled round walks every vertex's whole row)
+    const int32_t sssp_pull_div =
+        near_far && opt(c, "batch_sssp_pull_div", 4) > 0 &&
+                graph_max_degree(c, const_cast<Graph *>(g)) <= 64 &&
+                graph_is_symmetric_weighted(c, const_cast<Graph *>(g))
+            ? (int32_t)opt(c, "batch_sssp_pull_div", 4)
+            : 0;
     if (!SSSP) {
         k_batch_vis_init<W><<<grid, kBlock, 0, c->stream>>>(m, nsrc, comp, srcs, vis);
     } else {
@@ -898,7 +1023,7 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
             k_batch_sssp_nf<W><<<grid, kBlock, 0, c->stream>>>(
                 g->rowptr, g->col, (int32_t)g->nnz, m,
                 SsspBatchNearOp<W>{front, next, pend, stamp, dist, g->val, lvl, 0}, fs.f[cur],
-                fs.f[cur ^ 1], fs.ctl, l % 6);
+                fs.f[cur ^ 1], fs.ctl, l % 6, sssp_pull_div);
         else if (SSSP)
             k_batch_sssp_bf<W><<<grid, kBlock, 0, c->stream>>>(
                 g->rowptr, g->col, (int32_t)g->nnz,
@@ -919,7 +1044,8 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
         }
         k_clear_rows<W><<<grid, kBlock, 0, c->stream>>>(fs.f[cur], fs.ctl + (l % 3), front, lvl,
                                                         near_far, fs.ctl + ((l + 1) % 3),
-                                                        SSSP ? 0 : pull_div, (int32_t)g->nnz);
+                                                        SSSP ? sssp_pull_div : pull_div,
+                                                        (int32_t)g->nnz);
     };
 
     // chunks of K levels captured once and replayed: K is a multiple of 6 so

@@ -29,6 +29,9 @@ static const char *kOptionNames[] = {
     "batch_sssp_delta",  // batched SSSP near-far threshold step (0 = Bellman-Ford frontier,
                          // -1 = chosen per graph from its weights and mean row length)
     "batch_snapshot_mb", // HBM for the bit rows of pulled batched BFS levels (0 = none)
+    "batch_sssp_pull_div", // batched near-far SSSP pulls a round when union-frontier edges *
+                           // this > nnz, on graphs with rows of at most 64 entries and
+                           // symmetric weights (0 = never)
     "batch_long_row_group", // batched BFS on graphs with rows longer than 64: sources per
                             // pass (1..512)
     "profiling",         // 1 = time the dominant kernel with hipEvents
@@ -52,6 +55,7 @@ static bool option_default(const std::string &n, int64_t *v) {
         {"batch_sssp_delta", -1},
         {"batch_snapshot_mb", 24576},
         {"batch_long_row_group", 128},
+        {"batch_sssp_pull_div", 4},
         {"profiling", 0}};
     for (const auto &d : kDefaults)
         if (n == d.first) {
@@ -281,6 +285,7 @@ int apspie_ctx_set_option(apspie_ctx *c, const char *name, int64_t v) {
     if (n == "batch_pull_div" && (v < 0 || v > (1 << 20))) return APSPIE_ERR_INVALID;
     if (n == "batch_snapshot_mb" && (v < 0 || v > (1 << 22))) return APSPIE_ERR_INVALID;
     if (n == "batch_long_row_group" && (v < 1 || v > 512)) return APSPIE_ERR_INVALID;
+    if (n == "batch_sssp_pull_div" && (v < 0 || v > (1 << 20))) return APSPIE_ERR_INVALID;
     if (n == "batch_sssp_delta" && (v < -1 || v > (1 << 30))) return APSPIE_ERR_INVALID;
     if (n == "sssp_delta" && (v < -1 || v > (1 << 30))) return APSPIE_ERR_INVALID;
     if (n == "sssp_pull_div" && (v < 0 || v > (1 << 20))) return APSPIE_ERR_INVALID;

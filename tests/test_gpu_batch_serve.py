@@ -132,6 +132,39 @@ def test_sssp_batch_vs_oracle(ctx, key, nsrc):
     G.close()
 
 
+@pytest.mark.parametrize("pull_div", [0, 4, 1 << 20])
+@pytest.mark.parametrize("nsrc", [20, 100, 400])
+@pytest.mark.parametrize("key", ["lattice64", "random", "asym"])
+def test_sssp_batch_pulled_rounds(ctx, key, nsrc, pull_div):
+    """Pulled near-far rounds (rows of at most 64 entries, symmetric weights):
+    never (0), on wide rounds (4) and on every round with work (2^20), at
+    one, four and sixteen source words per vertex; a graph with isolated
+    vertices and small components; and asymmetric weights, which must keep
+    pushing -- every row equals the oracle."""
+    if key == "random":
+        g = T.random_graph(4000, 0.0008, 13)
+        w = g.w
+    else:
+        g = T.lattice(64)
+        w = g.w.copy()
+        if key == "asym":  # w(u, v) != w(v, u) on some edges
+            w[::7] += 3
+    ctx.set_option("batch_sssp_pull_div", pull_div)
+    ctx.set_option("batch_sssp_delta", 64)
+    try:
+        G = ctx.upload(g.rowptr32, g.col, w)
+        src = T.sources(g, nsrc, seed=31)
+        got = ctx.sssp_batch(G, src)
+        for i in range(0, nsrc, 3):
+            d = O.sssp(g.m, g.rowptr, g.col, w, src[i])
+            want = np.where(d == -1, A.SSSP_INF, d).astype(np.float32)
+            assert np.array_equal(got[i], want), (key, nsrc, pull_div, i)
+        G.close()
+    finally:
+        ctx.set_option("batch_sssp_pull_div", 4)
+        ctx.set_option("batch_sssp_delta", -1)
+
+
 @pytest.mark.parametrize("delta", [-1, 0, 1, 16, 256, 1 << 20])
 @pytest.mark.parametrize("nsrc", [7, 64, 150])
 @pytest.mark.parametrize("key", ["rmat11", "lattice32", "path"])
