@@ -129,6 +129,9 @@ def _declare(L):
         "apspie_bfs_batch": ([vp, vp, vp, i32, vp, C.c_int], C.c_int),
         "apspie_sssp_batch": ([vp, vp, vp, i32, vp, C.c_int], C.c_int),
         "apspie_autotune": ([vp, vp, i32], C.c_int),
+        "apspie_autotune_hash": ([vp, vp], C.c_int),
+        "apspie_tune_save": ([vp, vp, C.c_char_p], C.c_int),
+        "apspie_tune_load": ([vp, vp, C.c_char_p, C.POINTER(i32)], C.c_int),
         "apspie_gen_rmat": ([C.c_int, C.c_int, f64, f64, f64, u32, vp, vp, C.POINTER(i64)],
                             C.c_int),
         "apspie_gen_lattice": ([i32, u32, vp, vp, C.POINTER(i64)], C.c_int),
@@ -448,6 +451,20 @@ class Context:
 
     def autotune(self, g, batch=0):
         _check(self._L.apspie_autotune(self.h, g.h, int(batch)), self.h)
+
+    def autotune_hash(self, g):
+        """Tune the TC / SpGEMM tier bounds on this graph (A * A)."""
+        _check(self._L.apspie_autotune_hash(self.h, g.h), self.h)
+
+    def tune_save(self, g, path):
+        """Save the tuned options as this GPU's entry for g's graph class."""
+        _check(self._L.apspie_tune_save(self.h, g.h, str(path).encode()), self.h)
+
+    def tune_load(self, g, path):
+        """Apply the saved entry for this GPU and g's class; True if one matched."""
+        f = C.c_int32()
+        _check(self._L.apspie_tune_load(self.h, g.h, str(path).encode(), C.byref(f)), self.h)
+        return bool(f.value)
 
 
 class DeviceBuffer:

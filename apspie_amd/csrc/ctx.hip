@@ -34,6 +34,12 @@ static const char *kOptionNames[] = {
                            // symmetric weights (0 = never)
     "batch_long_row_group", // batched BFS on graphs with rows longer than 64: sources per
                             // pass (1..512)
+    "tc_medium_row",     // TC: rows with |L_r| up to this go to the 256-thread LDS tier, longer
+                         // ones to the 1024-thread tier (128..2048)
+    "spgemm_wave_ub",    // SpGEMM: rows with at most this many products to the wave tier (<= 256)
+    "spgemm_block_ub",   // SpGEMM: up to this to the block LDS tier (<= 4096)
+    "spgemm_dense",      // SpGEMM: longer rows to the dense accumulator when B has <= 32768
+                         // columns (1) or always to global tables (0)
     "profiling",         // 1 = time the dominant kernel with hipEvents
     nullptr};
 
@@ -56,6 +62,10 @@ static bool option_default(const std::string &n, int64_t *v) {
         {"batch_snapshot_mb", 24576},
         {"batch_long_row_group", 128},
         {"batch_sssp_pull_div", 4},
+        {"tc_medium_row", 2048},
+        {"spgemm_wave_ub", 256},
+        {"spgemm_block_ub", 4096},
+        {"spgemm_dense", 1},
         {"profiling", 0}};
     for (const auto &d : kDefaults)
         if (n == d.first) {
@@ -286,6 +296,10 @@ int apspie_ctx_set_option(apspie_ctx *c, const char *name, int64_t v) {
     if (n == "batch_snapshot_mb" && (v < 0 || v > (1 << 22))) return APSPIE_ERR_INVALID;
     if (n == "batch_long_row_group" && (v < 1 || v > 512)) return APSPIE_ERR_INVALID;
     if (n == "batch_sssp_pull_div" && (v < 0 || v > (1 << 20))) return APSPIE_ERR_INVALID;
+    if (n == "tc_medium_row" && (v < 128 || v > 2048)) return APSPIE_ERR_INVALID;
+    if (n == "spgemm_wave_ub" && (v < 1 || v > 256)) return APSPIE_ERR_INVALID;
+    if (n == "spgemm_block_ub" && (v < 1 || v > 4096)) return APSPIE_ERR_INVALID;
+    if (n == "spgemm_dense" && (v < 0 || v > 1)) return APSPIE_ERR_INVALID;
     if (n == "batch_sssp_delta" && (v < -1 || v > (1 << 30))) return APSPIE_ERR_INVALID;
     if (n == "sssp_delta" && (v < -1 || v > (1 << 30))) return APSPIE_ERR_INVALID;
     if (n == "sssp_pull_div" && (v < 0 || v > (1 << 20))) return APSPIE_ERR_INVALID;

@@ -63,14 +63,14 @@ __global__ void k_row_bound(const int32_t *arp, const int32_t *acol, const int32
 
 // bins: 0 wave, 1 block-LDS, 2 dense, 3 block-global; ctl[0..3] counts;
 // ctl[4] max u of the global bin
-__global__ void k_bin_rows(const int64_t *ub, int32_t m, bool dense_ok, int32_t *lists,
-                           uint32_t *ctl, int32_t *row_nnz) {
+__global__ void k_bin_rows(const int64_t *ub, int32_t m, int32_t wave_ub, int32_t block_ub,
+                           bool dense_ok, int32_t *lists, uint32_t *ctl, int32_t *row_nnz) {
     for (int64_t r0 = blockIdx.x * (int64_t)blockDim.x; r0 < m; r0 += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = r0 + threadIdx.x;
         const int64_t u = r < m ? ub[r] : 0;
         const int bin = u == 0 ? -1
-                        : u <= kWaveUb ? 0
-                        : u <= kBlockUb ? 1
+                        : u <= wave_ub ? 0
+                        : u <= block_ub ? 1
                         : dense_ok ? 2 : 3;
         if (r < m && bin < 0) row_nnz[r] = 0;
 #pragma unroll
@@ -454,10 +454,19 @@ Graph *spgemm_run(Ctx *c, const Graph *a, const Graph *b) {
         int32_t *lists = (int32_t *)scratch(c, "sg_lists", sizeof(int32_t) * 4 * (size_t)std::max(m, 1));
         uint32_t *ctl = (uint32_t *)scratch(c, "sg_ctl", 64);
         fill_i32(c, (int32_t *)ctl, 0, 16);
-        const bool dense_ok = b->n <= kDenseN;
+        // tier bounds (tunable below the tables' capacity): the wave tier's
+        // table holds rows of up to kWaveUb products, the block tier's up to
+        // kBlockUb; longer rows use the dense accumulator (B with at most
+        // kDenseN columns, unless switched off) or global-memory tables
+        const bool dense_ok = b->n <= kDenseN && opt(c, "spgemm_dense", 1) != 0;
+        const int32_t wave_ub =
+            (int32_t)std::min<int64_t>(std::max<int64_t>(opt(c, "spgemm_wave_ub", kWaveUb), 1), kWaveUb);
+        const int32_t block_ub = (int32_t)std::min<int64_t>(
+            std::max<int64_t>(opt(c, "spgemm_block_ub", kBlockUb), wave_ub), kBlockUb);
         if (m > 0) {
             k_row_bound<<<grid, kBlock, 0, c->stream>>>(a->rowptr, a->col, b->rowptr, m, b->n, ub);
-            k_bin_rows<<<grid, kBlock, 0, c->stream>>>(ub, m, dense_ok, lists, ctl, row_nnz);
+            k_bin_rows<<<grid, kBlock, 0, c->stream>>>(ub, m, wave_ub, block_ub, dense_ok, lists,
+                                                       ctl, row_nnz);
             APSPIE_LAUNCH_CHECK();
         }
         uint32_t h[5] = {0, 0, 0, 0, 0};

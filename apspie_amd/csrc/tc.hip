@@ -74,14 +74,14 @@ __global__ void k_check_sorted(const int32_t *col, int64_t nnz, const uint32_t *
 
 // bins by |L_r|: [0] small, [1] medium, [2] big, [3] huge (lists[b] has m
 // slots each); ctl[0..3] counts, ctl[4] max |L_r|
-__global__ void k_tc_bin(const int32_t *lowcnt, int32_t m, int32_t *lists, uint32_t *ctl,
-                         int32_t *per_row) {
+__global__ void k_tc_bin(const int32_t *lowcnt, int32_t m, int32_t medium_row, int32_t big_row,
+                         int32_t *lists, uint32_t *ctl, int32_t *per_row) {
     for (int64_t r0 = blockIdx.x * (int64_t)blockDim.x; r0 < m; r0 += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = r0 + threadIdx.x;
         int32_t n = r < m ? lowcnt[r] : 0;
         const int bin = n < 2 ? -1
                               : (n <= kSmallRow ? 0
-                                                : (n <= kMediumRow ? 1 : (n <= kBigRow ? 2 : 3)));
+                                                : (n <= medium_row ? 1 : (n <= big_row ? 2 : 3)));
         if (r < m && per_row) per_row[r] = 0;  // rows add into each other's counts
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
@@ -463,7 +463,13 @@ void tc_run(Ctx *c, const Graph *g0, uint64_t *total_out, int32_t *per_row) {
     fill_i32(c, (int32_t *)ctl, 0, 16);
     int32_t *per = per_row;  // may be null: total only, no per-row atomics
     int32_t *lists = (int32_t *)scratch(c, "tc_lists", sizeof(int32_t) * 4 * (size_t)m);
-    k_tc_bin<<<grid, kBlock, 0, c->stream>>>(g->lowcnt, m, lists, ctl, per);
+    // the medium / big tier bound (tunable up to the medium tier's LDS table
+    // capacity, kMediumRow): a row above it goes to the 1024-thread tier, whose
+    // LDS table holds rows up to kBigRow (longer ones use global memory)
+    const int32_t medium_row =
+        (int32_t)std::min<int64_t>(std::max<int64_t>(opt(c, "tc_medium_row", kMediumRow), kSmallRow),
+                                   kMediumRow);
+    k_tc_bin<<<grid, kBlock, 0, c->stream>>>(g->lowcnt, m, medium_row, kBigRow, lists, ctl, per);
     APSPIE_LAUNCH_CHECK();
     uint32_t h[5];
     APSPIE_HIP(hipMemcpyAsync(h, ctl, sizeof h, hipMemcpyDeviceToHost, c->stream));

@@ -391,6 +391,9 @@ def other_workloads(ctx, cpu, which):
             res["tc_rmat20"] = {"ms": ms, "triangles": int(total), "reference_wedges": wedges,
                                 "probes": ctx.stats()["edges_traversed"],
                                 "roofline_frac": alg / (ms / 1e3) / HBM_PEAK}
+            res["tc_rmat20"]["autotuned"] = tuned_vs_default(
+                ctx, lambda: ctx.tc(G20, per_row=True), lambda: ctx.autotune_hash(G20),
+                ("tc_medium_row",))
         G20.close()
     if "tc" in which:
         rp18, col18 = A.gen_rmat(18)
@@ -423,6 +426,9 @@ def other_workloads(ctx, cpu, which):
         res["spgemm_rmat14_AxA"] = {"ms": ms, "nnz_c": C.nnz, "products": prods,
                                     "roofline_frac": alg / (ms / 1e3) / HBM_PEAK}
         C.close()
+        res["spgemm_rmat14_AxA"]["autotuned"] = tuned_vs_default(
+            ctx, lambda: ctx.spgemm(G14, G14).close(), lambda: ctx.autotune_hash(G14),
+            ("spgemm_wave_ub", "spgemm_block_ub", "spgemm_dense"))
         G14.close()
         if cpu:
             m14 = len(rp14) - 1
@@ -435,6 +441,33 @@ def other_workloads(ctx, cpu, which):
     for k, v in res.items():
         log(f"  {k}: {v}")
     return res
+
+
+TUNABLE_DEFAULTS = {"tc_medium_row": 2048, "spgemm_wave_ub": 256, "spgemm_block_ub": 4096,
+                    "spgemm_dense": 1}
+
+
+def tuned_vs_default(ctx, run, tune, names, reps=5):
+    """Median wall time of run() at the default options, then after tune()
+    (the autotuner's picks for `names` reported), then the defaults again."""
+    def med():
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            run()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        return float(np.median(ts))
+    run()
+    base = med()
+    t0 = time.perf_counter()
+    tune()
+    tune_s = time.perf_counter() - t0
+    picked = {k: ctx.get_option(k) for k in names}
+    tuned = med()
+    for k in names:
+        ctx.set_option(k, TUNABLE_DEFAULTS[k])
+    return {"ms_default": base, "ms_tuned": tuned, "speedup": base / tuned, "picked": picked,
+            "tuning_s": tune_s}
 
 
 def batched(ctx):

@@ -281,9 +281,25 @@ int apspie_bfs_batch(apspie_ctx *ctx, const apspie_graph *g, const int32_t *sour
                      int32_t nsrc, int32_t *depth, int mem);
 int apspie_sssp_batch(apspie_ctx *ctx, const apspie_graph *g, const int32_t *sources,
                       int32_t nsrc, float *dist, int mem);
-/* Autotune batched/plain tile and bin choices for this graph; stores the
- * winners in the context's tuning table. */
+/* Autotune the traversal launch choices for this graph (batch == 0: the
+ * single-source BFS / SSSP ones; batch > 0: the batched ones at that many
+ * sources) and keep the winners in the context's options.  No reference
+ * counterpart: the reference hard-codes NTHREADS and its mgpu LaunchBox
+ * tables (spgemm.cuh:181-185). */
 int apspie_autotune(apspie_ctx *ctx, const apspie_graph *g, int32_t batch);
+/* Autotune the hash machinery on this graph: TC's tier bound (tc_medium_row)
+ * and SpGEMM's (spgemm_wave_ub, spgemm_block_ub, spgemm_dense) for A * A --
+ * which table size and block shape serves a row.  Replaces the reference's
+ * fixed TABLE_SIZE / MAX_PROBES / THREADS / BLOCKS (spgemmKernel.cuh:9-26). */
+int apspie_autotune_hash(apspie_ctx *ctx, const apspie_graph *g);
+/* The tuned options of this context as the entry for (this GPU, this graph's
+ * class) in a text table at `path` (one line per pair; an existing entry for
+ * the pair is replaced), and back: apspie_tune_load applies the entry that
+ * matches, setting *found to 1 (0 and nothing applied when there is none or
+ * the file does not exist).  The class is log2 of the vertex count, of the
+ * mean and of the longest row, symmetric or not, weighted or not. */
+int apspie_tune_save(apspie_ctx *ctx, const apspie_graph *g, const char *path);
+int apspie_tune_load(apspie_ctx *ctx, const apspie_graph *g, const char *path, int32_t *found);
 
 /* ---------------------------------------------------------------- synthetic */
 /* Host-side graph construction (synthetic inputs; the reference has none).

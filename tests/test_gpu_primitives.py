@@ -477,3 +477,109 @@ def test_components_need_symmetric(ctx):
     with pytest.raises(A.ApspieError):
         ctx.components(G)
     G.close()
+
+
+@pytest.mark.parametrize("wave_ub,block_ub,dense", [(64, 1024, 1), (128, 2048, 0),
+                                                     (256, 4096, 1), (1, 1, 0), (256, 4096, 0)])
+def test_spgemm_tier_bounds_exact(ctx, wave_ub, block_ub, dense):
+    """The tuner's SpGEMM knobs only move rows between tiers: every setting
+    (down to every row in the dense or global tier) gives the oracle's
+    product on R-MAT-12 squared and on a matrix with a very long row."""
+    ctx.set_option("spgemm_wave_ub", wave_ub)
+    ctx.set_option("spgemm_block_ub", block_ub)
+    ctx.set_option("spgemm_dense", dense)
+    try:
+        g = T.rmat(12)
+        Am = sp.csr_matrix((np.ones(len(g.col)), g.col, g.rowptr), shape=(g.m, g.m))
+        Hm = sp.lil_matrix((50, 6000))
+        Hm[0, :] = 1
+        Hm[1, :40] = 1
+        Hm = Hm.tocsr()
+        Bh = sp.random(6000, 7000, 0.002, random_state=5, format="csr")
+        Bh.data = np.random.default_rng(8).integers(1, 5, Bh.nnz).astype(np.float64)  # exact sums
+        for A_, B_ in ((Am, Am), (Hm, Bh)):
+            for M in (A_, B_):
+                M.sort_indices()
+            a = A_.data.astype(np.float32)
+            b = B_.data.astype(np.float32) if B_ is not A_ else a
+            Ga = ctx.upload(A_.indptr.astype(np.int32), A_.indices.astype(np.int32), a,
+                            n=A_.shape[1])
+            Gb = Ga if B_ is A_ else ctx.upload(B_.indptr.astype(np.int32),
+                                                B_.indices.astype(np.int32), b, n=B_.shape[1])
+            Gc = ctx.spgemm(Ga, Gb)
+            rp, cc, vv = Gc.download(values=True)
+            wrp, wc, wv = O.spgemm(A_.shape[0], A_.shape[1], B_.shape[1],
+                                   A_.indptr.astype(np.int64), A_.indices.astype(np.int32), a,
+                                   B_.indptr.astype(np.int64), B_.indices.astype(np.int32), b)
+            assert np.array_equal(rp, wrp.astype(np.int32)) and np.array_equal(cc, wc)
+            assert np.array_equal(vv, wv)
+            for G in {id(Ga): Ga, id(Gb): Gb, id(Gc): Gc}.values():
+                G.close()
+    finally:
+        ctx.set_option("spgemm_wave_ub", 256)
+        ctx.set_option("spgemm_block_ub", 4096)
+        ctx.set_option("spgemm_dense", 1)
+
+
+@pytest.mark.parametrize("medium_row", [128, 256, 1024, 2048])
+def test_tc_tier_bound_exact(ctx, medium_row):
+    """TC's tuned medium / big tier bound moves rows between the 256- and
+    1024-thread tiers only: the count and the per-row counts stay the
+    oracle's (R-MAT-14 has rows in every tier)."""
+    ctx.set_option("tc_medium_row", medium_row)
+    try:
+        g = T.rmat(14)
+        G = ctx.upload(g.rowptr32, g.col)
+        total, rows = ctx.tc(G)
+        wt, wr = O.tc(g.m, g.rowptr, g.col)
+        assert total == wt and np.array_equal(rows, wr)
+        G.close()
+    finally:
+        ctx.set_option("tc_medium_row", 2048)
+
+
+def test_autotune_hash_and_table_round_trip(tmp_path):
+    """autotune_hash picks tier bounds among its candidates and the results
+    stay exact; the tuned options saved for (this GPU, the graph's class) come
+    back on a fresh context for a graph of the same class, not for another
+    class; saving again replaces the entry; a malformed entry is refused."""
+    g = T.rmat(12)
+    path = tmp_path / "tune.txt"
+    with A.Context(0) as c:
+        G = c.upload(g.rowptr32, g.col)
+        c.autotune_hash(G)
+        picked = {k: c.get_option(k) for k in ("tc_medium_row", "spgemm_wave_ub",
+                                                 "spgemm_block_ub", "spgemm_dense")}
+        assert picked["tc_medium_row"] in (256, 512, 1024, 2048)
+        assert picked["spgemm_wave_ub"] in (64, 128, 256)
+        assert picked["spgemm_block_ub"] in (1024, 2048, 4096)
+        assert picked["spgemm_dense"] in (0, 1)
+        total, rows = c.tc(G)
+        wt, wr = O.tc(g.m, g.rowptr, g.col)
+        assert total == wt and np.array_equal(rows, wr)
+        c.set_option("bfs_alpha", 28)
+        c.tune_save(G, path)
+        c.tune_save(G, path)  # same (GPU, class): still one entry
+        lines = path.read_text().splitlines()
+        assert len(lines) == 1 and "class=v12_" in lines[0] and "bfs_alpha=28" in lines[0]
+        G.close()
+    with A.Context(0) as c:
+        G = c.upload(g.rowptr32, g.col)  # same graph, fresh context
+        assert c.get_option("bfs_alpha") == 14
+        assert c.tune_load(G, path)
+        assert c.get_option("bfs_alpha") == 28
+        for k, v in picked.items():
+            assert c.get_option(k) == v
+        other = T.lattice(32)
+        H = c.upload(other.rowptr32, other.col)
+        assert not c.tune_load(H, path)  # another class: nothing applied
+        assert not c.tune_load(G, tmp_path / "missing.txt")
+        H.close()
+        bad = tmp_path / "bad.txt"
+        bad.write_text(lines[0].replace("bfs_alpha=28", "bfs_alpha=x"))
+        with pytest.raises(A.ApspieError):
+            c.tune_load(G, bad)
+        bad.write_text(lines[0].replace("bfs_alpha=28", "bfs_tile=3000"))
+        with pytest.raises(A.ApspieError):
+            c.tune_load(G, bad)
+        G.close()
