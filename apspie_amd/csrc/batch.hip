@@ -170,12 +170,12 @@ __device__ __forceinline__ void pull_finish(int64_t i, uint32_t nw, uint32_t uv,
 // a claim to the queue (room was made for nl claims).
 constexpr int kLongStep = 256;
 template <int W>
-__device__ __forceinline__ void pull_long(PullQueue &q, uint32_t nl,
-                                          const int32_t *__restrict__ rowptr,
-                                          const int32_t *__restrict__ col,
-                                          const uint32_t *__restrict__ front, uint32_t *vis,
-                                          uint32_t *next, uint32_t *snap, int32_t *depth,
-                                          int32_t level) {
+__device__ __noinline__ void pull_long(PullQueue &q, uint32_t nl,
+                                       const int32_t *__restrict__ rowptr,
+                                       const int32_t *__restrict__ col,
+                                       const uint32_t *__restrict__ front, uint32_t *vis,
+                                       uint32_t *next, uint32_t *snap, int32_t *depth,
+                                       int32_t level) {
     const int lane = dev::lane_id();
     const int wave = threadIdx.x / 64, nwaves = kBlock / 64;
     for (uint32_t j = wave; j < nl; j += nwaves) {
@@ -281,21 +281,24 @@ __device__ __forceinline__ void batch_pull(PullQueue &q, const int32_t *__restri
 #pragma unroll
             for (int q = 0; q < Q; ++q)
                 if (u[k][q] >= 0) acc[k] |= front[(int64_t)u[k][q] * W + w];
-        // open long rows go to the LDS list (the group's leader reserves)
-        bool deferred[KP];
 #pragma unroll
         for (int k = 0; k < KP; ++k) {
-            const bool open = (__ballot((acc[k] & uv[k]) != uv[k]) & gmask) != 0;
-            int slot = -1;
-            if (open && re[k] - rs[k] > kLongRow && w == 0)
-                slot = (int)atomicAdd(&q.nl, 1u);
-            slot = __shfl(slot, leader, 64);
-            deferred[k] = slot >= 0 && slot < kLongCap;
-            if (deferred[k] && w == 0) q.lv[slot] = cv[k];
-        }
-#pragma unroll
-        for (int k = 0; k < KP; ++k) {
-            if (deferred[k]) continue;
+            // an open long row goes to the LDS list (the group's leader
+            // reserves a slot); the pass then finishes it as if nothing was
+            // found -- no more entries, no bits -- and the long-row pass after
+            // the block's next barrier rewrites its words
+            if (__ballot(re[k] - rs[k] > kLongRow) != 0) {  // uniform: a long row in this wave
+                const bool open = (__ballot((acc[k] & uv[k]) != uv[k]) & gmask) != 0;
+                int slot = -1;
+                if (open && re[k] - rs[k] > kLongRow && w == 0)
+                    slot = (int)atomicAdd(&q.nl, 1u);
+                slot = __shfl(slot, leader, 64);
+                if (slot >= 0 && slot < kLongCap) {
+                    if (w == 0) q.lv[slot] = cv[k];
+                    acc[k] = 0;
+                    re[k] = rs[k] + Q;
+                }
+            }
             for (int32_t e = rs[k] + Q; e < re[k] && (acc[k] & uv[k]) != uv[k]; e += Q) {
                 int32_t uu[Q];
 #pragma unroll
@@ -307,7 +310,6 @@ __device__ __forceinline__ void batch_pull(PullQueue &q, const int32_t *__restri
         }
 #pragma unroll
         for (int k = 0; k < KP; ++k) {
-            if (deferred[k]) continue;
             const uint32_t nw = acc[k] & uv[k];
             if (cv[k] < m)
                 pull_finish<W>((int64_t)cv[k] * W + w, nw, uv[k], vis, next, snap, depth, level,

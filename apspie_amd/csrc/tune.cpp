@@ -17,7 +17,8 @@
 //   apspie_autotune_hash: TC's medium / big tier bound (a 256-thread block
 //     with a 4096-slot LDS table vs a 1024-thread block with 32768 slots),
 //     then SpGEMM's wave / block tier bounds (512- and 8192-slot tables) and
-//     whether rows beyond them use the dense accumulator or global tables.
+//     whether rows beyond them use the dense accumulator or global tables
+//     (A * A; skipped when the product does not fit one call).
 // Every candidate is timed three times (median of the calls' device time);
 // the buffers come from the context's scratch pool (its HBM budget applies)
 // and every copy runs on its stream.
@@ -208,6 +209,12 @@ extern "C" int apspie_autotune_hash(apspie_ctx *c, const apspie_graph *g) {
             if (p) apspie_graph_destroy(p);
             return st;
         };
+        // a product too large for one call (more than 2^31 entries, or beyond
+        // the budget) is not tuned
+        const int first = product();
+        if (first == APSPIE_ERR_OVERFLOW || first == APSPIE_ERR_OOM || first == APSPIE_ERR_BUDGET)
+            return;
+        APSPIE_CHECK(first == APSPIE_OK, first, "autotune run failed: " + c->last_error);
         double best = 1e300;
         int64_t bw = 256, bk = 4096;
         for (int64_t wub : {64, 128, 256})

@@ -289,7 +289,8 @@ int apspie_sssp_batch(apspie_ctx *ctx, const apspie_graph *g, const int32_t *sou
 int apspie_autotune(apspie_ctx *ctx, const apspie_graph *g, int32_t batch);
 /* Autotune the hash machinery on this graph: TC's tier bound (tc_medium_row)
  * and SpGEMM's (spgemm_wave_ub, spgemm_block_ub, spgemm_dense) for A * A --
- * which table size and block shape serves a row.  Replaces the reference's
+ * which table size and block shape serves a row; SpGEMM is left untuned when
+ * A * A does not fit one call (more than 2^31 entries, or the budget).  Replaces the reference's
  * fixed TABLE_SIZE / MAX_PROBES / THREADS / BLOCKS (spgemmKernel.cuh:9-26). */
 int apspie_autotune_hash(apspie_ctx *ctx, const apspie_graph *g);
 /* The tuned options of this context as the entry for (this GPU, this graph's
