@@ -30,7 +30,8 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kSmallRow = 128;     // wave bin: |L_r| <= 128, table 256
 constexpr int kSmallTable = 256;
-constexpr int kMediumRow = 2048;   // block bin: |L_r| <= 2048, LDS table 4096
+constexpr int kMediumRow = 2048;   // block bin capacity: |L_r| <= 2048 (LDS table 4096); the
+                                   // bound used is tc_medium_row, 512 by default
 constexpr int kMediumTable = 4096;
 constexpr int kMediumChunk = 1024;  // staging of the medium tier (LDS: 5 blocks per CU)
 constexpr int kBigRow = 16384;     // 1024-thread block, LDS table 32768 (128 KB)
@@ -467,7 +468,7 @@ void tc_run(Ctx *c, const Graph *g0, uint64_t *total_out, int32_t *per_row) {
     // capacity, kMediumRow): a row above it goes to the 1024-thread tier, whose
     // LDS table holds rows up to kBigRow (longer ones use global memory)
     const int32_t medium_row =
-        (int32_t)std::min<int64_t>(std::max<int64_t>(opt(c, "tc_medium_row", kMediumRow), kSmallRow),
+        (int32_t)std::min<int64_t>(std::max<int64_t>(opt(c, "tc_medium_row", 512), kSmallRow),
                                    kMediumRow);
     k_tc_bin<<<grid, kBlock, 0, c->stream>>>(g->lowcnt, m, medium_row, kBigRow, lists, ctl, per);
     APSPIE_LAUNCH_CHECK();

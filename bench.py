@@ -443,7 +443,7 @@ def other_workloads(ctx, cpu, which):
     return res
 
 
-TUNABLE_DEFAULTS = {"tc_medium_row": 2048, "spgemm_wave_ub": 256, "spgemm_block_ub": 4096,
+TUNABLE_DEFAULTS = {"tc_medium_row": 512, "spgemm_wave_ub": 256, "spgemm_block_ub": 4096,
                     "spgemm_dense": 1}
 
 

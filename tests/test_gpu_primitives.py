@@ -535,7 +535,7 @@ def test_tc_tier_bound_exact(ctx, medium_row):
         assert total == wt and np.array_equal(rows, wr)
         G.close()
     finally:
-        ctx.set_option("tc_medium_row", 2048)
+        ctx.set_option("tc_medium_row", 512)
 
 
 def test_autotune_hash_and_table_round_trip(tmp_path):
