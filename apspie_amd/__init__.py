@@ -66,7 +66,8 @@ class _ServerCfg(C.Structure):
 
 class _ServerStats(C.Structure):
     _fields_ = [("batches", C.c_int64), ("queries", C.c_int64), ("busy_ms", C.c_double),
-                ("traverse_ms", C.c_double), ("copy_ms", C.c_double), ("alloc_ms", C.c_double)]
+                ("traverse_ms", C.c_double), ("copy_ms", C.c_double), ("alloc_ms", C.c_double),
+                ("retries", C.c_int64), ("retry_ms", C.c_double)]
 
 
 def lib():
@@ -97,6 +98,8 @@ def _declare(L):
         "apspie_ctx_set_option": ([vp, C.c_char_p, i64], C.c_int),
         "apspie_ctx_get_option": ([vp, C.c_char_p, C.POINTER(i64)], C.c_int),
         "apspie_ctx_trim": ([vp], C.c_int),
+        "apspie_ctx_reserve_arena": ([vp, i64], C.c_int),
+        "apspie_ctx_arena_bytes": ([vp], i64),
         "apspie_ctx_sync": ([vp], C.c_int),
         "apspie_dev_alloc": ([vp, i64, pp], C.c_int),
         "apspie_dev_free": ([vp, vp, i64], C.c_int),
@@ -322,6 +325,15 @@ class Context:
 
     def trim(self):
         _check(self._L.apspie_ctx_trim(self.h), self.h)
+
+    def reserve_arena(self, nbytes):
+        """Carve scratch buffers from one device block of nbytes (0 releases
+        it): growing or dropping them then waits only for this context's
+        stream."""
+        _check(self._L.apspie_ctx_reserve_arena(self.h, int(nbytes)), self.h)
+
+    def arena_bytes(self):
+        return self._L.apspie_ctx_arena_bytes(self.h)
 
     # -- device memory (plain HIP allocations owned by the caller)
     def dev_alloc(self, nbytes):

@@ -940,11 +940,16 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
     int32_t *depth = SSSP ? nullptr
                           : (int32_t *)scratch(c, "batch_dep", sizeof(int32_t) * m * (int64_t)B);
     // snapshots of pulled BFS levels, within batch_snapshot_mb (and half the
-    // context's remaining HBM budget, if it has one)
+    // room the context's HBM budget or arena leaves; a snapshot buffer kept
+    // from an earlier batch counts as room)
     BatchSnap sn{};
     if (!SSSP) {
         int64_t bytes = opt(c, "batch_snapshot_mb", 24576) << 20;
-        if (c->budget > 0) bytes = std::min<int64_t>(bytes, std::max<int64_t>(0, (c->budget - c->used) / 2));
+        if (c->budget > 0 || c->arena) {
+            auto kept = c->scratch.find("batch_snap");
+            const int64_t have = kept != c->scratch.end() ? (int64_t)kept->second.bytes : 0;
+            bytes = std::min<int64_t>(bytes, std::max<int64_t>(have, scratch_room(c) / 2));
+        }
         sn.stride = words;
         sn.cap = (int32_t)std::min<int64_t>(bytes / (int64_t)(sizeof(uint32_t) * words), 4096);
         sn.snap = sn.cap > 0 ? (uint32_t *)scratch(c, "batch_snap",

@@ -84,6 +84,13 @@ struct Ctx {
     int64_t budget = 0;  // 0 = unlimited
     int64_t used = 0;    // bytes held: uploaded graphs + scratch
     std::map<std::string, Buffer> scratch;
+    // optional arena: one block of device memory the scratch buffers are
+    // carved from (first fit, 256-byte granules, free ranges coalesced), so
+    // that growing or dropping them never calls hipFree, which waits for every
+    // stream on the device
+    char *arena = nullptr;
+    size_t arena_bytes = 0;
+    std::map<size_t, size_t> arena_free;  // offset -> length
     std::map<std::string, int64_t> options;
     std::string last_error;
     apspie_stats stats{};
@@ -106,6 +113,9 @@ int64_t opt(Ctx *c, const char *name, int64_t dflt);
 
 // Scratch buffer by name, grown on demand (contents not preserved on growth).
 void *scratch(Ctx *c, const std::string &name, size_t bytes);
+// Bytes a new scratch buffer could still take: the largest free range of the
+// arena, else what the HBM budget leaves (INT64_MAX without either).
+int64_t scratch_room(Ctx *c);
 void *dev_alloc(Ctx *c, size_t bytes);  // counted against the budget
 void dev_free(Ctx *c, void *p, size_t bytes);
 
@@ -180,7 +190,10 @@ void sssp_batch_run(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsr
                     float *dist_dev);
 void graph_from_coo_dev(Ctx *c, Graph *g, const int32_t *row, const int32_t *col,
                         const float *val, int64_t nnz, int symmetrize);
-void graph_transpose_dev(Ctx *c, const Graph *g, Graph *t);
+// Transposed copy of g into t.  With `in_scratch` its arrays are the
+// context's scratch buffers of that name prefix (a temporary: t owns nothing
+// and is valid until those buffers are reused), else t owns new device arrays.
+void graph_transpose_dev(Ctx *c, const Graph *g, Graph *t, const char *in_scratch = nullptr);
 int64_t graph_to_dcsc_dev(Ctx *c, const Graph *g, int32_t ps, int32_t pn, int32_t *part_ptr,
                           int32_t *col_ind, int32_t *col_off, int32_t *row_ind, float *val);
 void mt19937_priorities(Ctx *c, uint32_t seed, int32_t m, uint32_t *out_dev);

@@ -116,20 +116,92 @@ void dev_free(Ctx *c, void *p, size_t bytes) {
     c->used -= (int64_t)bytes;
 }
 
+namespace {
+
+constexpr size_t kGranule = 256;
+
+size_t granules(size_t bytes) { return (std::max<size_t>(bytes, 1) + kGranule - 1) / kGranule * kGranule; }
+
+// first free range of the arena that holds `bytes` (a granule multiple)
+void *arena_take(Ctx *c, size_t bytes) {
+    for (auto it = c->arena_free.begin(); it != c->arena_free.end(); ++it) {
+        if (it->second < bytes) continue;
+        const size_t off = it->first, len = it->second;
+        c->arena_free.erase(it);
+        if (len > bytes) c->arena_free.emplace(off + bytes, len - bytes);
+        return c->arena + off;
+    }
+    return nullptr;
+}
+
+// return a range, merged with the free ranges on either side
+void arena_give(Ctx *c, void *p, size_t bytes) {
+    auto it = c->arena_free.emplace((size_t)((char *)p - c->arena), bytes).first;
+    auto nx = std::next(it);
+    if (nx != c->arena_free.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        c->arena_free.erase(nx);
+    }
+    if (it != c->arena_free.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            c->arena_free.erase(it);
+        }
+    }
+}
+
+// drop every scratch buffer; the caller has synchronised the stream
+void drop_scratch(Ctx *c) {
+    if (c->arena) {
+        c->arena_free.clear();
+        c->arena_free.emplace(0, c->arena_bytes);
+    } else {
+        for (auto &kv : c->scratch) dev_free(c, kv.second.ptr, kv.second.bytes);
+    }
+    c->scratch.clear();
+}
+
+}  // namespace
+
 void *scratch(Ctx *c, const std::string &name, size_t bytes) {
     Buffer &b = c->scratch[name];
     if (b.bytes >= bytes && b.ptr) return b.ptr;
     if (b.ptr) {
         // pending work may still read the old buffer
         sync(c);
-        dev_free(c, b.ptr, b.bytes);
+        if (c->arena)
+            arena_give(c, b.ptr, b.bytes);
+        else
+            dev_free(c, b.ptr, b.bytes);
         b.ptr = nullptr;
         b.bytes = 0;
+    }
+    if (c->arena) {
+        const size_t want = granules(bytes);
+        b.ptr = arena_take(c, want);
+        if (!b.ptr)
+            throw Error{APSPIE_ERR_BUDGET, "scratch arena full: need " + std::to_string(want) +
+                                               " bytes in one range, the largest free range is " +
+                                               std::to_string(scratch_room(c)) +
+                                               " (apspie_ctx_trim drops the cached buffers)"};
+        b.bytes = want;
+        return b.ptr;
     }
     size_t want = std::max<size_t>(bytes, 256);
     b.ptr = dev_alloc(c, want);
     b.bytes = want;
     return b.ptr;
+}
+
+int64_t scratch_room(Ctx *c) {
+    if (c->arena) {
+        size_t best = 0;
+        for (const auto &kv : c->arena_free) best = std::max(best, kv.second);
+        return (int64_t)best;
+    }
+    if (c->budget > 0) return std::max<int64_t>(0, c->budget - c->used);
+    return INT64_MAX;
 }
 
 void copy_to_device(Ctx *c, void *dst, const void *src, size_t bytes, int mem) {
@@ -253,8 +325,11 @@ int apspie_ctx_destroy(apspie_ctx *c) {
     if (!c) return APSPIE_ERR_INVALID;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (auto &kv : c->scratch)
-        if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    if (c->arena)
+        (void)hipFree(c->arena);
+    else
+        for (auto &kv : c->scratch)
+            if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -321,10 +396,33 @@ int apspie_ctx_trim(apspie_ctx *c) {
     return guard(c, [&] {
         set_device(c);
         sync(c);
-        for (auto &kv : c->scratch) dev_free(c, kv.second.ptr, kv.second.bytes);
-        c->scratch.clear();
+        drop_scratch(c);
     });
 }
+
+int apspie_ctx_reserve_arena(apspie_ctx *c, int64_t bytes) {
+    if (!c || bytes < 0) return APSPIE_ERR_INVALID;
+    return guard(c, [&] {
+        set_device(c);
+        sync(c);
+        drop_scratch(c);
+        if (c->arena) {
+            dev_free(c, c->arena, c->arena_bytes);
+            c->arena = nullptr;
+            c->arena_bytes = 0;
+            c->arena_free.clear();
+        }
+        if (bytes == 0) return;
+        // whole granules, rounded down: a budget-sized arena stays within the budget
+        const size_t want = (size_t)bytes / kGranule * kGranule;
+        APSPIE_CHECK(want > 0, APSPIE_ERR_INVALID, "arena smaller than 256 bytes");
+        c->arena = (char *)dev_alloc(c, want);  // within the HBM budget
+        c->arena_bytes = want;
+        c->arena_free.emplace(0, want);
+    });
+}
+
+int64_t apspie_ctx_arena_bytes(apspie_ctx *c) { return c ? (int64_t)c->arena_bytes : -1; }
 
 const char *apspie_last_error(apspie_ctx *c) { return c ? c->last_error.c_str() : "no context"; }
 

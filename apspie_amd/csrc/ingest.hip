@@ -249,8 +249,8 @@ static void check_symmetry(Ctx *c, Graph *g) {
         g->sym_weights = 0;
         return;
     }
-    Graph t;
-    graph_transpose_dev(c, g, &t);
+    Graph t;  // a temporary in the context's scratch
+    graph_transpose_dev(c, g, &t, "sym_t");
     int32_t *flag = (int32_t *)scratch(c, "sym_flag", 256);
     fill_i32(c, flag, 0, 2);
     const int grid = fill_grid(c, kBlock, 4);
@@ -263,9 +263,6 @@ static void check_symmetry(Ctx *c, Graph *g) {
     int32_t h[2] = {1, 1};
     APSPIE_HIP(hipMemcpyAsync(h, flag, 8, hipMemcpyDeviceToHost, c->stream));
     sync(c);
-    dev_free(c, t.rowptr, sizeof(int32_t) * (t.m + 1));
-    dev_free(c, t.col, sizeof(int32_t) * std::max<int64_t>(t.nnz, 1));
-    if (t.val) dev_free(c, t.val, sizeof(float) * std::max<int64_t>(t.nnz, 1));
     g->symmetric = h[0] == 0 ? 1 : 0;
     g->sym_weights = (g->symmetric == 1 && g->val && h[1] == 0) ? 1 : 0;
 }
@@ -527,7 +524,7 @@ bool graph_is_symmetric_weighted(Ctx *c, Graph *g) {
     return g->sym_weights == 1;
 }
 
-void graph_transpose_dev(Ctx *c, const Graph *g, Graph *t) {
+void graph_transpose_dev(Ctx *c, const Graph *g, Graph *t, const char *in_scratch) {
     const int grid = fill_grid(c, kBlock, 4);
     const int64_t E = g->nnz, Ea = std::max<int64_t>(E, 1);
     t->owner = c;
@@ -535,7 +532,7 @@ void graph_transpose_dev(Ctx *c, const Graph *g, Graph *t) {
     t->m = g->n;
     t->n = g->m;
     t->nnz = E;
-    t->owns = true;
+    t->owns = !in_scratch;
     int32_t *rows = (int32_t *)scratch(c, "tr_rows", sizeof(int32_t) * Ea);
     uint32_t *keys = (uint32_t *)scratch(c, "tr_k", sizeof(uint32_t) * Ea);
     uint32_t *idx = (uint32_t *)scratch(c, "tr_i", sizeof(uint32_t) * Ea);
@@ -545,9 +542,16 @@ void graph_transpose_dev(Ctx *c, const Graph *g, Graph *t) {
         APSPIE_LAUNCH_CHECK();
         radix_sort_pairs(c, keys, idx, E, bits_for(g->n));  // stable: rows stay ascending
     }
-    t->rowptr = (int32_t *)dev_alloc(c, sizeof(int32_t) * (t->m + 1));
-    t->col = (int32_t *)dev_alloc(c, sizeof(int32_t) * Ea);
-    t->val = g->val ? (float *)dev_alloc(c, sizeof(float) * Ea) : nullptr;
+    if (in_scratch) {
+        const std::string p(in_scratch);
+        t->rowptr = (int32_t *)scratch(c, p + "_rowptr", sizeof(int32_t) * (t->m + 1));
+        t->col = (int32_t *)scratch(c, p + "_col", sizeof(int32_t) * Ea);
+        t->val = g->val ? (float *)scratch(c, p + "_val", sizeof(float) * Ea) : nullptr;
+    } else {
+        t->rowptr = (int32_t *)dev_alloc(c, sizeof(int32_t) * (t->m + 1));
+        t->col = (int32_t *)dev_alloc(c, sizeof(int32_t) * Ea);
+        t->val = g->val ? (float *)dev_alloc(c, sizeof(float) * Ea) : nullptr;
+    }
     t->bytes = (int64_t)(sizeof(int32_t) * (t->m + 1) + (g->val ? 8 : 4) * Ea);
     int32_t *orow = (int32_t *)scratch(c, "tr_orow", sizeof(int32_t) * Ea);
     if (E > 0) {

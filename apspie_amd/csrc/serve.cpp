@@ -93,8 +93,8 @@ struct Request {
 // Work done per (graph, kind): batches launched, queries answered and the
 // wall time workers spent on them (launch to results in host memory).
 struct Busy {
-    int64_t batches = 0, queries = 0;
-    double busy_ms = 0, traverse_ms = 0, copy_ms = 0, alloc_ms = 0;
+    int64_t batches = 0, queries = 0, retries = 0;
+    double busy_ms = 0, traverse_ms = 0, copy_ms = 0, alloc_ms = 0, retry_ms = 0;
 };
 
 // wall time of the phases of one batch
@@ -239,10 +239,21 @@ void execute(apspie_server *s, apspie_ctx *ctx, std::vector<std::shared_ptr<Requ
     const auto t0 = Clock::now();
     Phases ph;
     int st = run_batch(s, ctx, batch, ph);
+    int64_t retries = 0;
+    double retry_ms = 0;  // the failed attempt and the trim after it
     if (st == APSPIE_ERR_BUDGET || st == APSPIE_ERR_OOM) {
         apspie_ctx_trim(ctx);
+        retries = 1;
+        retry_ms = ms_since(t0);
+        ph = Phases{};
         st = run_batch(s, ctx, batch, ph);
         if ((st == APSPIE_ERR_BUDGET || st == APSPIE_ERR_OOM) && batch.size() > 1) {
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                Busy &b = s->graphs[batch[0]->graph].busy[batch[0]->kind];
+                b.retries += 2;
+                b.retry_ms += ms_since(t0);
+            }
             const size_t n = batch.size();
             std::vector<std::shared_ptr<Request>> lo(batch.begin(), batch.begin() + n / 2),
                 hi(batch.begin() + n / 2, batch.end());
@@ -265,6 +276,8 @@ void execute(apspie_server *s, apspie_ctx *ctx, std::vector<std::shared_ptr<Requ
     b.traverse_ms += ph.traverse_ms;
     b.copy_ms += ph.copy_ms;
     b.alloc_ms += ph.alloc_ms;
+    b.retries += retries;
+    b.retry_ms += retry_ms;
     s->cv_done.notify_all();
 }
 
@@ -305,7 +318,15 @@ void worker(apspie_server *s, int slot) {
             ++s->busy;
         }
         if (budget != applied) {
+            // the worker's scratch comes from one arena of its whole budget:
+            // a batch that needs other buffers than the last then reuses the
+            // arena's ranges instead of hipFree (which would wait for the
+            // other workers' streams); without the memory for it the worker
+            // keeps per-buffer allocations
+            apspie_ctx_reserve_arena(ctx, 0);
             apspie_ctx_set_hbm_budget(ctx, budget);
+            if (budget > 0 && apspie_ctx_reserve_arena(ctx, budget) != APSPIE_OK)
+                apspie_ctx_reserve_arena(ctx, 0);
             applied = budget;
         }
         execute(s, ctx, batch);
@@ -378,7 +399,10 @@ bool fits(apspie_server *s, int64_t bytes) {
 }
 
 // Admit a graph the admin context has just built: check the budget with its
-// actual size, prepare what TC needs, and publish it to the pool.
+// actual size, build what the traversals and TC derive from a graph (kept
+// with it: longest row, vertices without entries, symmetry and components,
+// value range, rows below the diagonal) so that the workers, whose scratch
+// fills their budget, never build it mid-batch, and publish it to the pool.
 int admit(apspie_server *s, apspie_graph *g, int32_t *graph_id) {
     int32_t m = 0;
     int64_t bytes = 0;
@@ -387,7 +411,19 @@ int admit(apspie_server *s, apspie_graph *g, int32_t *graph_id) {
         apspie_graph_destroy(g);
         return APSPIE_ERR_BUDGET;
     }
-    int st = apspie::guard(s->admin, [&] { apspie::graph_prepare_tc(s->admin, g); });
+    int st = apspie::guard(s->admin, [&] {
+        apspie::graph_max_degree(s->admin, g);
+        apspie::graph_iso_bits(s->admin, g);
+        if (apspie::graph_is_symmetric(s->admin, g)) apspie::graph_components(s->admin, g);
+        float lo = 0, hi = 0;
+        apspie::graph_weight_range(s->admin, g, &lo, &hi);
+        apspie_ctx_trim(s->admin);  // the admin's temporaries
+    });
+    if (st != APSPIE_OK) {
+        apspie_graph_destroy(g);
+        return st;
+    }
+    st = apspie::guard(s->admin, [&] { apspie::graph_prepare_tc(s->admin, g); });
     if (st != APSPIE_OK && st != APSPIE_ERR_INVALID) {  // unsorted rows: TC fails later
         apspie_graph_destroy(g);
         return st;
@@ -572,8 +608,8 @@ int apspie_server_busy_stats(apspie_server *s, int32_t graph_id, int kind,
     std::lock_guard<std::mutex> lk(s->mu);
     if (graph_id < 0 || graph_id >= (int32_t)s->graphs.size()) return APSPIE_ERR_INVALID;
     Busy &b = s->graphs[graph_id].busy[kind];
-    *out = apspie_server_stats{b.batches, b.queries, b.busy_ms, b.traverse_ms, b.copy_ms,
-                               b.alloc_ms};
+    *out = apspie_server_stats{b.batches, b.queries,   b.busy_ms, b.traverse_ms,
+                               b.copy_ms, b.alloc_ms, b.retries, b.retry_ms};
     if (reset) b = Busy{};
     return APSPIE_OK;
 }

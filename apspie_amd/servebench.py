@@ -143,8 +143,8 @@ def run_stream(server, ids, graphs, stream, rate, seed=11, digests=False, arena=
 
 def busy_breakdown(server, ids, graphs):
     """Busy time per (graph, kind) since the last reset, largest first, with
-    its phases (traversal / count calls, result copies to host, pinned
-    buffers obtained)."""
+    its phases (traversal / count calls, result copies, pinned buffers
+    obtained, attempts that ran out of budget and the trims after them)."""
     rows = []
     for gi, g in enumerate(graphs):
         for kind in (Q_BFS, Q_SSSP, Q_TC):
@@ -155,7 +155,8 @@ def busy_breakdown(server, ids, graphs):
                              "queries": q, "busy_ms": st["busy_ms"],
                              "ms_per_query": st["busy_ms"] / q,
                              "traverse_ms": st["traverse_ms"], "copy_ms": st["copy_ms"],
-                             "alloc_ms": st["alloc_ms"]})
+                             "alloc_ms": st["alloc_ms"], "retries": st["retries"],
+                             "retry_ms": st["retry_ms"]})
     total = sum(r["busy_ms"] for r in rows) or 1.0
     for r in rows:
         r["share"] = r["busy_ms"] / total

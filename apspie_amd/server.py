@@ -137,7 +137,9 @@ class Server:
 
     def busy_stats(self, graph_id, kind, reset=False):
         """The busy counters split by phase: dict of batches, queries, busy_ms,
-        traverse_ms, copy_ms, alloc_ms."""
+        traverse_ms, copy_ms, alloc_ms, and the attempts that ran out of
+        budget or memory (retries, retry_ms: their wall time and the trims
+        after them)."""
         st = _ServerStats()
         _check(self._L.apspie_server_busy_stats(self.h, graph_id, kind, C.byref(st), int(reset)))
         return {k: getattr(st, k) for k, _ in _ServerStats._fields_}

@@ -98,8 +98,21 @@ int64_t apspie_ctx_hbm_used(apspie_ctx *ctx);
  * apspie_ctx_get_option for the list of names. */
 int apspie_ctx_set_option(apspie_ctx *ctx, const char *name, int64_t value);
 int apspie_ctx_get_option(apspie_ctx *ctx, const char *name, int64_t *value);
-/* Release cached scratch memory. */
+/* Release cached scratch memory (with an arena: mark it all free again). */
 int apspie_ctx_trim(apspie_ctx *ctx);
+/* Carve the context's scratch buffers out of one device block of `bytes`
+ * (counted against the HBM budget) instead of one hipMalloc each: growing or
+ * dropping a buffer then synchronises only the context's stream, where a
+ * hipFree waits for every stream on the device -- what a server with one
+ * context per worker needs.  A buffer that no longer fits the arena's largest
+ * free range fails with APSPIE_ERR_BUDGET (apspie_ctx_trim, then call again).
+ * The block is `bytes` rounded down to a multiple of 256 (at least 256).
+ * 0 releases the arena; every call drops the cached scratch.  The reference's
+ * callers allocate one workspace up front and hand it to every call
+ * (d_scratch from allocScratch, scratch.hpp:5-32, util.cuh:646-672); the
+ * arena is that workspace kept inside the context. */
+int apspie_ctx_reserve_arena(apspie_ctx *ctx, int64_t bytes);
+int64_t apspie_ctx_arena_bytes(apspie_ctx *ctx);
 const char *apspie_last_error(apspie_ctx *ctx);
 const char *apspie_status_str(int status);
 int apspie_abi_version(void);
@@ -391,6 +404,8 @@ int apspie_server_busy(apspie_server *s, int32_t graph_id, int kind, int64_t *ba
 typedef struct {
     int64_t batches, queries;
     double busy_ms, traverse_ms, copy_ms, alloc_ms;
+    int64_t retries;  /* attempts that ran out of budget or memory */
+    double retry_ms;  /* wall time of those attempts and of the trims after them */
 } apspie_server_stats;
 int apspie_server_busy_stats(apspie_server *s, int32_t graph_id, int kind,
                              apspie_server_stats *out, int reset);

@@ -415,3 +415,45 @@ def test_bfs_batch_sources_in_small_components(ctx, group):
     finally:
         ctx.set_option("batch_long_row_group", 128)
         G.close()
+
+
+def test_context_arena_matches_pool():
+    """Scratch carved from an arena gives the results of per-buffer
+    allocations, over repeated calls that reuse its ranges; the arena counts
+    against the HBM budget; a buffer beyond its largest free range is a
+    budget error; 0 releases it and calls allocate per buffer again."""
+    g = T.rmat(12)
+    src = T.sources(g, 40, seed=5)
+    want_bfs = [O.bfs(g.m, g.rowptr, g.col, s)[0] for s in src]
+    want_sssp = [oracle_sssp(g, s) for s in src[:20]]
+    want_tri = O.tc(g.m, g.rowptr, g.col)[0]
+    with A.Context(0) as c:
+        G = c.upload(g.rowptr32, g.col, g.w)
+        used0 = c.hbm_used()
+        c.reserve_arena((64 << 20) + 100)  # rounded down to whole 256-byte granules
+        assert c.arena_bytes() == 64 << 20
+        assert c.hbm_used() == used0 + (64 << 20)
+        for _ in range(2):
+            for i, s in enumerate(src[:4]):
+                assert np.array_equal(c.bfs(G, s), want_bfs[i]), s
+                assert np.array_equal(c.sssp(G, s), want_sssp[i]), s
+            got = c.bfs_batch(G, src)
+            for i in range(len(src)):
+                assert np.array_equal(got[i], want_bfs[i]), i
+            got = c.sssp_batch(G, src[:20])
+            for i in range(20):
+                assert np.array_equal(got[i], want_sssp[i]), i
+            assert c.tc(G, per_row=False)[0] == want_tri
+        # 40 sources of SSSP hold a 4096 x 64 distance matrix (1 MiB) alone
+        c.reserve_arena(1 << 20)
+        with pytest.raises(A.ApspieError) as e:
+            c.sssp_batch(G, src)
+        assert e.value.status == -4  # APSPIE_ERR_BUDGET
+        c.trim()
+        c.reserve_arena(0)
+        assert c.arena_bytes() == 0
+        assert c.hbm_used() == used0
+        got = c.sssp_batch(G, src)
+        for i in range(20):
+            assert np.array_equal(got[i], want_sssp[i]), i
+        G.close()
