@@ -111,7 +111,7 @@ def test_rmat22_bfs_batch_equals_plain(ctx, rmat22):
 
 @pytest.mark.timeout(600)
 def test_rmat22_sssp_every_source(ctx, rmat22):
-    """All 64 sources (the benchmark times the first 16): golden hashes where
+    """All 64 sources (the benchmark times all of them): golden hashes where
     they exist, the exact invariant check for every source."""
     rp, rp64, col, w, G = rmat22
     g = GF["rmat22"]

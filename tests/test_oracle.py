@@ -584,3 +584,29 @@ def test_mxv_dense_hand_cases():
     y, t = O.mxv_dense(2, 2, rp, col, val, np.array([0, 1], np.float32),
                        np.array([4, 5], np.float32))
     assert y.tolist() == [0.0, 3.0] and t == 1
+
+
+@pytest.mark.parametrize("n,p,seed", [(2000, 0.0005, 1), (3000, 0.0008, 13), (500, 0.01, 4),
+                                      (1, 0.0, 0)])
+def test_components_vs_scipy(n, p, seed):
+    """Union-find labels (smallest vertex id of each component) against
+    scipy's connected components relabelled the same way, on graphs with many
+    small components and isolated vertices; labels by the largest id of each
+    component (what hooking the smaller root under the larger would give)
+    must not match."""
+    from tests import graphs as T
+    g = T.random_graph(n, p, seed)
+    a = sp.csr_matrix((np.ones(len(g.col)), g.col, g.rowptr), shape=(g.m, g.m))
+    _, lab = csg.connected_components(a, directed=False)
+    want = np.empty(g.m, np.int64)
+    for c in np.unique(lab):
+        members = np.nonzero(lab == c)[0]
+        want[members] = members.min()
+    got = O.components(g.m, g.rowptr, g.col)
+    assert np.array_equal(got, want)
+    if g.m > 1 and len(np.unique(lab)) < g.m:
+        wrong = want.copy()
+        for c in np.unique(lab):
+            members = np.nonzero(lab == c)[0]
+            wrong[members] = members.max()
+        assert not np.array_equal(got, wrong)
