@@ -19,6 +19,8 @@ static const char *kOptionNames[] = {
                          // -1 = chosen per graph from its weights and mean row length)
     "sssp_pull_div",     // pull an SSSP round when frontier edges * this > nnz (0 = never;
                          // symmetric weights only)
+    "sssp_light_heavy",  // near-far: heavy entries (value >= width) once per emptied
+                         // frontier instead of with every improvement (1 = on)
     "spmspv_path",       // 0 auto, 1 sort, 2 atomic
     "spmspv_sort_max",   // auto: use the sort path up to this many expanded edges
     "batch_blocks_per_cu",
@@ -54,6 +56,7 @@ static bool option_default(const std::string &n, int64_t *v) {
         {"bfs_beta", 24},
         {"sssp_delta", -1},
         {"sssp_pull_div", 0},
+        {"sssp_light_heavy", 1},
         {"spmspv_path", 0},
         {"spmspv_sort_max", 1 << 16},
         {"batch_blocks_per_cu", 4},
@@ -379,6 +382,7 @@ int apspie_ctx_set_option(apspie_ctx *c, const char *name, int64_t v) {
     if (n == "batch_sssp_delta" && (v < -1 || v > (1 << 30))) return APSPIE_ERR_INVALID;
     if (n == "sssp_delta" && (v < -1 || v > (1 << 30))) return APSPIE_ERR_INVALID;
     if (n == "sssp_pull_div" && (v < 0 || v > (1 << 20))) return APSPIE_ERR_INVALID;
+    if (n == "sssp_light_heavy" && (v < 0 || v > 1)) return APSPIE_ERR_INVALID;
     if (n == "profiling") c->profiling = v != 0;
     c->options[n] = v;
     return APSPIE_OK;
@@ -594,6 +598,7 @@ int apspie_graph_destroy(apspie_graph *g) {
     if (g->iso_bits) (void)hipFree(g->iso_bits);
     if (g->comp) (void)hipFree(g->comp);
     if (g->heavy_mem) (void)hipFree(g->heavy_mem);
+    if (g->lf_mem) (void)hipFree(g->lf_mem);
     delete g;
     return APSPIE_OK;
 }

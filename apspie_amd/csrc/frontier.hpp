@@ -20,6 +20,7 @@
 #pragma once
 
 #include <type_traits>
+#include <utility>
 
 #include "common.hpp"
 #include "device.hpp"
@@ -109,6 +110,20 @@ template <class Op>
 struct streams_columns<Op, std::void_t<decltype(Op::kStreamColumns)>>
     : std::bool_constant<Op::kStreamColumns> {};
 
+// An Op may supply `__device__ int32_t row_end(int32_t v) const`: the end of
+// the part of v's row that an appended entry covers (a prefix of the row,
+// e.g. its light entries); without it the whole row, rowptr[v + 1].
+template <class Op, class = void>
+struct has_row_end : std::false_type {};
+template <class Op>
+struct has_row_end<Op, std::void_t<decltype(std::declval<const Op &>().row_end(0))>>
+    : std::true_type {};
+template <class Op>
+__device__ __forceinline__ int32_t row_end_of(const Op &op, const int32_t *rowptr, int32_t v) {
+    if constexpr (has_row_end<Op>::value) return op.row_end(v);
+    else return rowptr[v + 1];
+}
+
 // prefetch loads the row extent of every edge's endpoint together with the
 // column instead of after a successful claim: one dependent round trip less
 // per level for 8 extra bytes per edge, worth it on small (latency-bound)
@@ -156,12 +171,12 @@ __device__ __forceinline__ void expand_tiles(ExpandLds<BLOCK, TILE, STORE_V_IN> 
                 int32_t prs = 0, pre = 0;
                 if (PRODUCE && prefetch) {
                     prs = rowptr[v];
-                    pre = rowptr[v + 1];
+                    pre = row_end_of(op, rowptr, v);
                 }
                 if (op.edge(u, idx, v) && PRODUCE) {
                     if (!prefetch) {
                         prs = rowptr[v];
-                        pre = rowptr[v + 1];
+                        pre = row_end_of(op, rowptr, v);
                     }
                     cv[k] = v;
                     crs[k] = prs;
@@ -274,7 +289,15 @@ __device__ __forceinline__ void nf_reset(uint64_t *ctl, int r) {
     for (int i = 0; i < kSpread; ++i) {
         ctl[16 + i * 16 + r % 3] = 0;
         ctl[16 + i * 16 + 3 + r % 3] = ~0ull;
+        ctl[16 + i * 16 + 6 + r % 3] = 0;
     }
+}
+// the pending vertices the split of round r moved below the new threshold
+// (slot 6 + r % 3 of the spread rows)
+__host__ __device__ inline uint64_t nf_moved(const uint64_t *ctl, int r) {
+    uint64_t t = 0;
+    for (int i = 0; i < kSpread; ++i) t += ctl[16 + i * 16 + 6 + r % 3];
+    return t;
 }
 // one block's share of a split's result (called by one thread)
 __device__ __forceinline__ void nf_report(uint64_t *ctl, int r, uint64_t left, uint32_t min_bits) {
@@ -284,6 +307,9 @@ __device__ __forceinline__ void nf_report(uint64_t *ctl, int r, uint64_t left, u
         __hip_atomic_fetch_min(row + 3 + r % 3, (uint64_t)min_bits, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
     }
+}
+__device__ __forceinline__ void nf_report_moved(uint64_t *ctl, int r, uint64_t moved) {
+    if (moved) dev::atomic_add_dev(ctl + 16 + (blockIdx.x % kSpread) * 16 + 6 + r % 3, moved);
 }
 // The step of a split: the width delta at first, doubled by every split that
 // follows a split that expanded nothing (the bucket [T, T + step) was empty:

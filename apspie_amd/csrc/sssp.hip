@@ -374,13 +374,28 @@ void sssp_do(Ctx *c, const Graph *g0, int32_t source, float *dist_out, int32_t p
 // min, the next bitmap's OR as the claim); a split's sweep meets every vertex
 // once and appends without a claim.
 //
+// Light / heavy mode (delta-stepping's split of the edges, Meyer and
+// Sanders): each row holds its light entries (value below delta) first, in
+// a copy of the CSR kept with the graph.  A vertex that enters the frontier
+// (claimed, or moved by a split) expands only its light entries and is
+// marked; a round that finds the frontier empty after an expansion, or
+// after a split that moved vertices, is a heavy sweep: the marked vertices'
+// heavy entries become the next frontier and the marks clear.  Other empty
+// rounds split.  A vertex is re-expanded with every improvement in its
+// bucket, but its heavy entries, whose relaxations land at or above T
+// whenever the step is delta, only once per emptied frontier with its
+// distance by then (R-MAT-22, width 64: 273M -> 165M expanded entries per
+// source).  The fixed point is the same: every vertex whose distance drops
+// is marked and reaches a sweep before the rounds end.
+//
 // ctl words (every word a round reads was written by the previous round, and
 // no word a round reads is written in that round): [0..2] frontier counter
 // ring, [4..7] stats, [10 + (r & 1)] the split step of round r (float bits),
-// [12 + (r & 1)] T of round r (float bits), [14 + (r & 1)]
-// 1 if round r - 1 was a split, and, in the spread slots, the pending-vertex
-// count left by the split of round r ([16 + 16 i + r % 3]) and the smallest
-// distance among them ([16 + 16 i + 3 + r % 3], float bits, ~0 if none).
+// [12 + (r & 1)] T of round r (float bits), [14 + (r & 1)] what round r - 1
+// was (0 an expansion, 1 a split, 2 a heavy sweep), and, in the spread
+// slots, the pending-vertex count left by the split of round r ([16 + 16 i
+// + r % 3]), the smallest distance among them ([16 + 16 i + 3 + r % 3],
+// float bits, ~0 if none) and the count it moved ([16 + 16 i + 6 + r % 3]).
 //
 // The threshold of a split is T + step, at least the next float above T (T +
 // step rounds to T once T exceeds step * 2^24); the step starts at delta.  A
@@ -400,6 +415,9 @@ struct SsspNearOp {
     uint32_t *fb_next;
     uint8_t *pend;
     int32_t tbits;  // threshold T (float bits; distances are non-negative)
+    const int32_t *rend;  // end of the row part a claimed vertex expands next
+    uint32_t *hb;         // light / heavy mode: bit per vertex whose heavy entries are due
+    __device__ __forceinline__ int32_t row_end(int32_t v) const { return rend[v]; }
     __device__ __forceinline__ bool edge(int32_t u, int32_t e, int32_t v) const {
         const int32_t nd = dev::fbits(dev::bitsf(dist[u]) + __builtin_nontemporal_load(w + e));
         if (nd >= dist[v]) return false;
@@ -408,7 +426,11 @@ struct SsspNearOp {
             pend[v] = 1;
             return false;
         }
-        return claim_bit(fb_next, v);
+        if (!claim_bit(fb_next, v)) return false;
+        // its light entries come next; the heavy ones once the frontier empties
+        if (hb) __hip_atomic_fetch_or(hb + (v >> 5), 1u << (v & 31), __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+        return true;
     }
 };
 
@@ -418,16 +440,24 @@ __global__ __launch_bounds__(kNfBlock) __attribute__((amdgpu_waves_per_eu(8))) v
 k_sssp_nf_round(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
                 const float *w, int32_t nnz, int32_t m, int32_t *dist, uint32_t *fb_next,
                 uint32_t *fb_clear, uint8_t *pend, FrontierBufs cur,
-                FrontierBufs nxt, uint64_t *ctl, int32_t r) {
+                FrontierBufs nxt, uint64_t *ctl, int32_t r, const int32_t *rend,
+                uint32_t *hb) {
     __shared__ ExpandLds<kNfBlock, TILE, true> lds;
-    __shared__ uint64_t s_part[kNfBlock / 64];
+    __shared__ uint64_t s_part[kNfBlock / 64], s_moved[kNfBlock / 64];
     __shared__ uint32_t s_min[kNfBlock / 64];
     const uint64_t cc = ctl[r % 3];
     const int32_t F = (int32_t)cnt_entries(cc), E = (int32_t)cnt_edges(cc);
     const int32_t tbits = (int32_t)ctl[12 + (r & 1)];
-    const bool split_last = ctl[14 + (r & 1)] != 0;
+    // what round r - 1 was: 0 an expansion, 1 a split, 2 a heavy sweep
+    const uint64_t prev = ctl[14 + (r & 1)];
+    const bool split_last = prev == 1;
     const uint64_t left_before = nf_left(ctl, r + 2);  // left by the previous round's split
-    const bool split = F == 0;
+    // light / heavy mode: an empty frontier after an expansion, or after a
+    // split that moved vertices, first expands the heavy entries of the
+    // vertices marked since the last heavy sweep
+    const bool hsweep =
+        F == 0 && hb && (prev == 0 || (prev == 1 && nf_moved(ctl, r + 2) > 0));
+    const bool split = F == 0 && !hsweep;
     const bool done = split && split_last && left_before == 0;
     const uint64_t min_before = nf_min_left(ctl, r + 2);
     const bool jump = split && !done && split_last && min_before != ~0ull;
@@ -439,7 +469,7 @@ k_sssp_nf_round(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ 
         ctl[(r + 2) % 3] = 0;
         ctl[10 + ((r + 1) & 1)] = (uint64_t)(uint32_t)dev::fbits(step);
         ctl[12 + ((r + 1) & 1)] = (uint64_t)(uint32_t)tnext;
-        ctl[14 + ((r + 1) & 1)] = split ? 1 : 0;
+        ctl[14 + ((r + 1) & 1)] = split ? 1 : (hsweep ? 2 : 0);
         nf_reset(ctl, r + 1);
         if (F > 0) {
             ctl[4] += (uint64_t)E;
@@ -455,14 +485,43 @@ k_sssp_nf_round(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ 
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m / 32 + 1;
          i += (int64_t)gridDim.x * blockDim.x)
         fb_clear[i] = 0;
-    if (!split) {
-        SsspNearOp op{w, dist, fb_next, pend, tbits};
+    if (F > 0) {
+        SsspNearOp op{w, dist, fb_next, pend, tbits, rend, hb};
         expand_tiles<kNfBlock, TILE, true, true, true>(lds, op, rowptr, col, nnz, cur, F, E, nxt,
                                                        ctl + ((r + 1) % 3));
         return;
     }
+    if (hsweep) {
+        // the marked vertices' heavy entries [rend[v], rowptr[v + 1]) form the
+        // next frontier; the 32 vertices of a bitmap word sit in 32
+        // consecutive lanes, and the word's first lane clears it after the
+        // wave has read it (no round marks while this one runs)
+        for (int64_t pbase = (int64_t)blockIdx.x * kNfBlock * kNfPer; pbase < m;
+             pbase += (int64_t)gridDim.x * kNfBlock * kNfPer) {
+            int32_t cv[kNfPer], crs[kNfPer], cd[kNfPer];
+#pragma unroll
+            for (int k = 0; k < kNfPer; ++k) {
+                const int64_t v = pbase + (int64_t)k * kNfBlock + threadIdx.x;
+                cv[k] = (int32_t)v;
+                crs[k] = 0;
+                cd[k] = 0;
+                if (v < m) {
+                    const uint32_t word = hb[v >> 5];
+                    if ((word >> (v & 31)) & 1u) {
+                        const int32_t le = rend[v];
+                        crs[k] = le;
+                        cd[k] = rowptr[v + 1] - le;
+                    }
+                    if ((v & 31) == 0 && word) hb[v >> 5] = 0;
+                }
+            }
+            produce<kNfBlock, kNfPer, TILE, true>(cv, crs, cd, nxt, ctl + ((r + 1) % 3), lds.scan,
+                                                  &lds.base);
+        }
+        return;
+    }
     // split: move the pending vertices below the new threshold into the frontier
-    uint64_t left = 0;
+    uint64_t left = 0, moved = 0;
     uint32_t min_left = ~0u;  // smallest distance left pending (bits)
     for (int64_t pbase = (int64_t)blockIdx.x * kNfBlock * kNfPer; pbase < m;
          pbase += (int64_t)gridDim.x * kNfBlock * kNfPer) {
@@ -490,28 +549,108 @@ k_sssp_nf_round(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ 
             if (take[k] || drop[k]) pend[cv[k]] = 0;
             if (take[k]) {  // the sweep meets each vertex once: no claim needed
                 crs[k] = rowptr[cv[k]];
-                cd[k] = rowptr[cv[k] + 1] - crs[k];
+                cd[k] = rend[cv[k]] - crs[k];
+                ++moved;
+                if (hb)
+                    __hip_atomic_fetch_or(hb + (cv[k] >> 5), 1u << (cv[k] & 31), __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
             }
         }
         produce<kNfBlock, kNfPer, TILE, true>(cv, crs, cd, nxt, ctl + ((r + 1) % 3), lds.scan,
                                               &lds.base);
     }
     const uint64_t ws = dev::wave_sum_u64(left);
+    const uint64_t wv = dev::wave_sum_u64(moved);
     const uint32_t wm = dev::wave_min_u32(min_left);
     if (dev::lane_id() == 0) {
         s_part[threadIdx.x / 64] = ws;
+        s_moved[threadIdx.x / 64] = wv;
         s_min[threadIdx.x / 64] = wm;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint64_t t = 0;
+        uint64_t t = 0, mv = 0;
         uint32_t mn = ~0u;
         for (int q = 0; q < kNfBlock / 64; ++q) {
             t += s_part[q];
+            mv += s_moved[q];
             mn = min(mn, s_min[q]);
         }
         nf_report(ctl, r, t, mn);
+        nf_report_moved(ctl, r, mv);
     }
+}
+
+// Rows split by value for the light / heavy mode: one wave per row counts
+// its light entries (value below delta), then writes them first and the
+// heavy ones after them, each part in CSR order (ballot ranks), and records
+// where the light part ends.
+constexpr int kLfBlock = 256;
+__global__ __launch_bounds__(kLfBlock) void k_light_first(const int32_t *__restrict__ rowptr,
+                                                          const int32_t *__restrict__ col,
+                                                          const float *__restrict__ val,
+                                                          int32_t m, float delta, int32_t *lcol,
+                                                          float *lval, int32_t *lend) {
+    const int lane = dev::lane_id();
+    const uint64_t below = (1ull << lane) - 1;
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / 64);
+    for (int64_t v = blockIdx.x * (int64_t)(blockDim.x / 64) + threadIdx.x / 64; v < m;
+         v += nwaves) {
+        const int32_t rs = rowptr[v], re = rowptr[v + 1];
+        uint64_t nl = 0;
+        for (int32_t e = rs + lane; e < re; e += 64) nl += val[e] < delta ? 1 : 0;
+        nl = dev::wave_sum_u64(nl);
+        int32_t lpos = rs, hpos = rs + (int32_t)nl;
+        for (int32_t e0 = rs; e0 < re; e0 += 64) {
+            const int32_t e = e0 + lane;
+            const bool in = e < re;
+            const float x = in ? val[e] : 0.0f;
+            const bool light = in && x < delta;
+            const uint64_t bl = __ballot(light), bh = __ballot(in && !light);
+            if (in) {
+                const int32_t p = light ? lpos + __popcll(bl & below) : hpos + __popcll(bh & below);
+                lcol[p] = col[e];
+                lval[p] = x;
+            }
+            lpos += __popcll(bl);
+            hpos += __popcll(bh);
+        }
+        if (lane == 0) lend[v] = rs + (int32_t)nl;
+    }
+}
+
+// The light-first copy of a graph's rows for width delta, built once per
+// width and kept with the graph.
+void graph_light_first(Ctx *c, Graph *g, float delta) {
+    std::lock_guard<std::mutex> lock(g->derived_mu);
+    if (g->lf_mem && g->lf_delta == delta) return;
+    if (g->lf_mem) {  // built for another width: calls that read it have returned
+        APSPIE_HIP(hipDeviceSynchronize());
+        APSPIE_HIP(hipFree(g->lf_mem));
+        g->lf_mem = nullptr;
+        g->lf_delta = -1.0f;
+    }
+    const int64_t E = std::max<int64_t>(g->nnz, 1);
+    const size_t ecol = ((size_t)E * 4 + 255) / 256 * 256;
+    const size_t bytes = 2 * ecol + sizeof(int32_t) * ((size_t)g->m + 1);
+    void *mem = nullptr;
+    // owned by the graph (freed with it), like its other derived arrays
+    APSPIE_HIP(hipMalloc(&mem, bytes));
+    int32_t *lcol = (int32_t *)mem;
+    float *lval = (float *)((char *)mem + ecol);
+    int32_t *lend = (int32_t *)((char *)mem + 2 * ecol);
+    if (g->m > 0) {
+        const int grid = (int)std::min<int64_t>(ceil_div(g->m, kLfBlock / 64), 16 * c->cus);
+        k_light_first<<<grid, kLfBlock, 0, c->stream>>>(g->rowptr, g->col, g->val, g->m, delta,
+                                                        lcol, lval, lend);
+        APSPIE_LAUNCH_CHECK();
+    }
+    sync(c);
+    g->lf_mem = mem;
+    g->lf_col = lcol;
+    g->lf_val = lval;
+    g->lf_end = lend;
+    g->lf_delta = delta;
 }
 
 template <int TILE>
@@ -521,6 +660,27 @@ void sssp_near_far(Ctx *c, const Graph *g, int32_t source, float *dist_out, floa
     const int64_t words = g->m / 32 + 1;
     uint32_t *fb = (uint32_t *)scratch(c, "sssp_fb", sizeof(uint32_t) * 3 * words);
     uint8_t *pend = (uint8_t *)scratch(c, "sssp_pend", (size_t)g->m + 4);
+    // light / heavy mode (option sssp_light_heavy, on by default) when the
+    // weights fall on both sides of delta: a vertex that enters the
+    // frontier expands its light entries (value below delta), and its heavy
+    // ones once per emptied frontier, with its distance by then -- a heavy
+    // relaxation lands at or above the threshold whenever the step is delta,
+    // so re-expanding it with every improvement only repeats far marks
+    float lo = 0.0f, hi = 0.0f;
+    const bool lh = opt(c, "sssp_light_heavy", 1) != 0 &&
+                    graph_weight_range(c, const_cast<Graph *>(g), &lo, &hi) && lo < delta &&
+                    hi >= delta;
+    const int32_t *col = g->col, *rend = g->rowptr + 1;
+    const float *val = g->val;
+    uint32_t *hb = nullptr;
+    if (lh) {
+        graph_light_first(c, const_cast<Graph *>(g), delta);
+        col = g->lf_col;
+        val = g->lf_val;
+        rend = g->lf_end;
+        hb = (uint32_t *)scratch(c, "sssp_hb", sizeof(uint32_t) * words);
+        fill_i32(c, (int32_t *)hb, 0, words);
+    }
     fill_i32(c, dist, __builtin_bit_cast(int32_t, kInf), g->m);
     fill_i32(c, (int32_t *)fb, 0, 3 * words);
     fill_i32(c, (int32_t *)pend, 0, ceil_div(g->m + 4, 4));
@@ -542,9 +702,9 @@ void sssp_near_far(Ctx *c, const Graph *g, int32_t source, float *dist_out, floa
         kt.begin();
         for (int k = 0; k < chunk; ++k, ++r) {
             k_sssp_nf_round<TILE><<<grid, kNfBlock, 0, c->stream>>>(
-                g->rowptr, g->col, g->val, (int32_t)g->nnz, g->m, dist,
+                g->rowptr, col, val, (int32_t)g->nnz, g->m, dist,
                 fb + ((r + 1) % 3) * words, fb + ((r + 2) % 3) * words, pend, fs.f[r & 1],
-                fs.f[(r + 1) & 1], fs.ctl, r);
+                fs.f[(r + 1) & 1], fs.ctl, r, rend, hb);
             APSPIE_LAUNCH_CHECK();
         }
         kt.end(chunk);
@@ -552,9 +712,11 @@ void sssp_near_far(Ctx *c, const Graph *g, int32_t source, float *dist_out, floa
                                   c->stream));
         sync(c);
         c->stats.host_syncs++;
-        // finished when the last round was a split that left nothing pending
-        // and found nothing below the new threshold
-        if (cnt_entries(h[r % 3]) == 0 && h[14 + (r & 1)] != 0 && nf_left(h, r + 2) == 0) break;
+        // finished when the last round was a split that moved nothing and
+        // left nothing pending
+        if (cnt_entries(h[r % 3]) == 0 && h[14 + (r & 1)] == 1 && nf_left(h, r + 2) == 0 &&
+            nf_moved(h, r + 2) == 0)
+            break;
         APSPIE_CHECK(r <= 8 * g->m + 1024, APSPIE_ERR_DEVICE, "SSSP did not converge");
         chunk = grow;
         grow = std::min(grow * 2, 256);

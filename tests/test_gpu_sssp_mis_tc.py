@@ -163,6 +163,44 @@ def test_sssp_pull_needs_symmetric_weights(ctx):
         ctx.set_option("sssp_delta", -1)
 
 
+@pytest.mark.parametrize("lh", [0, 1])
+@pytest.mark.parametrize("wr", [(1, 255), (1, 10 ** 8), (200, 300)])
+@pytest.mark.parametrize("key", ["rmat12", "lattice64", "directed"])
+def test_sssp_light_heavy_vs_oracle(ctx, key, wr, lh):
+    """Near-far rounds with heavy entries deferred to once per emptied
+    frontier (sssp_light_heavy = 1) and without: the oracle's distances for
+    widths that split the weights (some rows all light, some all heavy),
+    with two widths in turn on one graph (its light-first copy rebuilt), on
+    a directed graph and with weights far above the width."""
+    rng = np.random.default_rng(7)
+    if key == "directed":
+        n = 3000
+        u = rng.integers(0, n, 20000)
+        v = rng.integers(0, n, 20000)
+        k = np.unique((u * n + v)[u != v])
+        rowptr, col, _ = O.coo_to_csr(n, (k // n).astype(np.int32), (k % n).astype(np.int32))
+        m = n
+    else:
+        g = T.rmat(12) if key == "rmat12" else T.lattice(64)
+        m, rowptr, col = g.m, g.rowptr, g.col
+    wt = rng.integers(wr[0], wr[1] + 1, len(col)).astype(np.float32)
+    G = ctx.upload(rowptr.astype(np.int32), col, wt)
+    deg = np.diff(rowptr)
+    srcs = [int(s) for s in np.nonzero(deg > 0)[0][[0, -1]]] + [int(np.argmax(deg))]
+    want = {s: to_ref_sentinel(O.sssp(m, rowptr, col, wt, s)) for s in srcs}
+    ctx.set_option("sssp_light_heavy", lh)
+    try:
+        for delta in (int(np.median(wt)), 64, 16):
+            ctx.set_option("sssp_delta", delta)
+            for src in srcs:
+                got = ctx.sssp(G, src)
+                assert np.array_equal(got, want[src]), (key, wr, lh, delta, src)
+    finally:
+        ctx.set_option("sssp_light_heavy", 1)
+        ctx.set_option("sssp_delta", -1)
+    G.close()
+
+
 def test_sssp_near_far_edge_cases(ctx):
     ctx.set_option("sssp_delta", 16)
     try:
