@@ -749,13 +749,21 @@ double sssp_weight_delta(Ctx *c, Graph *g) {
     return d;
 }
 
-// The width used when sssp_delta is left at -1: Bellman-Ford rounds (0)
-// below 2^22 vertices, where the rounds' fixed cost outweighs the
-// re-expansions near-far saves (measured at weights 1..255: R-MAT 16-21 and
-// lattices 512^2-1024^2 run as fast or faster without splits), else the
-// width the weights suggest.
+// The width used when sssp_delta is left at -1: the width the weights
+// suggest from 2^22 vertices on, and from 2^20 on where that width splits
+// the weights into light and heavy entries (light / heavy rounds on);
+// Bellman-Ford rounds (0) otherwise, where the rounds' fixed cost outweighs
+// the re-expansions near-far saves.  Measured at weights 1..255, ms per
+// source, Bellman-Ford / width 64 with light / heavy rounds: R-MAT-18
+// 0.58 / 0.94, R-MAT-20 1.42 / 1.33, R-MAT-21 2.21 / 1.91, R-MAT-22 4.03 /
+// 2.90; the lattices' width (512) is above every weight, and 512^2-1024^2
+// run faster without splits.
 double sssp_auto_delta(Ctx *c, Graph *g) {
-    return g->m < (1 << 22) ? 0 : sssp_weight_delta(c, g);
+    if (g->m >= (1 << 22)) return sssp_weight_delta(c, g);
+    if (g->m < (1 << 20) || opt(c, "sssp_light_heavy", 1) == 0) return 0;
+    const double d = sssp_weight_delta(c, g);
+    float lo = 0.0f, hi = 0.0f;
+    return d > 0 && graph_weight_range(c, g, &lo, &hi) && lo < d && hi >= d ? d : 0;
 }
 
 // The batched SSSP width when batch_sssp_delta is left at -1: the weights'
