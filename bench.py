@@ -319,11 +319,13 @@ def sssp_rmat22(ctx, G, rowptr, col, srcs, optr, cpu):
     Gw = ctx.upload(rowptr, col, w)
     fout = ctx.dev_alloc(4 * m)
     edges, relaxed, t, alg = 0, 0, 0.0, 0.0
-    for s in srcs:
-        ctx.bfs(G, s, out=optr, mem=A.DEVICE)  # same reached set -> E_r, n_r
+    for s in srcs:  # the same reached set as SSSP's -> E_r, n_r per source
+        ctx.bfs(G, s, out=optr, mem=A.DEVICE)
         st = ctx.stats()
         edges += st["edges_traversed"]
         alg += 12.0 * st["edges_traversed"] + 20.0 * st["vertices_reached"] + 4.0 * m
+    ctx.sssp(Gw, srcs[0], out=fout.ptr, mem=A.DEVICE)  # warm-up (the light-first copy)
+    for s in srcs:  # one query after another on the weighted graph
         ctx.sssp(Gw, s, out=fout.ptr, mem=A.DEVICE)
         st = ctx.stats()
         relaxed += st["edges_traversed"]
