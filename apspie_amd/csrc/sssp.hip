@@ -640,7 +640,9 @@ void graph_light_first(Ctx *c, Graph *g, float delta) {
     float *lval = (float *)((char *)mem + ecol);
     int32_t *lend = (int32_t *)((char *)mem + 2 * ecol);
     if (g->m > 0) {
-        const int grid = (int)std::min<int64_t>(ceil_div(g->m, kLfBlock / 64), 16 * c->cus);
+        // a wave per row and every row its own wave: the two passes over a
+        // row are a few dependent loads, so the grid covers the rows at once
+        const int grid = (int)std::min<int64_t>(ceil_div(g->m, kLfBlock / 64), 1 << 24);
         k_light_first<<<grid, kLfBlock, 0, c->stream>>>(g->rowptr, g->col, g->val, g->m, delta,
                                                         lcol, lval, lend);
         APSPIE_LAUNCH_CHECK();

@@ -210,7 +210,21 @@ extern "C" int apspie_autotune_hash(apspie_ctx *c, const apspie_graph *g) {
             return st;
         };
         // a product too large for one call (more than 2^31 entries, or beyond
-        // the budget) is not tuned
+        // the budget) is not tuned, nor one of more than 2^31 multiplications
+        // (sum over rows of the squared row length: exact for a symmetric
+        // A, the scale otherwise), whose every run would take about a second
+        {
+            std::vector<int32_t> rp((size_t)g->m + 1);
+            copy_from_device(c, rp.data(), g->rowptr, sizeof(int32_t) * rp.size(),
+                             APSPIE_MEM_HOST);
+            sync(c);
+            double prods = 0;
+            for (int32_t v = 0; v < g->m; ++v) {
+                const double d = (double)rp[(size_t)v + 1] - rp[v];
+                prods += d * d;
+            }
+            if (prods > 2147483648.0) return;
+        }
         const int first = product();
         if (first == APSPIE_ERR_OVERFLOW || first == APSPIE_ERR_OOM || first == APSPIE_ERR_BUDGET)
             return;
