@@ -115,6 +115,8 @@ struct Ctx {
     int32_t mis_prio_m = -1;
     // graphs whose memory is accounted to this context
     std::set<Graph *> graphs;
+    // event pairs of the kernel timer (option "profiling"), reused by calls
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> timer_events;
 };
 
 int64_t opt(Ctx *c, const char *name, int64_t dflt);

@@ -333,6 +333,10 @@ int apspie_ctx_destroy(apspie_ctx *c) {
     else
         for (auto &kv : c->scratch)
             if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    for (auto &p : c->timer_events) {
+        (void)hipEventDestroy(p.first);
+        (void)hipEventDestroy(p.second);
+    }
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);

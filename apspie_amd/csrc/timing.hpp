@@ -8,28 +8,30 @@
 
 namespace apspie {
 
+// The event pairs come from a pool kept by the context (created on first
+// use, destroyed with it): a call records into them without creating any.
 class KernelTimer {
   public:
     explicit KernelTimer(Ctx *c, bool allowed = true) : c_(c), on_(allowed && c->profiling) {}
-    ~KernelTimer() {
-        for (auto &p : ev_) {
-            (void)hipEventDestroy(p.first);
-            (void)hipEventDestroy(p.second);
-        }
-    }
     // an interval around `launches` back-to-back launches of the kernel (a
     // span over several launches keeps the events from separating them)
     void begin() {
         if (!on_) return;
-        std::pair<hipEvent_t, hipEvent_t> p;
-        APSPIE_HIP(hipEventCreate(&p.first));
-        APSPIE_HIP(hipEventCreate(&p.second));
-        APSPIE_HIP(hipEventRecord(p.first, c_->stream));
-        ev_.push_back(p);
+        auto &pool = c_->timer_events;
+        if (used_ == pool.size()) {
+            std::pair<hipEvent_t, hipEvent_t> p{nullptr, nullptr};
+            APSPIE_HIP(hipEventCreate(&p.first));
+            const hipError_t e = hipEventCreate(&p.second);
+            if (e != hipSuccess) (void)hipEventDestroy(p.first);
+            APSPIE_HIP(e);
+            pool.push_back(p);
+        }
+        APSPIE_HIP(hipEventRecord(pool[used_].first, c_->stream));
+        ++used_;
     }
     void end(int64_t launches = 1) {
         if (!on_) return;
-        APSPIE_HIP(hipEventRecord(ev_.back().second, c_->stream));
+        APSPIE_HIP(hipEventRecord(c_->timer_events[used_ - 1].second, c_->stream));
         n_.push_back(launches);
     }
     // sums the recorded intervals and their launch counts into the stats
@@ -37,10 +39,11 @@ class KernelTimer {
         if (!on_) return;
         double total = 0;
         int64_t n = 0;
-        for (size_t i = 0; i < ev_.size(); ++i) {
-            APSPIE_HIP(hipEventSynchronize(ev_[i].second));
+        for (size_t i = 0; i < used_; ++i) {
+            const auto &p = c_->timer_events[i];
+            APSPIE_HIP(hipEventSynchronize(p.second));
             float ms = 0;
-            APSPIE_HIP(hipEventElapsedTime(&ms, ev_[i].first, ev_[i].second));
+            APSPIE_HIP(hipEventElapsedTime(&ms, p.first, p.second));
             total += ms;
             n += n_[i];
         }
@@ -51,7 +54,7 @@ class KernelTimer {
   private:
     Ctx *c_;
     bool on_;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_;
+    size_t used_ = 0;
     std::vector<int64_t> n_;
 };
 
