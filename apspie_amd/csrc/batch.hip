@@ -1135,8 +1135,11 @@ void batch_run(Ctx *c, const Graph *g, const int32_t *src, int32_t nsrc, int32_t
         const int32_t k = std::min(group, nsrc - s0);
         int32_t *d = depth ? depth + (int64_t)s0 * g->m : nullptr;
         float *f = dist ? dist + (int64_t)s0 * g->m : nullptr;
+        // the narrowest bit row that holds the pass's sources (W words of 32)
         if (k <= 32) batch_pass<1, SSSP>(c, g, src + s0, k, d, f);
+        else if (k <= 64) batch_pass<2, SSSP>(c, g, src + s0, k, d, f);
         else if (k <= 128) batch_pass<4, SSSP>(c, g, src + s0, k, d, f);
+        else if (k <= 256) batch_pass<8, SSSP>(c, g, src + s0, k, d, f);
         else batch_pass<16, SSSP>(c, g, src + s0, k, d, f);
     }
 }
