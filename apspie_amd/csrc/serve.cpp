@@ -305,7 +305,15 @@ void worker(apspie_server *s, int slot) {
             // worker's budget rather than find out by failing
             if (budget > 0 && head->kind != APSPIE_Q_TC) {
                 const int64_t fit = budget / (3 * head->bytes);
-                cap = (int)std::max<int64_t>(1, std::min<int64_t>(cap, fit));
+                if (fit < cap) {
+                    // a pass runs its sources in bit rows of 32, 64, 128, 256 or
+                    // 512: a batch the budget cuts short is cut to a full row
+                    // (80 sources would run in the 128-wide row at the cost of
+                    // 128; 64 fill the 64-wide one)
+                    int64_t full = fit >= 32 ? 32 : fit;
+                    while (full * 2 <= fit) full *= 2;
+                    cap = (int)std::max<int64_t>(1, full);
+                }
             }
             for (auto it = s->queue.begin(); it != s->queue.end() && (int)batch.size() < cap;) {
                 if ((*it)->graph == head->graph && (*it)->kind == head->kind) {
