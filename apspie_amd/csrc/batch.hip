@@ -1121,14 +1121,16 @@ template <bool SSSP>
 void batch_run(Ctx *c, const Graph *g, const int32_t *src, int32_t nsrc, int32_t *depth,
                float *dist) {
     // BFS on a graph with long rows (low diameter, skewed degrees) runs in
-    // groups of batch_long_row_group sources (default 128): at 512 sources a
-    // pulled vertex waits for all 16 words before it stops scanning (R-MAT-22:
-    // 0.109 / 0.119 / 0.142 ms per query at 512 sources in groups of 128 /
-    // 128 in one group / 512 in one group, plain 0.206)
+    // groups of batch_long_row_group sources (default 256): a pulled vertex
+    // keeps scanning until every word of its row has met a frontier
+    // neighbour, and a few far sources keep a whole group scanning.  R-MAT-22,
+    // 512 sources of the giant component, four source sets: 0.159 / 0.138 /
+    // 0.154 ms per query on average in groups of 128 / 256 / 512 (0.107-0.165
+    // in groups of 256; plain 0.206)
     static_assert(kQueue % kBlock == 0, "queue flushes take whole block rows");
     const int32_t group =
         !SSSP && graph_max_degree(c, const_cast<Graph *>(g)) > 64
-            ? (int32_t)std::min<int64_t>(std::max<int64_t>(opt(c, "batch_long_row_group", 128), 1),
+            ? (int32_t)std::min<int64_t>(std::max<int64_t>(opt(c, "batch_long_row_group", 256), 1),
                                          kMaxBatch)
             : kMaxBatch;
     for (int32_t s0 = 0; s0 < nsrc; s0 += group) {

@@ -64,7 +64,7 @@ static bool option_default(const std::string &n, int64_t *v) {
         {"batch_pull_div", 4},
         {"batch_sssp_delta", -1},
         {"batch_snapshot_mb", 24576},
-        {"batch_long_row_group", 128},
+        {"batch_long_row_group", 256},
         {"batch_sssp_pull_div", 4},
         {"tc_medium_row", 512},
         {"spgemm_wave_ub", 256},

@@ -413,7 +413,7 @@ def test_bfs_batch_sources_in_small_components(ctx, group):
         for i, s in enumerate(srcs):
             assert np.array_equal(got[i], ctx.bfs(G, s)), (group, i, s)
     finally:
-        ctx.set_option("batch_long_row_group", 128)
+        ctx.set_option("batch_long_row_group", 256)
         G.close()
 
 
