@@ -289,15 +289,7 @@ __device__ __forceinline__ void nf_reset(uint64_t *ctl, int r) {
     for (int i = 0; i < kSpread; ++i) {
         ctl[16 + i * 16 + r % 3] = 0;
         ctl[16 + i * 16 + 3 + r % 3] = ~0ull;
-        ctl[16 + i * 16 + 6 + r % 3] = 0;
     }
-}
-// the pending vertices the split of round r moved below the new threshold
-// (slot 6 + r % 3 of the spread rows)
-__host__ __device__ inline uint64_t nf_moved(const uint64_t *ctl, int r) {
-    uint64_t t = 0;
-    for (int i = 0; i < kSpread; ++i) t += ctl[16 + i * 16 + 6 + r % 3];
-    return t;
 }
 // one block's share of a split's result (called by one thread)
 __device__ __forceinline__ void nf_report(uint64_t *ctl, int r, uint64_t left, uint32_t min_bits) {
@@ -307,9 +299,6 @@ __device__ __forceinline__ void nf_report(uint64_t *ctl, int r, uint64_t left, u
         __hip_atomic_fetch_min(row + 3 + r % 3, (uint64_t)min_bits, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
     }
-}
-__device__ __forceinline__ void nf_report_moved(uint64_t *ctl, int r, uint64_t moved) {
-    if (moved) dev::atomic_add_dev(ctl + 16 + (blockIdx.x % kSpread) * 16 + 6 + r % 3, moved);
 }
 // The step of a split: the width delta at first, doubled by every split that
 // follows a split that expanded nothing (the bucket [T, T + step) was empty:

@@ -376,26 +376,24 @@ void sssp_do(Ctx *c, const Graph *g0, int32_t source, float *dist_out, int32_t p
 //
 // Light / heavy mode (delta-stepping's split of the edges, Meyer and
 // Sanders): each row holds its light entries (value below delta) first, in
-// a copy of the CSR kept with the graph.  A vertex that enters the frontier
-// (claimed, or moved by a split) expands only its light entries and is
-// marked; a round that finds the frontier empty after an expansion, or
-// after a split that moved vertices, is a heavy sweep: the marked vertices'
-// heavy entries become the next frontier and the marks clear.  Other empty
-// rounds split.  A vertex is re-expanded with every improvement in its
-// bucket, but its heavy entries, whose relaxations land at or above T
-// whenever the step is delta, only once per emptied frontier with its
-// distance by then (R-MAT-22, width 64: 273M -> 165M expanded entries per
-// source).  The fixed point is the same: every vertex whose distance drops
-// is marked and reaches a sweep before the rounds end.
+// a copy of the CSR kept with the graph.  A vertex claimed for the frontier
+// expands only its light entries and is marked; the next split adds the
+// marked vertices' heavy entries to the frontier (and clears the marks)
+// while it moves the pending vertices, each with its light and its heavy
+// entries.  A vertex is re-expanded with every improvement in its bucket,
+// but its heavy entries, whose relaxations land at or above T whenever the
+// step is delta, only once per bucket, with its distance at the bucket's
+// end (R-MAT-22, width 64: 273M -> 165M expanded entries per source).  The
+// fixed point is the same: every vertex whose distance drops is marked or
+// moved, and a split sweeps every mark before the rounds can end.
 //
 // ctl words (every word a round reads was written by the previous round, and
 // no word a round reads is written in that round): [0..2] frontier counter
 // ring, [4..7] stats, [10 + (r & 1)] the split step of round r (float bits),
-// [12 + (r & 1)] T of round r (float bits), [14 + (r & 1)] what round r - 1
-// was (0 an expansion, 1 a split, 2 a heavy sweep), and, in the spread
-// slots, the pending-vertex count left by the split of round r ([16 + 16 i
-// + r % 3]), the smallest distance among them ([16 + 16 i + 3 + r % 3],
-// float bits, ~0 if none) and the count it moved ([16 + 16 i + 6 + r % 3]).
+// [12 + (r & 1)] T of round r (float bits), [14 + (r & 1)]
+// 1 if round r - 1 was a split, and, in the spread slots, the pending-vertex
+// count left by the split of round r ([16 + 16 i + r % 3]) and the smallest
+// distance among them ([16 + 16 i + 3 + r % 3], float bits, ~0 if none).
 //
 // The threshold of a split is T + step, at least the next float above T (T +
 // step rounds to T once T exceeds step * 2^24); the step starts at delta.  A
@@ -443,21 +441,15 @@ k_sssp_nf_round(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ 
                 FrontierBufs nxt, uint64_t *ctl, int32_t r, const int32_t *rend,
                 uint32_t *hb) {
     __shared__ ExpandLds<kNfBlock, TILE, true> lds;
-    __shared__ uint64_t s_part[kNfBlock / 64], s_moved[kNfBlock / 64];
+    __shared__ uint64_t s_part[kNfBlock / 64];
     __shared__ uint32_t s_min[kNfBlock / 64];
     const uint64_t cc = ctl[r % 3];
     const int32_t F = (int32_t)cnt_entries(cc), E = (int32_t)cnt_edges(cc);
     const int32_t tbits = (int32_t)ctl[12 + (r & 1)];
     // what round r - 1 was: 0 an expansion, 1 a split, 2 a heavy sweep
-    const uint64_t prev = ctl[14 + (r & 1)];
-    const bool split_last = prev == 1;
+    const bool split_last = ctl[14 + (r & 1)] != 0;
     const uint64_t left_before = nf_left(ctl, r + 2);  // left by the previous round's split
-    // light / heavy mode: an empty frontier after an expansion, or after a
-    // split that moved vertices, first expands the heavy entries of the
-    // vertices marked since the last heavy sweep
-    const bool hsweep =
-        F == 0 && hb && (prev == 0 || (prev == 1 && nf_moved(ctl, r + 2) > 0));
-    const bool split = F == 0 && !hsweep;
+    const bool split = F == 0;
     const bool done = split && split_last && left_before == 0;
     const uint64_t min_before = nf_min_left(ctl, r + 2);
     const bool jump = split && !done && split_last && min_before != ~0ull;
@@ -469,7 +461,7 @@ k_sssp_nf_round(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ 
         ctl[(r + 2) % 3] = 0;
         ctl[10 + ((r + 1) & 1)] = (uint64_t)(uint32_t)dev::fbits(step);
         ctl[12 + ((r + 1) & 1)] = (uint64_t)(uint32_t)tnext;
-        ctl[14 + ((r + 1) & 1)] = split ? 1 : (hsweep ? 2 : 0);
+        ctl[14 + ((r + 1) & 1)] = split ? 1 : 0;
         nf_reset(ctl, r + 1);
         if (F > 0) {
             ctl[4] += (uint64_t)E;
@@ -491,49 +483,24 @@ k_sssp_nf_round(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ 
                                                        ctl + ((r + 1) % 3));
         return;
     }
-    if (hsweep) {
-        // the marked vertices' heavy entries [rend[v], rowptr[v + 1]) form the
-        // next frontier; the 32 vertices of a bitmap word sit in 32
-        // consecutive lanes, and the word's first lane clears it after the
-        // wave has read it (no round marks while this one runs)
-        for (int64_t pbase = (int64_t)blockIdx.x * kNfBlock * kNfPer; pbase < m;
-             pbase += (int64_t)gridDim.x * kNfBlock * kNfPer) {
-            int32_t cv[kNfPer], crs[kNfPer], cd[kNfPer];
-#pragma unroll
-            for (int k = 0; k < kNfPer; ++k) {
-                const int64_t v = pbase + (int64_t)k * kNfBlock + threadIdx.x;
-                cv[k] = (int32_t)v;
-                crs[k] = 0;
-                cd[k] = 0;
-                if (v < m) {
-                    const uint32_t word = hb[v >> 5];
-                    if ((word >> (v & 31)) & 1u) {
-                        const int32_t le = rend[v];
-                        crs[k] = le;
-                        cd[k] = rowptr[v + 1] - le;
-                    }
-                    if ((v & 31) == 0 && word) hb[v >> 5] = 0;
-                }
-            }
-            produce<kNfBlock, kNfPer, TILE, true>(cv, crs, cd, nxt, ctl + ((r + 1) % 3), lds.scan,
-                                                  &lds.base);
-        }
-        return;
-    }
-    // split: move the pending vertices below the new threshold into the frontier
-    uint64_t left = 0, moved = 0;
+    // split: move the pending vertices below the new threshold into the
+    // frontier, and in light / heavy mode add the heavy entries of the
+    // vertices marked since the last split (a moved vertex brings its light
+    // and its heavy entries).  The 32 vertices of a mark word sit in 32
+    // consecutive lanes, and the word's first lane clears it after the wave
+    // has read it (nothing marks during a split).
+    uint64_t left = 0;
     uint32_t min_left = ~0u;  // smallest distance left pending (bits)
     for (int64_t pbase = (int64_t)blockIdx.x * kNfBlock * kNfPer; pbase < m;
          pbase += (int64_t)gridDim.x * kNfBlock * kNfPer) {
-        int32_t cv[kNfPer], crs[kNfPer], cd[kNfPer];
-        bool take[kNfPer], drop[kNfPer];
+        int32_t cv[kNfPer], crs[kNfPer], cd[kNfPer], hrs[kNfPer], hd[kNfPer];
+        bool take[kNfPer], drop[kNfPer], marked[kNfPer];
 #pragma unroll
         for (int k = 0; k < kNfPer; ++k) {
             const int64_t v = pbase + (int64_t)k * kNfBlock + threadIdx.x;
             cv[k] = (int32_t)v;
-            crs[k] = 0;
-            cd[k] = 0;
-            take[k] = drop[k] = false;
+            crs[k] = cd[k] = hrs[k] = hd[k] = 0;
+            take[k] = drop[k] = marked[k] = false;
             if (v < m && pend[v]) {
                 const int32_t dv = dist[v];
                 if (dv < tbits) drop[k] = true;  // lowered below T since: already expanded
@@ -543,6 +510,11 @@ k_sssp_nf_round(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ 
                     min_left = min(min_left, (uint32_t)dv);
                 }
             }
+            if (hb && v < m) {
+                const uint32_t word = hb[v >> 5];
+                marked[k] = (word >> (v & 31)) & 1u;
+                if ((v & 31) == 0 && word) hb[v >> 5] = 0;
+            }
         }
 #pragma unroll
         for (int k = 0; k < kNfPer; ++k) {
@@ -550,34 +522,33 @@ k_sssp_nf_round(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ 
             if (take[k]) {  // the sweep meets each vertex once: no claim needed
                 crs[k] = rowptr[cv[k]];
                 cd[k] = rend[cv[k]] - crs[k];
-                ++moved;
-                if (hb)
-                    __hip_atomic_fetch_or(hb + (cv[k] >> 5), 1u << (cv[k] & 31), __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (take[k] || marked[k]) {
+                hrs[k] = rend[cv[k]];
+                hd[k] = hb ? rowptr[cv[k] + 1] - hrs[k] : 0;
             }
         }
         produce<kNfBlock, kNfPer, TILE, true>(cv, crs, cd, nxt, ctl + ((r + 1) % 3), lds.scan,
                                               &lds.base);
+        if (hb)
+            produce<kNfBlock, kNfPer, TILE, true>(cv, hrs, hd, nxt, ctl + ((r + 1) % 3),
+                                                  lds.scan, &lds.base);
     }
     const uint64_t ws = dev::wave_sum_u64(left);
-    const uint64_t wv = dev::wave_sum_u64(moved);
     const uint32_t wm = dev::wave_min_u32(min_left);
     if (dev::lane_id() == 0) {
         s_part[threadIdx.x / 64] = ws;
-        s_moved[threadIdx.x / 64] = wv;
         s_min[threadIdx.x / 64] = wm;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint64_t t = 0, mv = 0;
+        uint64_t t = 0;
         uint32_t mn = ~0u;
         for (int q = 0; q < kNfBlock / 64; ++q) {
             t += s_part[q];
-            mv += s_moved[q];
             mn = min(mn, s_min[q]);
         }
         nf_report(ctl, r, t, mn);
-        nf_report_moved(ctl, r, mv);
     }
 }
 
@@ -714,11 +685,9 @@ void sssp_near_far(Ctx *c, const Graph *g, int32_t source, float *dist_out, floa
                                   c->stream));
         sync(c);
         c->stats.host_syncs++;
-        // finished when the last round was a split that moved nothing and
-        // left nothing pending
-        if (cnt_entries(h[r % 3]) == 0 && h[14 + (r & 1)] == 1 && nf_left(h, r + 2) == 0 &&
-            nf_moved(h, r + 2) == 0)
-            break;
+        // finished when the last round was a split that left nothing pending
+        // and found nothing below the new threshold
+        if (cnt_entries(h[r % 3]) == 0 && h[14 + (r & 1)] != 0 && nf_left(h, r + 2) == 0) break;
         APSPIE_CHECK(r <= 8 * g->m + 1024, APSPIE_ERR_DEVICE, "SSSP did not converge");
         chunk = grow;
         grow = std::min(grow * 2, 256);

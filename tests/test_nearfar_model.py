@@ -227,14 +227,13 @@ def test_batch_rounds_bounded_with_large_weights(lo, hi):
 
 
 def near_far_light_heavy(m, rowptr, col, w, src, delta, max_rounds=400_000):
-    """The light / heavy schedule of sssp.hip k_sssp_nf_round: a vertex that
-    enters the frontier (claimed, or moved by a split) expands only its
-    light entries (weight below delta) and is marked; a round that finds the
-    frontier empty after an expansion, or after a split that moved vertices,
-    is a heavy sweep: the marked vertices' heavy entries form the next
-    frontier and the marks clear; other empty rounds split as in near_far.
-    The source expands its whole row.  (distances, rounds, expanded edges);
-    rounds = -1 when max_rounds ran out."""
+    """The light / heavy schedule of sssp.hip k_sssp_nf_round: a vertex
+    claimed for the frontier expands only its light entries (weight below
+    delta) and is marked; a split adds the marked vertices' heavy entries to
+    the next frontier and clears the marks, while it moves the pending
+    vertices below the new threshold, each with its light and its heavy
+    entries.  The source expands its whole row.  (distances, rounds,
+    expanded edges); rounds = -1 when max_rounds ran out."""
     w = np.asarray(w, np.float32)
     dist = np.full(m, F32_INF, np.float32)
     dist[src] = 0
@@ -248,7 +247,7 @@ def near_far_light_heavy(m, rowptr, col, w, src, delta, max_rounds=400_000):
     T = step = np.float32(delta)
     frontier = [(src, list(range(rowptr[src], rowptr[src + 1])))]
     frontier = [f for f in frontier if f[1]]
-    prev, left_prev, min_prev, moved_prev = "expand", 0, None, 0
+    split_last, left_prev, min_prev = False, 0, None
     rounds = expanded = 0
     while rounds < max_rounds:
         rounds += 1
@@ -269,33 +268,27 @@ def near_far_light_heavy(m, rowptr, col, w, src, delta, max_rounds=400_000):
                             marked[v] = True
                             if part(v, True):
                                 nxt.append((v, part(v, True)))
-            frontier, prev = nxt, "expand"
+            frontier, split_last = nxt, False
             continue
-        if prev == "expand" or (prev == "split" and moved_prev > 0):
-            frontier = [(v, part(v, False)) for v in np.nonzero(marked)[0] if part(v, False)]
-            marked[:] = False
-            prev = "sweep"
-            continue
-        split_last = prev == "split"
         if split_last and left_prev == 0:
+            assert not marked.any()  # every mark was swept by the split
             return dist, rounds, expanded
         tn, step = next_threshold(T, step, split_last and min_prev is not None, min_prev)
-        left, mn, moved, nxt = 0, None, 0, []
+        nxt = [(v, part(v, False)) for v in np.nonzero(marked)[0] if part(v, False)]
+        marked[:] = False
+        left, mn = 0, None
         for v in np.nonzero(pend)[0]:
             d = dist[v]
             if d < T:
                 pend[v] = False
             elif d < tn:
                 pend[v] = False
-                marked[v] = True
-                moved += 1
-                if part(v, True):
-                    nxt.append((v, part(v, True)))
+                nxt += [(v, es) for es in (part(v, True), part(v, False)) if es]
             else:
                 left += 1
                 mn = d if mn is None else min(mn, d)
-        left_prev, min_prev, moved_prev = left, mn, moved
-        T, frontier, prev = tn, nxt, "split"
+        left_prev, min_prev = left, mn
+        T, frontier, split_last = tn, nxt, True
     return dist, -1, expanded
 
 
@@ -305,8 +298,8 @@ def near_far_light_heavy(m, rowptr, col, w, src, delta, max_rounds=400_000):
 def test_light_heavy_fixed_point_equals_oracle(key, lo, hi, delta):
     """Deferring the heavy entries changes the order of relaxations only:
     the fixed point is the oracle's Dijkstra, also when every entry is
-    heavy (weights far above the width: the marks of moved vertices
-    without light entries must still reach a heavy sweep)."""
+    heavy (weights far above the width: a moved vertex brings its heavy
+    entries itself, and the rounds end only after a split swept the marks)."""
     g = T.rmat(8) if key == "rmat8" else T.lattice(8)
     w = weights(g, lo, hi, seed=lo % 89)
     for src in T.sources(g, 3):
