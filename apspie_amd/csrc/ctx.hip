@@ -19,8 +19,8 @@ static const char *kOptionNames[] = {
                          // -1 = chosen per graph from its weights and mean row length)
     "sssp_pull_div",     // pull an SSSP round when frontier edges * this > nnz (0 = never;
                          // symmetric weights only)
-    "sssp_light_heavy",  // near-far: heavy entries (value >= width) once per emptied
-                         // frontier instead of with every improvement (1 = on)
+    "sssp_light_heavy",  // near-far: heavy entries (value >= width) once per bucket, at
+                         // the next split, instead of with every improvement (1 = on)
     "spmspv_path",       // 0 auto, 1 sort, 2 atomic
     "spmspv_sort_max",   // auto: use the sort path up to this many expanded edges
     "batch_blocks_per_cu",

@@ -446,7 +446,6 @@ k_sssp_nf_round(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ 
     const uint64_t cc = ctl[r % 3];
     const int32_t F = (int32_t)cnt_entries(cc), E = (int32_t)cnt_edges(cc);
     const int32_t tbits = (int32_t)ctl[12 + (r & 1)];
-    // what round r - 1 was: 0 an expansion, 1 a split, 2 a heavy sweep
     const bool split_last = ctl[14 + (r & 1)] != 0;
     const uint64_t left_before = nf_left(ctl, r + 2);  // left by the previous round's split
     const bool split = F == 0;
@@ -634,11 +633,11 @@ void sssp_near_far(Ctx *c, const Graph *g, int32_t source, float *dist_out, floa
     uint32_t *fb = (uint32_t *)scratch(c, "sssp_fb", sizeof(uint32_t) * 3 * words);
     uint8_t *pend = (uint8_t *)scratch(c, "sssp_pend", (size_t)g->m + 4);
     // light / heavy mode (option sssp_light_heavy, on by default) when the
-    // weights fall on both sides of delta: a vertex that enters the
-    // frontier expands its light entries (value below delta), and its heavy
-    // ones once per emptied frontier, with its distance by then -- a heavy
-    // relaxation lands at or above the threshold whenever the step is delta,
-    // so re-expanding it with every improvement only repeats far marks
+    // weights fall on both sides of delta: a claimed vertex expands its light
+    // entries (value below delta), and its heavy ones once per bucket, at
+    // the next split, with its distance by then -- a heavy relaxation lands
+    // at or above the threshold whenever the step is delta, so re-expanding
+    // it with every improvement only repeats far marks
     float lo = 0.0f, hi = 0.0f;
     const bool lh = opt(c, "sssp_light_heavy", 1) != 0 &&
                     graph_weight_range(c, const_cast<Graph *>(g), &lo, &hi) && lo < delta &&
