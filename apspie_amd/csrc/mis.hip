@@ -12,8 +12,9 @@
 // reference's threshold schedule (minimum -= delta) only changes the number
 // of rounds, not the set, and is not used.
 //
-// Per round (no host round trip): k_expand marks candidates that lose to an
-// undecided neighbour (pull over the candidate's row, no atomics), k_build
+// Per round (no host round trip): k_mis_lose (first round: pulled over the
+// candidate rows) or k_expand (later rounds: pushed over the candidate
+// edges) marks candidates that lose to an undecided neighbour, k_build
 // admits the winners and lists them, k_expand pushes from the winners to
 // remove their undecided neighbours, and k_build compacts the survivors into
 // the next candidate frontier.
@@ -39,18 +40,6 @@ __device__ __forceinline__ uint64_t mis_key(const uint32_t *prio, int32_t v) {
 // its low word is 0x7fffffff - v and v < 2^31 - 1), so one 8-byte load per
 // edge answers "undecided with a larger key"
 
-// candidate u loses if some undecided neighbour has a larger key
-struct LoseOp {
-    const uint64_t *live;
-    int32_t *lose;
-    int32_t round;
-    __device__ __forceinline__ bool edge(int32_t u, int32_t, int32_t v) const {
-        if (v == u || lose[u] == round) return false;
-        if (live[v] > live[u]) lose[u] = round;
-        return false;
-    }
-};
-
 // a candidate that did not lose joins (the join list is built from these)
 struct JoinOp {
     const int32_t *lose;
@@ -75,6 +64,93 @@ struct RemoveOp {
             state[v] = 0;
             live[v] = 0;
         }
+        return false;
+    }
+};
+
+// The lose test pulled over each candidate's row, stopping at the first
+// undecided neighbour with a larger key (with random keys, after about two
+// entries for most candidates), where pushing every candidate edge read all
+// of them: a lane per candidate takes the first kLoseQ entries with their
+// loads in flight; rows still open after them go to an LDS list that the
+// block's waves then finish a row at a time, 256 entries per step.
+constexpr int kLoseQ = 8;
+constexpr int kLoseStep = 256;
+
+__global__ __launch_bounds__(kBlock) void k_mis_lose(const int32_t *__restrict__ rowptr,
+                                                     const int32_t *__restrict__ col,
+                                                     const uint64_t *__restrict__ live,
+                                                     int32_t *lose, int32_t round,
+                                                     FrontierBufs cur, const uint64_t *cnt_cur,
+                                                     uint64_t *cnt_clear, uint64_t *stats) {
+    __shared__ int32_t s_u[kBlock], s_e[kBlock], s_end[kBlock];
+    __shared__ int32_t s_n;
+    const uint64_t cc = *cnt_cur;
+    const int32_t F = (int32_t)cnt_entries(cc);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *cnt_clear = 0;
+        if (F > 0) {
+            stats[0] += (uint64_t)cnt_edges(cc);
+            stats[1] += (uint64_t)F;
+            stats[2] += 1;
+        }
+    }
+    const int lane = dev::lane_id(), wave = threadIdx.x / 64;
+    for (int64_t base = (int64_t)blockIdx.x * kBlock; base < F;
+         base += (int64_t)gridDim.x * kBlock) {
+        if (threadIdx.x == 0) s_n = 0;
+        __syncthreads();
+        const int64_t j = base + threadIdx.x;
+        if (j < F) {
+            const int32_t u = cur.v[j], rs = cur.start[j], re = rowptr[u + 1];
+            const uint64_t ku = live[u];
+            int32_t c[kLoseQ];
+#pragma unroll
+            for (int t = 0; t < kLoseQ; ++t) c[t] = rs + t < re ? col[rs + t] : -1;
+            bool lost = false;
+#pragma unroll
+            for (int t = 0; t < kLoseQ; ++t)
+                if (c[t] >= 0 && live[c[t]] > ku) lost = true;
+            if (lost) lose[u] = round;
+            else if (re - rs > kLoseQ) {
+                const int32_t q = atomicAdd(&s_n, 1);
+                s_u[q] = u;
+                s_e[q] = rs + kLoseQ;
+                s_end[q] = re;
+            }
+        }
+        __syncthreads();
+        const int32_t n = s_n;
+        for (int32_t q = wave; q < n; q += kBlock / 64) {
+            const int32_t u = s_u[q], re = s_end[q];
+            const uint64_t ku = live[u];
+            bool lost = false;
+            for (int32_t e0 = s_e[q]; e0 < re && !lost; e0 += kLoseStep) {
+                bool mine = false;
+#pragma unroll
+                for (int t = 0; t < kLoseStep / 64; ++t) {
+                    const int32_t e = e0 + t * 64 + lane;
+                    if (e < re && live[col[e]] > ku) mine = true;
+                }
+                lost = __ballot(mine) != 0;
+            }
+            if (lost && lane == 0) lose[u] = round;
+        }
+        __syncthreads();  // s_n and the list are rewritten by the next pass
+    }
+}
+
+// candidate u loses if some undecided neighbour has a larger key (pushed
+// over the candidate edges, load-balanced by tiles: the later rounds' few
+// candidates are the ones with long rows of mostly decided neighbours,
+// which a pulled row would walk to its end)
+struct LoseOp {
+    const uint64_t *live;
+    int32_t *lose;
+    int32_t round;
+    __device__ __forceinline__ bool edge(int32_t u, int32_t, int32_t v) const {
+        if (v == u || lose[u] == round) return false;
+        if (live[v] > live[u]) lose[u] = round;
         return false;
     }
 };
@@ -123,10 +199,18 @@ void mis_run(Ctx *c, const Graph *g, const uint32_t *prio, int32_t *state) {
             const int cur = round & 1;
             uint64_t *cnt = fs.ctl + (round % 3), *nxt = fs.ctl + ((round + 1) % 3);
             kt.begin();
-            // lose marks; this launch also clears the join list's counter
-            k_expand<kBlock, kTile, true, false, false><<<grid, kBlock, 0, c->stream>>>(
-                LoseOp{live, lose, round}, g->rowptr, g->col, (int32_t)g->nnz, fs.f[cur],
-                cnt, fs.f[cur ^ 1], nullptr, jcnt, fs.ctl + 4);
+            // lose marks, pulled in the first round (every vertex a candidate,
+            // random keys: most rows stop after a few entries; R-MAT-20: 173
+            // -> 53 us) and pushed after it (84 / 47 / 36 us pulled against
+            // 50 / 17 / 15 pushed in rounds 2-4); this launch also clears the
+            // join list's counter
+            if (round == 0)
+                k_mis_lose<<<grid, kBlock, 0, c->stream>>>(g->rowptr, g->col, live, lose, round,
+                                                          fs.f[cur], cnt, jcnt, fs.ctl + 4);
+            else
+                k_expand<kBlock, kTile, true, false, false><<<grid, kBlock, 0, c->stream>>>(
+                    LoseOp{live, lose, round}, g->rowptr, g->col, (int32_t)g->nnz, fs.f[cur],
+                    cnt, fs.f[cur ^ 1], nullptr, jcnt, fs.ctl + 4);
             kt.end();
             k_build<kBlock, 8, kTile, true><<<grid, kBlock, 0, c->stream>>>(
                 JoinOp{lose, state, live, round}, g->rowptr, fs.f[cur].v, 0, cnt, js.f[0], jcnt);
