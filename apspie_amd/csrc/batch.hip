@@ -970,9 +970,12 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
                               hipMemcpyHostToDevice, c->stream));
     const int grid = fill_grid(c, kBlock, (int)opt(c, "batch_blocks_per_cu", 4));
     // near-far rounds for SSSP (threshold step batch_sssp_delta, 0 = off,
-    // -1 = chosen per graph)
+    // -1 = chosen per graph; passes of up to 128 sources take twice that
+    // width: fewer rounds, whose fixed cost weighs more on a narrow row --
+    // 2048^2 lattice, 32 / 64 / 128 sources: 0.876 -> 0.829 / 0.705 -> 0.682
+    // / 0.625 -> 0.616 ms per query; 512 sources 0.596 -> 0.603)
     double bdelta = SSSP ? (double)opt(c, "batch_sssp_delta", -1) : 0;
-    if (bdelta < 0) bdelta = sssp_batch_auto_delta(c, const_cast<Graph *>(g));
+    if (bdelta < 0) bdelta = sssp_batch_auto_delta(c, const_cast<Graph *>(g)) * (W <= 4 ? 2 : 1);
     const float delta = (float)bdelta;
     const bool near_far = SSSP && delta > 0;
     uint32_t *pend =
