@@ -353,6 +353,28 @@ def test_server_tight_budget_sizes_batches():
         srv.drain()
 
 
+def test_server_budget_not_granule_multiple():
+    """A budget that is not a multiple of 256 bytes still gives every worker
+    an arena (rounded down) and every result equals the oracle's, over a
+    stream that switches graphs and kinds so that batches reuse the arenas'
+    ranges."""
+    g1, g2 = T.lattice(64), T.rmat(11)
+    with S.Server(slots=2, max_batch=64, hbm_budget=(256 << 20) + 100) as srv:
+        ids = [srv.add_graph(g.rowptr32, g.col, g.w) for g in (g1, g2)]
+        tickets = []
+        for rep in range(3):
+            for gi, g in zip(ids, (g1, g2)):
+                for s in T.sources(g, 20, seed=rep):
+                    tickets.append((g, s, A.Q_BFS, srv.submit(gi, A.Q_BFS, s)))
+                for s in T.sources(g, 6, seed=rep + 7):
+                    tickets.append((g, s, A.Q_SSSP, srv.submit(gi, A.Q_SSSP, s)))
+        for g, s, kind, t in tickets:
+            res, _ = srv.wait(t, kind, g.m)
+            want = O.bfs(g.m, g.rowptr, g.col, s)[0] if kind == A.Q_BFS else oracle_sssp(g, s)
+            assert np.array_equal(res, want), (kind, s)
+        srv.drain()
+
+
 def test_server_zero_copy_views():
     """wait(copy=False) hands out read-only views of pinned result buffers:
     equal to the copying wait, intact while later requests run, returned to
