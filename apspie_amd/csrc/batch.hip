@@ -532,8 +532,14 @@ __device__ __forceinline__ void batch_sssp_pull(PullQueue &q, const int32_t *__r
                                                 FrontierBufs nxt, uint64_t *cnt_next) {
     constexpr int B = 32 * W;
     constexpr int VPB = kBlock / W;
-    constexpr int KP = 2;  // vertices per group and pass
-    constexpr int Q = 4;   // row entries loaded at once
+    // vertices per group and pass, measured on the 2048^2 lattice (ms per
+    // query): one for rows of one word (32 sources: 0.826 with two, 0.812
+    // with one), two for rows of two (64 sources: 0.685, 0.697 with four),
+    // four from four words on (128 / 256 / 512 sources: 0.616 / 0.598 /
+    // 0.597 with two, 0.611 / 0.591 / 0.579 with four)
+    constexpr int KP = W == 1 ? 1 : W == 2 ? 2 : 4;
+    static_assert(KP * kBlock <= kQueue, "one pass's claims fit the LDS queue");
+    constexpr int Q = 4;  // row entries loaded at once
     if (threadIdx.x == 0) q.n = 0;
     __syncthreads();
     const int g = threadIdx.x / W, w = threadIdx.x % W;
