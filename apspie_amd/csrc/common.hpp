@@ -74,14 +74,19 @@ struct Graph {
     int32_t *heavy_v = nullptr, *heavy_start = nullptr, *heavy_off = nullptr,
             *heavy_tfirst = nullptr;
     uint64_t heavy_cnt = 0;
-    // near-far SSSP with light / heavy entries: a copy of col / val whose
-    // rows hold their light entries (value below lf_delta) first, in CSR
-    // order, then the heavy ones, and the end of each row's light part; one
-    // block of device memory owned by the graph, rebuilt for another width
-    void *lf_mem = nullptr;
-    int32_t *lf_col = nullptr, *lf_end = nullptr;
-    float *lf_val = nullptr;
-    float lf_delta = -1.0f;
+    // near-far SSSP with light / heavy entries, per width delta: a copy of
+    // col / val whose rows hold their light entries (value below delta)
+    // first, in CSR order, then the heavy ones, and the end of each row's
+    // light part; one block of device memory each, owned by the graph and
+    // kept until it is destroyed (a call on another context may still be
+    // launching rounds over the copy of another width)
+    struct LightFirst {
+        float delta;
+        void *mem;
+        int32_t *col, *end;
+        float *val;
+    };
+    std::vector<LightFirst> lf;
     std::mutex derived_mu;      // guards lazily built derived data
 };
 

@@ -602,7 +602,7 @@ int apspie_graph_destroy(apspie_graph *g) {
     if (g->iso_bits) (void)hipFree(g->iso_bits);
     if (g->comp) (void)hipFree(g->comp);
     if (g->heavy_mem) (void)hipFree(g->heavy_mem);
-    if (g->lf_mem) (void)hipFree(g->lf_mem);
+    for (const auto &x : g->lf) (void)hipFree(x.mem);
     delete g;
     return APSPIE_OK;
 }

@@ -589,17 +589,14 @@ __global__ __launch_bounds__(kLfBlock) void k_light_first(const int32_t *__restr
     }
 }
 
-// The light-first copy of a graph's rows for width delta, built once per
-// width and kept with the graph.
-void graph_light_first(Ctx *c, Graph *g, float delta) {
+// The light-first copy of a graph's rows for width delta, built on first
+// use and kept with the graph until it is destroyed: none is freed while the
+// graph lives, since a call on another context may still be launching
+// rounds over the copy it took (of this width or another).
+Graph::LightFirst graph_light_first(Ctx *c, Graph *g, float delta) {
     std::lock_guard<std::mutex> lock(g->derived_mu);
-    if (g->lf_mem && g->lf_delta == delta) return;
-    if (g->lf_mem) {  // built for another width: calls that read it have returned
-        APSPIE_HIP(hipDeviceSynchronize());
-        APSPIE_HIP(hipFree(g->lf_mem));
-        g->lf_mem = nullptr;
-        g->lf_delta = -1.0f;
-    }
+    for (const Graph::LightFirst &x : g->lf)
+        if (x.delta == delta) return x;
     const int64_t E = std::max<int64_t>(g->nnz, 1);
     const size_t ecol = ((size_t)E * 4 + 255) / 256 * 256;
     const size_t bytes = 2 * ecol + sizeof(int32_t) * ((size_t)g->m + 1);
@@ -609,20 +606,22 @@ void graph_light_first(Ctx *c, Graph *g, float delta) {
     int32_t *lcol = (int32_t *)mem;
     float *lval = (float *)((char *)mem + ecol);
     int32_t *lend = (int32_t *)((char *)mem + 2 * ecol);
-    if (g->m > 0) {
-        // a wave per row and every row its own wave: the two passes over a
-        // row are a few dependent loads, so the grid covers the rows at once
-        const int grid = (int)std::min<int64_t>(ceil_div(g->m, kLfBlock / 64), 1 << 24);
-        k_light_first<<<grid, kLfBlock, 0, c->stream>>>(g->rowptr, g->col, g->val, g->m, delta,
-                                                        lcol, lval, lend);
-        APSPIE_LAUNCH_CHECK();
+    try {
+        if (g->m > 0) {
+            // a wave per row and every row its own wave: the two passes over
+            // a row are a few dependent loads, so the grid covers the rows
+            const int grid = (int)std::min<int64_t>(ceil_div(g->m, kLfBlock / 64), 1 << 24);
+            k_light_first<<<grid, kLfBlock, 0, c->stream>>>(g->rowptr, g->col, g->val, g->m,
+                                                            delta, lcol, lval, lend);
+            APSPIE_LAUNCH_CHECK();
+        }
+        sync(c);
+    } catch (...) {
+        (void)hipFree(mem);
+        throw;
     }
-    sync(c);
-    g->lf_mem = mem;
-    g->lf_col = lcol;
-    g->lf_val = lval;
-    g->lf_end = lend;
-    g->lf_delta = delta;
+    g->lf.push_back(Graph::LightFirst{delta, mem, lcol, lend, lval});
+    return g->lf.back();
 }
 
 template <int TILE>
@@ -646,10 +645,10 @@ void sssp_near_far(Ctx *c, const Graph *g, int32_t source, float *dist_out, floa
     const float *val = g->val;
     uint32_t *hb = nullptr;
     if (lh) {
-        graph_light_first(c, const_cast<Graph *>(g), delta);
-        col = g->lf_col;
-        val = g->lf_val;
-        rend = g->lf_end;
+        const Graph::LightFirst lf = graph_light_first(c, const_cast<Graph *>(g), delta);
+        col = lf.col;
+        val = lf.val;
+        rend = lf.end;
         hb = (uint32_t *)scratch(c, "sssp_hb", sizeof(uint32_t) * words);
         fill_i32(c, (int32_t *)hb, 0, words);
     }

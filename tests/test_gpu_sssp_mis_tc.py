@@ -170,7 +170,7 @@ def test_sssp_light_heavy_vs_oracle(ctx, key, wr, lh):
     """Near-far rounds with heavy entries deferred to once per emptied
     frontier (sssp_light_heavy = 1) and without: the oracle's distances for
     widths that split the weights (some rows all light, some all heavy),
-    with two widths in turn on one graph (its light-first copy rebuilt), on
+    with three widths in turn on one graph (a light-first copy each), on
     a directed graph and with weights far above the width."""
     rng = np.random.default_rng(7)
     if key == "directed":
@@ -199,6 +199,48 @@ def test_sssp_light_heavy_vs_oracle(ctx, key, wr, lh):
         ctx.set_option("sssp_light_heavy", 1)
         ctx.set_option("sssp_delta", -1)
     G.close()
+
+
+def test_sssp_widths_on_two_contexts_share_a_graph():
+    """Two contexts on two threads run near-far SSSP with light / heavy
+    rounds at different widths on one graph at the same time: each width
+    has its own light-first copy, kept while the graph lives, so neither
+    call's rounds read a copy the other call replaced; every result equals
+    the oracle's."""
+    import threading
+
+    g = T.rmat(12)
+    wt = np.random.default_rng(11).integers(1, 256, len(g.col)).astype(np.float32)
+    deg = np.diff(g.rowptr)
+    srcs = [int(s) for s in np.nonzero(deg > 0)[0][:: max(1, g.m // 12)]]
+    want = {s: to_ref_sentinel(O.sssp(g.m, g.rowptr, g.col, wt, s)) for s in srcs}
+    c1, c2 = A.Context(0), A.Context(0)
+    try:
+        G = c1.upload(g.rowptr32, g.col, wt)
+        bad, errors = [], []
+
+        def run(cx, delta):
+            try:
+                cx.set_option("sssp_light_heavy", 1)
+                cx.set_option("sssp_delta", delta)
+                for _ in range(3):
+                    for s in srcs:
+                        if not np.array_equal(cx.sssp(G, s), want[s]):
+                            bad.append((delta, s))
+            except Exception as e:  # surfaced by the asserts below
+                errors.append(repr(e))
+
+        th = [threading.Thread(target=run, args=a) for a in ((c1, 16), (c2, 64))]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        assert not errors, errors
+        assert not bad, bad
+        G.close()
+    finally:
+        c1.close()
+        c2.close()
 
 
 def test_sssp_near_far_edge_cases(ctx):
