@@ -711,7 +711,6 @@ __global__ __launch_bounds__(kBlock) void k_batch_sssp_nf(
     const int g = threadIdx.x / W, w = threadIdx.x % W;
     const int lane = dev::lane_id();
     const uint64_t gmask = W == 64 ? ~0ull : (((1ull << W) - 1) << ((lane / W) * W));
-    const int leader = (lane / W) * W;  // the group's first lane
     const int32_t round = *op.lvl;
     uint64_t left = 0;
     uint32_t min_left = ~0u;  // smallest distance left pending (bits)

@@ -48,15 +48,6 @@ __device__ __forceinline__ float sr_add(int sr, float a, float b) {
     }
 }
 
-__device__ __forceinline__ float sr_identity(int sr) {
-    switch (sr) {
-    case APSPIE_SR_PLUS_TIMES: return 0.0f;
-    case APSPIE_SR_BOOL: return 0.0f;
-    case APSPIE_SR_MIN_PLUS: return INFINITY;
-    default: return -INFINITY;
-    }
-}
-
 // stage 1: gather row start and length of every frontier entry
 __global__ void k_gather(const int32_t *rowptr, const int32_t *x_ind, int64_t nx,
                          int32_t *start, int32_t *deg) {
