@@ -1076,7 +1076,7 @@ void batch_pass(Ctx *c, const Graph *g, const int32_t *src_host, int32_t nsrc, i
     int l = 0;
     uint64_t *h = (uint64_t *)c->pinned;
     KernelTimer kt(c);
-    for (int chunk = 0;; ++chunk) {
+    for (;;) {
         kt.begin();
         if (use_graph && exec) {
             APSPIE_HIP(hipGraphLaunch(exec, c->stream));
